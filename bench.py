@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""Benchmark of the FE assembly hot path on one MI355X.
+
+Workload (default): plain FP64 linear-elasticity stiffness on an H8 block of
+100 x 100 x 100 = 1,000,000 elements (1,030,301 nodes, 3 DOF/node, 24 x 24
+element matrices, 2x2x2 Gauss rule, E = 200e3, nu = 0.3, interior nodes
+perturbed by seeded U[-1,1) * 0.15 h).  Mesh, DOFs and the CSC pattern are
+resident in HBM before the timed region; one step is one numeric assembly
+(element integration + global summation into the CSC values).
+
+Prints one JSON line (the driver's contract); progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "finetools.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "assembled elements/sec + nnz/sec (H8 elasticity FP64); p50/p99 latency served"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def iso_D(E, nu):
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    D = np.zeros((6, 6))
+    D[:3, :3] = lam
+    for i in range(3):
+        D[i, i] = lam + 2 * mu
+        D[3 + i, 3 + i] = mu
+    return D
+
+
+def algorithmic_bytes_numeric(npe, ne, nn, nnz):
+    """Compulsory HBM bytes of one numeric assembly: int32 connectivity and
+    FP64 coordinates read once, FP64 values written once."""
+    return 4 * npe * ne + 24 * nn + 8 * nnz
+
+
+def cpu_baseline(n_cells, seed, D, min_seconds=10.0, max_runs=5):
+    """Serial CPU restatement of the reference's element loop + sparse() timed
+    on a bounded sample of the same workload (elements/s)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    conn, xyz = O.block_mesh(O.H8, (n_cells,) * 3, L=1.0, amp_frac=0.15, seed=seed)
+    ne = conn.shape[0]
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < max_runs and (len(times) < 3 or time.perf_counter() - t_start < min_seconds):
+        t0 = time.perf_counter()
+        O.assemble_csc(O.H8, O.LIN_ELASTIC, conn, xyz, 3, D.ravel())
+        times.append(time.perf_counter() - t0)
+    t = statistics.median(times)
+    return {"value": ne / t, "unit": "elements/s", "cores": 1, "kind": "port",
+            "sample": "H8 %d^3 = %d elements linear elasticity FP64, element loop + sparse(), median of %d runs "
+                      "(%.2f s each)" % (n_cells, ne, len(times), t)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cells", type=int, default=100, help="cells per direction of the H8 block")
+    ap.add_argument("--cpu-cells", type=int, default=50, help="cells per direction of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X; "
+              "multi-GPU sharding is a design note, not a run (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import feasm
+    from feasm import core
+
+    n = a.cells
+    D = iso_D(200e3, 0.3)
+    ctx = feasm.Context(0)
+    conn, xyz = core.gen_h8block(1.0, 1.0, 1.0, n, n, n)
+    xyz = core.gen_perturb(xyz, [0, 0, 0], [1, 1, 1], 0.15 / n, 2)
+    ne, nn = conn.shape[0], xyz.shape[0]
+    log("mesh: %d elements, %d nodes" % (ne, nn))
+    mesh = core.DeviceMesh(ctx, feasm.H8, conn, xyz)
+    del conn
+    mesh.set_dofs(3)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    mesh.build_pattern()
+    ctx.synchronize()
+    t_sym = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    M = mesh.bilform(feasm.LIN_ELASTIC, feasm.RULE_GAUSS, 2, D)
+    ctx.synchronize()
+    t_first_numeric = time.perf_counter() - t0
+    nnz = M.nnz
+    log("pattern: nnz %d, symbolic %.1f ms, first numeric %.1f ms" % (nnz, 1e3 * t_sym, 1e3 * t_first_numeric))
+
+    for _ in range(a.warmup):
+        mesh.bilform_into(M, feasm.LIN_ELASTIC, feasm.RULE_GAUSS, 2, D)
+    ctx.synchronize()
+    ctx.set_timing(max(a.steps, 1))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        mesh.bilform_into(M, feasm.LIN_ELASTIC, feasm.RULE_GAUSS, 2, D)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    tk = ctx.timings()
+    ctx.set_timing(0)
+    ms_step = 1e3 * (t1 - t0) / a.steps
+    el_ms, ga_ms = float(np.mean(tk[:, 0])), float(np.mean(tk[:, 1]))
+    kern_ms = el_ms + ga_ms
+    log("step %.3f ms; element kernel %.3f ms, gather kernel %.3f ms" % (ms_step, el_ms, ga_ms))
+
+    bytes_alg = algorithmic_bytes_numeric(8, ne, nn, nnz)
+    achieved = bytes_alg / (kern_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": ne / (ms_step * 1e-3),
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded perturbed H8 block)",
+        "config": {"workload": "H8 %dx%dx%d linear elasticity FP64, 3 DOF/node, 2x2x2 Gauss, E=200e3 nu=0.3, "
+                               "+-0.15h perturbation; numeric re-assembly into the resident CSC pattern"
+                               % (n, n, n),
+                   "elements": ne, "nodes": nn, "nnz": nnz, "parallelism": "single GPU"},
+        "nnz_per_s": nnz / (ms_step * 1e-3),
+        "first_assembly_ms": 1e3 * (t_sym + t_first_numeric),
+        "symbolic_ms": 1e3 * t_sym,
+        "kernels_ms": {"element": el_ms, "gather": ga_ms},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "element + gather (numeric phase)",
+                     "algorithmic_bytes": bytes_alg},
+        "served": None,
+    }
+    del M
+    mesh.close()
+    if not a.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(a.cpu_cells, 2, D)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,654 @@
+// C ABI of libfeasm.so (declared in include/feasm.h).
+#include <algorithm>
+#include <cmath>
+
+#include "common.hpp"
+
+using namespace feasm;
+
+namespace feasm {
+void gen_h8block(double L, double W, double H, int64_t nL, int64_t nW, int64_t nH, int64_t *conn, double *xyz);
+int64_t gen_t10block(double L, double W, double H, int64_t nL, int64_t nW, int64_t nH, double amp_v, double amp_m,
+                     uint64_t seed, int64_t *conn10, std::vector<double> &xyz10);
+void perturb_nodes(int64_t nnodes, double *xyz, const double lo[3], const double hi[3], double amp, uint64_t seed);
+}  // namespace feasm
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(feasm_ctx *ctx, int code, const char *what) {
+    g_last_error = what;
+    if (ctx) ctx->last_error = what;
+    return code;
+}
+
+#define API_TRY try {
+#define API_CATCH(ctxp)                                              \
+    }                                                                \
+    catch (const feasm::Error &e) {                                  \
+        return fail((ctxp), e.code, e.what());                       \
+    }                                                                \
+    catch (const std::bad_alloc &) {                                 \
+        return fail((ctxp), FEASM_ENOMEM, "host out of memory");     \
+    }                                                                \
+    catch (const std::exception &e) {                                \
+        return fail((ctxp), FEASM_EINVAL, e.what());                 \
+    }                                                                \
+    return FEASM_OK;
+
+struct Timer {
+    feasm_ctx *ctx;
+    explicit Timer(feasm_ctx *c) : ctx(c) {}
+    void mark(int i) {
+        if (ctx->timing) FEASM_HIP(hipEventRecord(ctx->ring_event(i), ctx->stream));
+    }
+    void done() {
+        if (ctx->timing) ctx->ring_advance();
+    }
+};
+
+FormParams make_form(int form, int ndn, const double *coeff, int ncoeff) {
+    FormParams fp;
+    std::memset(&fp, 0, sizeof(fp));
+    fp.form = form;
+    fp.ndn = ndn;
+    FEASM_REQUIRE(coeff != nullptr || ncoeff == 0, FEASM_EINVAL, "null coefficient array");
+    switch (form) {
+        case FEASM_FORM_DIFFUSION_ISO:
+            FEASM_REQUIRE(ncoeff == 1, FEASM_EINVAL, "isotropic diffusion takes one coefficient");
+            FEASM_REQUIRE(ndn == 1, FEASM_EINVAL, "diffusion needs 1 DOF per node");
+            break;
+        case FEASM_FORM_DIFFUSION_MAT:
+            FEASM_REQUIRE(ncoeff == 9, FEASM_EINVAL, "matrix diffusion takes a 3x3 coefficient");
+            FEASM_REQUIRE(ndn == 1, FEASM_EINVAL, "diffusion needs 1 DOF per node");
+            break;
+        case FEASM_FORM_LIN_ELASTIC:
+            FEASM_REQUIRE(ncoeff == 36, FEASM_EINVAL, "linear elasticity takes a 6x6 material matrix");
+            FEASM_REQUIRE(ndn == 3, FEASM_EINVAL, "Wrong dimensions");  // blmat! (DeforModelRedModule.jl:457)
+            break;
+        case FEASM_FORM_DOT:
+            FEASM_REQUIRE(ndn >= 1 && ndn <= 3, FEASM_EUNSUPPORTED, "dot form supports 1..3 DOFs per node");
+            FEASM_REQUIRE(ncoeff == ndn * ndn, FEASM_EINVAL, "dot form takes an ndn x ndn coefficient");
+            break;
+        default: throw Error(FEASM_EINVAL, "unknown form");
+    }
+    for (int i = 0; i < ncoeff; i++) fp.coef[i] = coeff[i];
+    return fp;
+}
+
+void check_rule(const feasm_mesh *m, int rule) {
+    if (m->etype == FEASM_H8)
+        FEASM_REQUIRE(rule == FEASM_RULE_GAUSS, FEASM_EUNSUPPORTED, "H8 elements take a Gauss rule");
+    else
+        FEASM_REQUIRE(rule == FEASM_RULE_TET, FEASM_EUNSUPPORTED, "T10 elements take a tetrahedral rule");
+}
+
+// the DOF checks of assemble! (src/AssemblyModule.jl:265-273) for a general
+// numbering, in the order the reference meets them: element by element,
+// columns then rows
+void check_dofs_like_assemble(const feasm_mesh *m, const std::vector<int64_t> &dofs1) {
+    int64_t n = m->nalldofs;
+    const int ndn = m->ndn;
+    std::vector<int32_t> conn(m->nelem * m->npe);
+    FEASM_HIP(hipMemcpy(conn.data(), m->conn->p, conn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    // rows == cols == the element's DOFs: the first column is checked, then
+    // every row of that column; later columns repeat rows already checked
+    for (int64_t e = 0; e < m->nelem; e++) {
+        const int64_t d0 = dofs1[(int64_t)conn[e * m->npe] * ndn];
+        if (d0 < 1) throw Error(FEASM_EDOF, "Column degree of freedom < 1");
+        if (d0 > n) throw Error(FEASM_EDOF, "Column degree of freedom > size");
+        for (int a = 0; a < m->npe; a++)
+            for (int k = 0; k < ndn; k++) {
+                const int64_t d = dofs1[(int64_t)conn[e * m->npe + a] * ndn + k];
+                if (d < 1) throw Error(FEASM_EDOF, "Row degree of freedom < 1");
+                if (d > n) throw Error(FEASM_EDOF, "Row degree of freedom > size");
+            }
+    }
+}
+
+void ensure_pattern(feasm_mesh *m) {
+    if (!m->pattern || m->pattern->ndn != m->ndn) m->pattern = build_pattern(m, m->ndn);
+}
+
+void *ensure_stage(feasm_mesh *m, int form, int precision) {
+    const size_t bytes = (size_t)m->nelem * npairs(m->npe) * stage_width(form) *
+                         (precision == FEASM_MIXED ? sizeof(float) : sizeof(double));
+    if (!m->stage || m->stage->bytes < bytes) {
+        m->stage.reset();
+        m->stage = dev_alloc(std::max<size_t>(bytes, 256));
+    }
+    return m->stage->p;
+}
+
+void numeric_into(feasm_mesh *m, const FormParams &fp, int rule, int order, int precision, double *nzval) {
+    feasm_ctx *ctx = m->ctx;
+    const DevQTable &qt = device_qtable(ctx, m->etype, rule, order);
+    void *stage = ensure_stage(m, fp.form, precision);
+    Timer tm(ctx);
+    tm.mark(0);
+    launch_element_stage(ctx, m, qt, fp, precision, stage);
+    tm.mark(1);
+    launch_gather(ctx, m, *m->pattern, fp, precision, stage, nzval);
+    tm.mark(2);
+    tm.done();
+}
+
+feasm_matrix *coo_path(feasm_mesh *m, const FormParams &fp, int rule, int order, int precision) {
+    feasm_ctx *ctx = m->ctx;
+    const DevQTable &qt = device_qtable(ctx, m->etype, rule, order);
+    const int64_t kd = (int64_t)m->ndn * m->npe;
+    const int64_t nt = m->nelem * kd * kd;
+    DevPtr I = dev_alloc(nt * sizeof(int32_t)), J = dev_alloc(nt * sizeof(int32_t)),
+           V = dev_alloc(nt * sizeof(double));
+    launch_element_coo(ctx, m, qt, fp, precision, dp<int32_t>(I), dp<int32_t>(J), dp<double>(V));
+    return sparse_device(ctx, nt, dp<int32_t>(I), dp<int32_t>(J), dp<double>(V), m->nalldofs, m->nalldofs);
+}
+
+}  // namespace
+
+void feasm_ctx::timing_configure(int slots) {
+    FEASM_HIP(hipStreamSynchronize(stream));
+    for (auto e : ring) (void)hipEventDestroy(e);
+    ring.clear();
+    ring_slots = slots;
+    ring_next = ring_used = 0;
+    timing = slots > 0;
+    ring.resize(3 * (size_t)slots);
+    for (auto &e : ring) FEASM_HIP(hipEventCreate(&e));
+}
+
+void *feasm_ctx::scratch_get(size_t bytes) {
+    if (!scratch || scratch->bytes < bytes) {
+        if (scratch) FEASM_HIP(hipStreamSynchronize(stream));
+        scratch.reset();
+        scratch = dev_alloc(std::max<size_t>(bytes, 1 << 20));
+    }
+    return scratch->p;
+}
+
+// ======================================================================= context
+extern "C" int feasm_ctx_create(int device, feasm_ctx **out) {
+    API_TRY
+    FEASM_REQUIRE(out, FEASM_EINVAL, "null output pointer");
+    int ndev = 0;
+    FEASM_HIP(hipGetDeviceCount(&ndev));
+    FEASM_REQUIRE(device >= 0 && device < ndev, FEASM_EINVAL, "no such HIP device");
+    FEASM_HIP(hipSetDevice(device));
+    std::unique_ptr<feasm_ctx> c(new feasm_ctx);
+    c->device = device;
+    FEASM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    *out = c.release();
+    API_CATCH(nullptr)
+}
+
+extern "C" int feasm_ctx_destroy(feasm_ctx *ctx) {
+    if (!ctx) return FEASM_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->qtables.clear();
+    ctx->scratch.reset();
+    for (auto e : ctx->ring) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return FEASM_OK;
+}
+
+extern "C" const char *feasm_last_error(const feasm_ctx *ctx) {
+    return ctx ? ctx->last_error.c_str() : g_last_error.c_str();
+}
+
+extern "C" void *feasm_ctx_stream(feasm_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+extern "C" int feasm_ctx_synchronize(feasm_ctx *ctx) {
+    API_TRY
+    FEASM_REQUIRE(ctx, FEASM_EINVAL, "null context");
+    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    API_CATCH(ctx)
+}
+
+extern "C" int feasm_mem_info(feasm_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes) {
+    API_TRY
+    size_t f = 0, t = 0;
+    FEASM_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = (int64_t)f;
+    if (total_bytes) *total_bytes = (int64_t)t;
+    API_CATCH(ctx)
+}
+
+extern "C" int feasm_ctx_set_timing(feasm_ctx *ctx, int slots) {
+    API_TRY
+    FEASM_REQUIRE(ctx, FEASM_EINVAL, "null context");
+    FEASM_REQUIRE(slots >= 0 && slots <= 4096, FEASM_EINVAL, "0..4096 timing slots");
+    ctx->timing_configure(slots);
+    API_CATCH(ctx)
+}
+
+extern "C" int feasm_timings(feasm_ctx *ctx, double *out, int max_calls, int *ncalls) {
+    API_TRY
+    FEASM_REQUIRE(ctx && out && ncalls, FEASM_EINVAL, "null argument");
+    FEASM_REQUIRE(ctx->timing, FEASM_ESTATE, "timing is off");
+    const int n = std::min(ctx->ring_used, max_calls);
+    // oldest first
+    const int first = (ctx->ring_next - ctx->ring_used + ctx->ring_slots) % ctx->ring_slots;
+    const int skip = ctx->ring_used - n;
+    for (int i = 0; i < n; i++) {
+        const int s = (first + skip + i) % ctx->ring_slots;
+        hipEvent_t *ev = &ctx->ring[3 * (size_t)s];
+        FEASM_HIP(hipEventSynchronize(ev[2]));
+        float a = 0, b = 0;
+        FEASM_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+        FEASM_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+        out[3 * i + 0] = a;
+        out[3 * i + 1] = b;
+        out[3 * i + 2] = (double)a + (double)b;
+    }
+    *ncalls = n;
+    API_CATCH(ctx)
+}
+
+extern "C" int feasm_timings_reset(feasm_ctx *ctx) {
+    if (!ctx) return fail(nullptr, FEASM_EINVAL, "null context");
+    ctx->ring_next = ctx->ring_used = 0;
+    return FEASM_OK;
+}
+
+// ======================================================================= meshes
+static feasm_mesh *new_mesh(feasm_ctx *ctx, int etype, int64_t nelem, int npe, int64_t nnodes) {
+    FEASM_REQUIRE(ctx, FEASM_EINVAL, "null context");
+    FEASM_REQUIRE(etype == FEASM_H8 || etype == FEASM_T10, FEASM_EINVAL, "unknown element type");
+    FEASM_REQUIRE(npe == (etype == FEASM_H8 ? 8 : 10), FEASM_EINVAL, "nodes per element does not match the type");
+    FEASM_REQUIRE(nelem >= 0 && nnodes >= 0, FEASM_EINVAL, "negative size");
+    FEASM_REQUIRE(nnodes < (1ll << 31), FEASM_EUNSUPPORTED, "more than 2^31 nodes");
+    FEASM_REQUIRE(nelem < (1ll << 25), FEASM_EUNSUPPORTED, "more than 2^25 elements in one mesh");
+    auto *m = new feasm_mesh;
+    m->ctx = ctx;
+    m->etype = etype;
+    m->npe = npe;
+    m->nelem = nelem;
+    m->nnodes = nnodes;
+    return m;
+}
+
+extern "C" int feasm_mesh_create(feasm_ctx *ctx, int elem_type, int64_t nelem, int npe, const int64_t *conn,
+                                 int64_t nnodes, const double *xyz_colmajor, feasm_mesh **out) {
+    API_TRY
+    FEASM_REQUIRE(out && (conn || nelem == 0) && (xyz_colmajor || nnodes == 0), FEASM_EINVAL, "null argument");
+    std::unique_ptr<feasm_mesh> m(new_mesh(ctx, elem_type, nelem, npe, nnodes));
+    std::vector<int32_t> c0(nelem * npe);
+    for (int64_t i = 0; i < nelem * npe; i++) {
+        FEASM_REQUIRE(conn[i] >= 1 && conn[i] <= nnodes, FEASM_EINVAL, "connectivity refers to a missing node");
+        c0[i] = (int32_t)(conn[i] - 1);
+    }
+    std::vector<double> x(3 * nnodes);
+    for (int64_t n = 0; n < nnodes; n++)
+        for (int k = 0; k < 3; k++) x[3 * n + k] = xyz_colmajor[n + nnodes * k];
+    m->conn = dev_alloc(std::max<size_t>(c0.size() * sizeof(int32_t), 4));
+    m->xyz = dev_alloc(std::max<size_t>(x.size() * sizeof(double), 8));
+    FEASM_HIP(hipMemcpy(m->conn->p, c0.data(), c0.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    FEASM_HIP(hipMemcpy(m->xyz->p, x.data(), x.size() * sizeof(double), hipMemcpyHostToDevice));
+    *out = m.release();
+    API_CATCH(ctx)
+}
+
+extern "C" int feasm_mesh_create_device(feasm_ctx *ctx, int elem_type, int64_t nelem, int npe,
+                                        const int32_t *d_conn0, int64_t nnodes, const double *d_xyz,
+                                        feasm_mesh **out) {
+    API_TRY
+    FEASM_REQUIRE(out && d_conn0 && d_xyz, FEASM_EINVAL, "null argument");
+    std::unique_ptr<feasm_mesh> m(new_mesh(ctx, elem_type, nelem, npe, nnodes));
+    m->conn = dev_alloc(std::max<size_t>(nelem * npe * sizeof(int32_t), 4));
+    m->xyz = dev_alloc(std::max<size_t>(nnodes * 3 * sizeof(double), 8));
+    FEASM_HIP(hipMemcpyAsync(m->conn->p, d_conn0, nelem * npe * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                             ctx->stream));
+    FEASM_HIP(hipMemcpyAsync(m->xyz->p, d_xyz, nnodes * 3 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    *out = m.release();
+    API_CATCH(ctx)
+}
+
+extern "C" int feasm_mesh_destroy(feasm_mesh *mesh) {
+    if (!mesh) return FEASM_OK;
+    (void)hipStreamSynchronize(mesh->ctx->stream);
+    delete mesh;
+    return FEASM_OK;
+}
+
+extern "C" int feasm_mesh_set_xyz(feasm_mesh *m, const double *xyz_rowmajor) {
+    API_TRY
+    FEASM_REQUIRE(m && xyz_rowmajor, FEASM_EINVAL, "null argument");
+    FEASM_HIP(hipMemcpy(m->xyz->p, xyz_rowmajor, m->nnodes * 3 * sizeof(double), hipMemcpyHostToDevice));
+    API_CATCH(m ? m->ctx : nullptr)
+}
+
+static void set_dofs(feasm_mesh *m, int ndn, const std::vector<int64_t> &rowmajor1, int64_t nalldofs, bool nm) {
+    std::vector<int32_t> d0(rowmajor1.size());
+    for (size_t i = 0; i < d0.size(); i++) d0[i] = (int32_t)(rowmajor1[i] - 1);
+    m->dofs = dev_alloc(std::max<size_t>(d0.size() * sizeof(int32_t), 4));
+    FEASM_HIP(hipMemcpy(m->dofs->p, d0.data(), d0.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (m->ndn != ndn) m->pattern.reset();
+    m->ndn = ndn;
+    m->nalldofs = nalldofs;
+    m->nodemajor = nm;
+}
+
+extern "C" int feasm_dofs_set(feasm_mesh *m, int ndn, const int64_t *dofnums_colmajor, int64_t nalldofs) {
+    API_TRY
+    FEASM_REQUIRE(m && dofnums_colmajor, FEASM_EINVAL, "null argument");
+    FEASM_REQUIRE(ndn >= 1 && ndn <= 3, FEASM_EUNSUPPORTED, "1 to 3 DOFs per node");
+    FEASM_REQUIRE(nalldofs >= 0 && nalldofs < (1ll << 31), FEASM_EUNSUPPORTED, "more than 2^31 DOFs");
+    std::vector<int64_t> rm(m->nnodes * ndn);
+    bool nm = (nalldofs == m->nnodes * ndn);
+    for (int64_t n = 0; n < m->nnodes; n++)
+        for (int k = 0; k < ndn; k++) {
+            const int64_t d = dofnums_colmajor[n + m->nnodes * k];
+            rm[n * ndn + k] = d;
+            if (d != n * ndn + k + 1) nm = false;
+        }
+    set_dofs(m, ndn, rm, nalldofs, nm);
+    if (!nm) check_dofs_like_assemble(m, rm);
+    API_CATCH(m ? m->ctx : nullptr)
+}
+
+extern "C" int feasm_dofs_set_nodemajor(feasm_mesh *m, int ndn) {
+    API_TRY
+    FEASM_REQUIRE(m, FEASM_EINVAL, "null mesh");
+    FEASM_REQUIRE(ndn >= 1 && ndn <= 3, FEASM_EUNSUPPORTED, "1 to 3 DOFs per node");
+    std::vector<int64_t> rm(m->nnodes * ndn);
+    for (int64_t i = 0; i < (int64_t)rm.size(); i++) rm[i] = i + 1;
+    set_dofs(m, ndn, rm, m->nnodes * ndn, true);
+    API_CATCH(m ? m->ctx : nullptr)
+}
+
+// ======================================================================= bilinear forms
+extern "C" int feasm_pattern_build(feasm_mesh *m) {
+    API_TRY
+    FEASM_REQUIRE(m && m->ndn > 0, FEASM_ESTATE, "set the DOF numbering first");
+    FEASM_REQUIRE(m->nodemajor, FEASM_EUNSUPPORTED, "resident pattern needs the node-major numbering");
+    m->pattern.reset();
+    ensure_pattern(m);
+    API_CATCH(m ? m->ctx : nullptr)
+}
+
+extern "C" int feasm_pattern_info(feasm_mesh *m, int64_t *nnz, int64_t *nblocks) {
+    API_TRY
+    FEASM_REQUIRE(m && m->pattern, FEASM_ESTATE, "no pattern built");
+    if (nnz) *nnz = m->pattern->nnz;
+    if (nblocks) *nblocks = m->pattern->nblocks;
+    API_CATCH(m ? m->ctx : nullptr)
+}
+
+extern "C" int feasm_bilform(feasm_mesh *m, int form, int rule, int order, const double *coeff, int ncoeff,
+                             int precision, feasm_matrix **out) {
+    API_TRY
+    FEASM_REQUIRE(m && out, FEASM_EINVAL, "null argument");
+    FEASM_REQUIRE(m->ndn > 0, FEASM_ESTATE, "set the DOF numbering first");
+    FEASM_REQUIRE(precision == FEASM_FP64 || precision == FEASM_MIXED, FEASM_EINVAL, "unknown precision");
+    check_rule(m, rule);
+    const FormParams fp = make_form(form, m->ndn, coeff, ncoeff);
+    if (!m->nodemajor) {
+        *out = coo_path(m, fp, rule, order, precision);
+        return FEASM_OK;
+    }
+    ensure_pattern(m);
+    std::unique_ptr<feasm_matrix> M(new feasm_matrix);
+    M->ctx = m->ctx;
+    M->m = M->n = m->pattern->n;
+    M->nnz = m->pattern->nnz;
+    M->colptr = m->pattern->colptr;
+    M->rowval = m->pattern->rowval;
+    M->pattern = m->pattern;
+    M->form_class = m->ndn;
+    M->nzval = dev_alloc(std::max<size_t>(M->nnz * sizeof(double), 8));
+    numeric_into(m, fp, rule, order, precision, dp<double>(M->nzval));
+    FEASM_HIP(hipStreamSynchronize(m->ctx->stream));
+    *out = M.release();
+    API_CATCH(m ? m->ctx : nullptr)
+}
+
+extern "C" int feasm_bilform_into(feasm_mesh *m, int form, int rule, int order, const double *coeff, int ncoeff,
+                                  int precision, feasm_matrix *mat) {
+    API_TRY
+    FEASM_REQUIRE(m && mat, FEASM_EINVAL, "null argument");
+    FEASM_REQUIRE(m->nodemajor && m->pattern && mat->pattern == m->pattern, FEASM_ESTATE,
+                  "matrix does not hold this mesh's resident pattern");
+    FEASM_REQUIRE(precision == FEASM_FP64 || precision == FEASM_MIXED, FEASM_EINVAL, "unknown precision");
+    check_rule(m, rule);
+    const FormParams fp = make_form(form, m->ndn, coeff, ncoeff);
+    numeric_into(m, fp, rule, order, precision, dp<double>(mat->nzval));
+    API_CATCH(m ? m->ctx : nullptr)
+}
+
+extern "C" int feasm_bilform_coo(feasm_mesh *m, int form, int rule, int order, const double *coeff, int ncoeff,
+                                 int precision, feasm_matrix **out) {
+    API_TRY
+    FEASM_REQUIRE(m && out, FEASM_EINVAL, "null argument");
+    FEASM_REQUIRE(m->ndn > 0, FEASM_ESTATE, "set the DOF numbering first");
+    FEASM_REQUIRE(precision == FEASM_FP64 || precision == FEASM_MIXED, FEASM_EINVAL, "unknown precision");
+    check_rule(m, rule);
+    const FormParams fp = make_form(form, m->ndn, coeff, ncoeff);
+    *out = coo_path(m, fp, rule, order, precision);
+    API_CATCH(m ? m->ctx : nullptr)
+}
+
+// ======================================================================= matrices
+extern "C" int feasm_matrix_dims(const feasm_matrix *mat, int64_t *m, int64_t *n, int64_t *nnz) {
+    if (!mat) return fail(nullptr, FEASM_EINVAL, "null matrix");
+    if (m) *m = mat->m;
+    if (n) *n = mat->n;
+    if (nnz) *nnz = mat->nnz;
+    return FEASM_OK;
+}
+
+extern "C" int feasm_matrix_copy_csc(feasm_matrix *mat, int base, int64_t *colptr, int64_t *rowval, double *nzval) {
+    API_TRY
+    FEASM_REQUIRE(mat && colptr && (rowval || mat->nnz == 0) && (nzval || mat->nnz == 0), FEASM_EINVAL,
+                  "null argument");
+    FEASM_REQUIRE(base == 0 || base == 1, FEASM_EINVAL, "base must be 0 or 1");
+    hipStream_t s = mat->ctx->stream;
+    FEASM_HIP(hipMemcpyAsync(colptr, mat->colptr->p, (mat->n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    std::vector<int32_t> rv(mat->nnz);
+    if (mat->nnz) {
+        FEASM_HIP(hipMemcpyAsync(rv.data(), mat->rowval->p, mat->nnz * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        FEASM_HIP(hipMemcpyAsync(nzval, mat->nzval->p, mat->nnz * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    FEASM_HIP(hipStreamSynchronize(s));
+    for (int64_t i = 0; i <= mat->n; i++) colptr[i] += base;
+    for (int64_t i = 0; i < mat->nnz; i++) rowval[i] = (int64_t)rv[i] + base;
+    API_CATCH(mat ? mat->ctx : nullptr)
+}
+
+extern "C" int feasm_matrix_device_csc(feasm_matrix *mat, const int64_t **colptr, const int32_t **rowval,
+                                       double **nzval) {
+    if (!mat) return fail(nullptr, FEASM_EINVAL, "null matrix");
+    if (colptr) *colptr = dp<int64_t>(mat->colptr);
+    if (rowval) *rowval = dp<int32_t>(mat->rowval);
+    if (nzval) *nzval = dp<double>(mat->nzval);
+    return FEASM_OK;
+}
+
+extern "C" int feasm_matrix_destroy(feasm_matrix *mat) {
+    if (!mat) return FEASM_OK;
+    (void)hipStreamSynchronize(mat->ctx->stream);
+    delete mat;
+    return FEASM_OK;
+}
+
+// ======================================================================= generic assembler
+// SysmatAssemblerSparse (src/AssemblyModule.jl:88-329).  Triples are buffered
+// on the host exactly as the reference buffers them (one assemble! call per
+// element matrix arrives from host code) and moved to the device once, at
+// makematrix!, where the device sparse() builds the CSC.
+struct feasm_coo {
+    feasm_ctx *ctx = nullptr;
+    int64_t buffer_length = 0, buffer_pointer = 0;  // 1-based pointer as the reference
+    int64_t row_nalldofs = 0, col_nalldofs = 0;
+    bool nomatrixresult = false;
+    std::vector<int32_t> I, J;
+    std::vector<double> V;
+};
+
+extern "C" int feasm_coo_create(feasm_ctx *ctx, int nomatrixresult, feasm_coo **out) {
+    API_TRY
+    FEASM_REQUIRE(out, FEASM_EINVAL, "null argument");
+    auto *c = new feasm_coo;
+    c->ctx = ctx;
+    c->nomatrixresult = nomatrixresult != 0;
+    *out = c;
+    API_CATCH(ctx)
+}
+
+// startassembly! (src/AssemblyModule.jl:209-238): the buffers are sized only
+// while the pointer is below 1 (a fresh assembler); afterwards they and the
+// matrix dimensions stay as they are.
+extern "C" int feasm_coo_start(feasm_coo *c, int64_t elem_nrows, int64_t elem_ncols, int64_t n_elem_mats,
+                               int64_t row_nalldofs, int64_t col_nalldofs) {
+    API_TRY
+    FEASM_REQUIRE(c, FEASM_EINVAL, "null assembler");
+    FEASM_REQUIRE(elem_nrows >= 0 && elem_ncols >= 0 && n_elem_mats >= 0, FEASM_EINVAL, "negative size");
+    FEASM_REQUIRE(row_nalldofs < (1ll << 31) && col_nalldofs < (1ll << 31), FEASM_EUNSUPPORTED,
+                  "more than 2^31 DOFs");
+    if (c->buffer_pointer < 1) {
+        c->buffer_length = elem_nrows * elem_ncols * n_elem_mats;
+        c->I.resize(c->buffer_length);
+        c->J.resize(c->buffer_length);
+        c->V.resize(c->buffer_length);
+        c->buffer_pointer = 1;
+        c->row_nalldofs = row_nalldofs;
+        c->col_nalldofs = col_nalldofs;
+    }
+    API_CATCH(c ? c->ctx : nullptr)
+}
+
+static void coo_append(feasm_coo *c, const double *mat, const int64_t *rows, int64_t nrows, const int64_t *cols,
+                       int64_t ncols) {
+    int64_t p = c->buffer_pointer;
+    if (c->buffer_length < p + ncols * nrows - 1) {  // _resize_buffers, :61-76
+        c->buffer_length += ncols * nrows * 1000;
+        c->I.resize(c->buffer_length);
+        c->J.resize(c->buffer_length);
+        c->V.resize(c->buffer_length);
+    }
+    for (int64_t j = 0; j < ncols; j++) {
+        const int64_t dj = cols[j];
+        if (dj < 1) throw Error(FEASM_EDOF, "Column degree of freedom < 1");
+        if (dj > c->col_nalldofs) throw Error(FEASM_EDOF, "Column degree of freedom > size");
+        for (int64_t i = 0; i < nrows; i++) {
+            const int64_t di = rows[i];
+            if (di < 1) throw Error(FEASM_EDOF, "Row degree of freedom < 1");
+            if (di > c->row_nalldofs) throw Error(FEASM_EDOF, "Row degree of freedom > size");
+        }
+    }
+    for (int64_t j = 0; j < ncols; j++)
+        for (int64_t i = 0; i < nrows; i++) {
+            c->V[p - 1] = mat[i + nrows * j];
+            c->I[p - 1] = (int32_t)(rows[i] - 1);
+            c->J[p - 1] = (int32_t)(cols[j] - 1);
+            p++;
+        }
+    c->buffer_pointer = p;
+}
+
+extern "C" int feasm_coo_assemble(feasm_coo *c, const double *mat, int64_t mat_nrows, int64_t mat_ncols,
+                                  const int64_t *rows, int64_t nrows, const int64_t *cols, int64_t ncols) {
+    API_TRY
+    FEASM_REQUIRE(c, FEASM_EINVAL, "null assembler");
+    FEASM_REQUIRE(c->buffer_pointer >= 1, FEASM_ESTATE, "startassembly! has not been called");
+    FEASM_REQUIRE(mat_nrows == nrows && mat_ncols == ncols, FEASM_ESIZE, "Wrong size of matrix");
+    coo_append(c, mat, rows, nrows, cols, ncols);
+    API_CATCH(c ? c->ctx : nullptr)
+}
+
+extern "C" int feasm_coo_assemble_many(feasm_coo *c, int64_t nmats, const double *mats, const int64_t *rows,
+                                       int64_t nrows, const int64_t *cols, int64_t ncols) {
+    API_TRY
+    FEASM_REQUIRE(c, FEASM_EINVAL, "null assembler");
+    FEASM_REQUIRE(c->buffer_pointer >= 1, FEASM_ESTATE, "startassembly! has not been called");
+    for (int64_t e = 0; e < nmats; e++)
+        coo_append(c, mats + e * nrows * ncols, rows + e * nrows, nrows, cols + e * ncols, ncols);
+    API_CATCH(c ? c->ctx : nullptr)
+}
+
+extern "C" int feasm_coo_set_nomatrixresult(feasm_coo *c, int flag) {
+    if (!c) return fail(nullptr, FEASM_EINVAL, "null assembler");
+    c->nomatrixresult = flag != 0;
+    return FEASM_OK;
+}
+
+extern "C" int64_t feasm_coo_ntriples(const feasm_coo *c) { return c ? std::max<int64_t>(c->buffer_pointer - 1, 0) : -1; }
+
+extern "C" int feasm_coo_makematrix(feasm_coo *c, feasm_matrix **out) {
+    API_TRY
+    FEASM_REQUIRE(c && out, FEASM_EINVAL, "null argument");
+    *out = nullptr;
+    if (c->nomatrixresult) return FEASM_OK;  // triples kept, pointer not reset (:309-317)
+    const int64_t nt = std::max<int64_t>(c->buffer_pointer - 1, 0);
+    feasm_ctx *ctx = c->ctx;
+    FEASM_REQUIRE(ctx, FEASM_ESTATE, "assembler has no device context");
+    DevPtr I = dev_alloc(std::max<size_t>(nt * sizeof(int32_t), 4)), J = dev_alloc(std::max<size_t>(nt * 4, 4)),
+           V = dev_alloc(std::max<size_t>(nt * sizeof(double), 8));
+    if (nt) {
+        FEASM_HIP(hipMemcpy(I->p, c->I.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice));
+        FEASM_HIP(hipMemcpy(J->p, c->J.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice));
+        FEASM_HIP(hipMemcpy(V->p, c->V.data(), nt * sizeof(double), hipMemcpyHostToDevice));
+    }
+    *out = sparse_device(ctx, nt, dp<int32_t>(I), dp<int32_t>(J), dp<double>(V), c->row_nalldofs, c->col_nalldofs);
+    c->buffer_pointer = 1;
+    API_CATCH(c ? c->ctx : nullptr)
+}
+
+extern "C" int feasm_coo_destroy(feasm_coo *c) {
+    delete c;
+    return FEASM_OK;
+}
+
+extern "C" int feasm_sparse_device(feasm_ctx *ctx, int64_t ntrip, const int32_t *d_I, const int32_t *d_J,
+                                   const double *d_V, int64_t m, int64_t n, feasm_matrix **out) {
+    API_TRY
+    FEASM_REQUIRE(ctx && out, FEASM_EINVAL, "null argument");
+    FEASM_REQUIRE(m < (1ll << 31) && n < (1ll << 31), FEASM_EUNSUPPORTED, "more than 2^31 rows or columns");
+    *out = sparse_device(ctx, ntrip, d_I, d_J, d_V, m, n);
+    API_CATCH(ctx)
+}
+
+// ======================================================================= generators
+extern "C" int feasm_gen_h8block(double L, double W, double H, int64_t nL, int64_t nW, int64_t nH, int64_t *nelem,
+                                 int64_t *nnodes, int64_t *conn, double *xyz) {
+    API_TRY
+    FEASM_REQUIRE(nL > 0 && nW > 0 && nH > 0, FEASM_EINVAL, "block needs at least one cell per direction");
+    const int64_t ne = nL * nW * nH, nn = (nL + 1) * (nW + 1) * (nH + 1);
+    if (nelem) *nelem = ne;
+    if (nnodes) *nnodes = nn;
+    if (conn && xyz) gen_h8block(L, W, H, nL, nW, nH, conn, xyz);
+    API_CATCH(nullptr)
+}
+
+extern "C" int feasm_gen_t10block(double L, double W, double H, int64_t nL, int64_t nW, int64_t nH,
+                                  double amp_vertex, double amp_mid, uint64_t seed, int64_t *nelem, int64_t *nnodes,
+                                  int64_t *conn, double *xyz) {
+    API_TRY
+    FEASM_REQUIRE(nL > 0 && nW > 0 && nH > 0, FEASM_EINVAL, "block needs at least one cell per direction");
+    const int64_t ne = 6 * nL * nW * nH;
+    // nodes: vertices + distinct edges of the six-tet split (closed form)
+    const int64_t nx = nL, ny = nW, nz = nH;
+    const int64_t nv = (nx + 1) * (ny + 1) * (nz + 1);
+    const int64_t ned = nx * (ny + 1) * (nz + 1) + (nx + 1) * ny * (nz + 1) + (nx + 1) * (ny + 1) * nz  // axis
+                        + nx * ny * (nz + 1) + nx * (ny + 1) * nz + (nx + 1) * ny * nz               // face diagonals
+                        + nx * ny * nz;                                                             // body diagonal
+    if (nelem) *nelem = ne;
+    if (nnodes) *nnodes = nv + ned;
+    if (conn && xyz) {
+        std::vector<double> x;
+        const int64_t ntot = gen_t10block(L, W, H, nL, nW, nH, amp_vertex, amp_mid, seed, conn, x);
+        FEASM_REQUIRE(ntot == nv + ned, FEASM_EINVAL, "internal node count mismatch");
+        std::memcpy(xyz, x.data(), x.size() * sizeof(double));
+    }
+    API_CATCH(nullptr)
+}
+
+extern "C" int feasm_gen_perturb(int64_t nnodes, double *xyz, const double lo[3], const double hi[3], double amp,
+                                 uint64_t seed) {
+    API_TRY
+    FEASM_REQUIRE(xyz && lo && hi, FEASM_EINVAL, "null argument");
+    perturb_nodes(nnodes, xyz, lo, hi, amp, seed);
+    API_CATCH(nullptr)
+}

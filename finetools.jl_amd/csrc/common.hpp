@@ -1,0 +1,184 @@
+// Internal declarations shared by the translation units of libfeasm.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/feasm.h"
+
+namespace feasm {
+
+// ---------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define FEASM_HIP(call)                                                                              \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            throw ::feasm::Error(e_ == hipErrorOutOfMemory ? FEASM_ENOMEM : FEASM_EHIP,              \
+                                 std::string("HIP error in ") + #call + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define FEASM_REQUIRE(cond, code, msg)                         \
+    do {                                                       \
+        if (!(cond)) throw ::feasm::Error((code), (msg));      \
+    } while (0)
+
+// ---------------------------------------------------------------- device buffers
+// Owning device allocation; shared so that a matrix can keep the pattern
+// arrays of its mesh alive.
+struct DevMem {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    explicit DevMem(size_t n) : bytes(n) {
+        if (n) FEASM_HIP(hipMalloc(&p, n));
+    }
+    ~DevMem() {
+        if (p) (void)hipFree(p);
+    }
+    DevMem(const DevMem &) = delete;
+    DevMem &operator=(const DevMem &) = delete;
+};
+using DevPtr = std::shared_ptr<DevMem>;
+
+inline DevPtr dev_alloc(size_t bytes) { return std::make_shared<DevMem>(bytes); }
+
+template <typename T>
+inline T *dp(const DevPtr &d) {
+    return d ? static_cast<T *>(d->p) : nullptr;
+}
+
+// ---------------------------------------------------------------- quadrature tables
+// Basis values and parametric gradients at the quadrature points of one
+// (element type, rule) pair, as integrationdata() returns them
+// (src/IntegDomainModule.jl:632-648).  Layout: N[q*npe + a],
+// dN[(q*npe + a)*3 + k], w[q].
+struct QTable {
+    int etype = 0, rule = 0, order = 0, npe = 0, nqp = 0;
+    std::vector<double> N, dN, w;
+};
+QTable make_qtable(int etype, int rule, int order);  // tables.cpp
+
+// Device copy of a QTable, one contiguous buffer: [N | dN | w].
+struct DevQTable {
+    int npe = 0, nqp = 0;
+    DevPtr buf;
+    const double *N() const { return dp<double>(buf); }
+    const double *dN() const { return dp<double>(buf) + npe * nqp; }
+    const double *w() const { return dp<double>(buf) + 4 * npe * nqp; }
+};
+
+}  // namespace feasm
+
+// ---------------------------------------------------------------- handles
+struct feasm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    // timing ring: when on, every numeric assembly records three events
+    // (before the element kernel, between the kernels, after the gather) in
+    // the next slot of the ring
+    bool timing = false;
+    int ring_slots = 0, ring_next = 0, ring_used = 0;
+    std::vector<hipEvent_t> ring;  // 3 events per slot
+    void timing_configure(int slots);
+    hipEvent_t ring_event(int k) { return ring[3 * ring_next + k]; }
+    void ring_advance() {
+        ring_next = (ring_next + 1) % ring_slots;
+        if (ring_used < ring_slots) ring_used++;
+    }
+    std::map<int, feasm::DevQTable> qtables;  // key = etype*1000 + rule*100 + order
+    // reusable scratch for library temporaries (sort/scan storage)
+    feasm::DevPtr scratch;
+    void *scratch_get(size_t bytes);
+};
+
+struct Pattern {
+    int ndn = 0;
+    int64_t nnodes = 0, nblocks = 0, ncontrib = 0, n = 0, nnz = 0;
+    feasm::DevPtr adj_ptr;   // int64 [nnodes+1] node-block row pointer per column node
+    feasm::DevPtr adj;       // int32 [nblocks]  row node of each block (sorted per column node)
+    feasm::DevPtr bcol;      // int32 [nblocks]  column node of each block
+    feasm::DevPtr cptr;      // int64 [nblocks+1] contribution list pointer
+    feasm::DevPtr contrib;   // uint32 [ncontrib] (e << 7) | (pair << 1) | transposed
+    feasm::DevPtr colptr;    // int64 [n+1]  DOF-level CSC column pointer (0-based)
+    feasm::DevPtr rowval;    // int32 [nnz]  DOF-level CSC row index (0-based)
+};
+
+struct feasm_mesh {
+    feasm_ctx *ctx = nullptr;
+    int etype = 0, npe = 0;
+    int64_t nelem = 0, nnodes = 0;
+    feasm::DevPtr conn;     // int32 [nelem*npe] 0-based
+    feasm::DevPtr xyz;      // double [nnodes*3]
+    feasm::DevPtr n2e_ptr;  // int64 [nnodes+1]
+    feasm::DevPtr n2e;      // uint32 [nelem*npe]  (e << 4) | local index, ascending e
+    int ndn = 0;
+    int64_t nalldofs = 0;
+    bool nodemajor = false;
+    feasm::DevPtr dofs;     // int32 [nnodes*ndn] 0-based DOF numbers (node-major rows)
+    std::shared_ptr<Pattern> pattern;
+    feasm::DevPtr stage;    // per-element staging of the node-pair blocks
+};
+
+struct feasm_matrix {
+    feasm_ctx *ctx = nullptr;
+    int64_t m = 0, n = 0, nnz = 0;
+    feasm::DevPtr colptr;  // int64 [n+1]
+    feasm::DevPtr rowval;  // int32 [nnz]
+    feasm::DevPtr nzval;   // double [nnz]
+    std::shared_ptr<Pattern> pattern;  // set when the values sit in a mesh's resident pattern
+    int form_class = -1;               // ndn of the blocks (for bilform_into checks)
+};
+
+namespace feasm {
+
+// tables.cpp
+const DevQTable &device_qtable(feasm_ctx *ctx, int etype, int rule, int order);
+
+// pattern.hip
+void build_node_to_element(feasm_mesh *m);
+std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn);
+
+// element.hip
+struct FormParams {
+    int form = 0;
+    int ndn = 1;
+    double coef[36];  // iso: c; mat: kappa 3x3; elastic: D 6x6 (row-major)
+};
+// number of upper-triangle node pairs and values per pair in the staging
+int npairs(int npe);
+int stage_width(int form);  // 9 for elasticity/matrix diffusion... see element.hip
+void launch_element_stage(feasm_ctx *ctx, const feasm_mesh *m, const DevQTable &qt, const FormParams &fp,
+                          int precision, void *stage);
+void launch_element_coo(feasm_ctx *ctx, const feasm_mesh *m, const DevQTable &qt, const FormParams &fp,
+                        int precision, int32_t *I, int32_t *J, double *V);
+void launch_gather(feasm_ctx *ctx, const feasm_mesh *m, const Pattern &p, const FormParams &fp, int precision,
+                   const void *stage, double *nzval);
+
+// coo.hip
+feasm_matrix *sparse_device(feasm_ctx *ctx, int64_t ntrip, const int32_t *I, const int32_t *J, const double *V,
+                            int64_t m, int64_t n);
+
+// scan helpers (coo.hip)
+void exclusive_scan_i64(feasm_ctx *ctx, const int64_t *in, int64_t *out, int64_t n);
+void exclusive_scan_i32_to_i64(feasm_ctx *ctx, const int32_t *in, int64_t *out, int64_t n);
+
+inline unsigned grid_for(int64_t n, int block, unsigned cap = 1u << 20) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > (int64_t)cap) g = cap;
+    return (unsigned)g;
+}
+
+}  // namespace feasm

@@ -1,0 +1,262 @@
+// Symbolic phase: the device-resident CSC pattern of a mesh's global matrix.
+//
+// The reference has no symbolic phase: assemble! buffers every triple and
+// SparseArrays.sparse discovers the pattern by sorting them on every
+// makematrix! (src/AssemblyModule.jl:250-329).  Here the pattern is built
+// once per (mesh, ndn) from the connectivity and kept in HBM:
+//   1. node -> incident elements, ascending element number (the order in
+//      which sparse() meets duplicates: element-major COO);
+//   2. node adjacency (sorted, unique) = the node-block pattern; a node pair
+//      (m, c) is a block of ndn x ndn entries;
+//   3. per block, the list of element contributions (element, upper node
+//      pair, transposed?) in ascending element order, so that the numeric
+//      phase sums duplicates in exactly the reference's order;
+//   4. the DOF-level CSC colptr/rowval for the node-major numbering
+//      dof(node, k) = node*ndn + k (numberdofs! with no EBCs,
+//      src/FieldModule.jl:356-377), rows sorted within each column.
+#include "common.hpp"
+
+#include <rocprim/rocprim.hpp>
+
+namespace feasm {
+
+namespace {
+
+constexpr int kCap = 256;  // max distinct neighbours of one node
+
+__global__ void k_n2e_count(const int32_t *__restrict__ conn, int64_t n, int32_t *__restrict__ cnt) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&cnt[conn[i]], 1);
+}
+
+__global__ void k_n2e_fill(const int32_t *__restrict__ conn, int64_t nelem, int npe,
+                           unsigned long long *__restrict__ cursor, uint32_t *__restrict__ n2e) {
+    const int64_t n = nelem * npe;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = i / npe;
+        const int a = (int)(i % npe);
+        const unsigned long long pos = atomicAdd(&cursor[conn[i]], 1ull);
+        n2e[pos] = ((uint32_t)e << 4) | (uint32_t)a;
+    }
+}
+
+// the atomic fill leaves each node's list in arbitrary order; sort it
+__global__ void k_n2e_sort(const int64_t *__restrict__ ptr, uint32_t *__restrict__ n2e, int64_t nnodes) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nnodes;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = ptr[c], e = ptr[c + 1];
+        for (int64_t i = b + 1; i < e; i++) {
+            const uint32_t v = n2e[i];
+            int64_t j = i - 1;
+            while (j >= b && n2e[j] > v) {
+                n2e[j + 1] = n2e[j];
+                j--;
+            }
+            n2e[j + 1] = v;
+        }
+    }
+}
+
+// distinct neighbours of node c (including c), sorted; count pass writes
+// deg[c], fill pass writes adj[adj_ptr[c] ...]
+__global__ void k_adj(const int64_t *__restrict__ n2e_ptr, const uint32_t *__restrict__ n2e,
+                      const int32_t *__restrict__ conn, int npe, int64_t nnodes, const int64_t *__restrict__ adj_ptr,
+                      int32_t *__restrict__ deg, int32_t *__restrict__ adj, int32_t *__restrict__ overflow) {
+    int32_t list[kCap];
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nnodes;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        int len = 0;
+        bool over = false;
+        for (int64_t t = n2e_ptr[c]; t < n2e_ptr[c + 1]; t++) {
+            const int64_t e = n2e[t] >> 4;
+            for (int b = 0; b < npe; b++) {
+                const int32_t m = conn[e * npe + b];
+                // insertion into the sorted unique list
+                int j = len - 1;
+                while (j >= 0 && list[j] > m) j--;
+                if (j >= 0 && list[j] == m) continue;
+                if (len == kCap) {
+                    over = true;
+                    continue;
+                }
+                for (int k = len - 1; k > j; k--) list[k + 1] = list[k];
+                list[j + 1] = m;
+                len++;
+            }
+        }
+        if (over) *overflow = 1;
+        if (adj_ptr == nullptr) {
+            deg[c] = len;
+        } else {
+            const int64_t o = adj_ptr[c];
+            for (int k = 0; k < len; k++) adj[o + k] = list[k];
+        }
+    }
+}
+
+__global__ void k_bcol(const int64_t *__restrict__ adj_ptr, int64_t nnodes, int32_t *__restrict__ bcol) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nnodes;
+         c += (int64_t)gridDim.x * blockDim.x)
+        for (int64_t t = adj_ptr[c]; t < adj_ptr[c + 1]; t++) bcol[t] = (int32_t)c;
+}
+
+// contributions of block t = (row node m, column node c): elements holding
+// both nodes, ascending; count pass (cptr == nullptr) or fill pass
+__global__ void k_contrib(const int32_t *__restrict__ bcol, const int32_t *__restrict__ adj, int64_t nblocks,
+                          const int64_t *__restrict__ n2e_ptr, const uint32_t *__restrict__ n2e,
+                          const int32_t *__restrict__ conn, int npe, const int64_t *__restrict__ cptr,
+                          int32_t *__restrict__ ccount, uint32_t *__restrict__ contrib) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nblocks;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t c = bcol[t], m = adj[t];
+        int cnt = 0;
+        int64_t w = cptr ? cptr[t] : 0;
+        for (int64_t s = n2e_ptr[c]; s < n2e_ptr[c + 1]; s++) {
+            const uint32_t u = n2e[s];
+            const int64_t e = u >> 4;
+            const int ac = (int)(u & 15u);
+            int am = -1;
+            for (int b = 0; b < npe; b++)
+                if (conn[e * npe + b] == m) {
+                    am = b;
+                    break;
+                }
+            if (am < 0) continue;
+            if (cptr) {
+                const int lo = am < ac ? am : ac, hi = am < ac ? ac : am;
+                const uint32_t p = (uint32_t)(hi * (hi + 1) / 2 + lo);
+                contrib[w++] = ((uint32_t)e << 7) | (p << 1) | (am > ac ? 1u : 0u);
+            }
+            cnt++;
+        }
+        if (!cptr) ccount[t] = cnt;
+    }
+}
+
+__global__ void k_dof_pattern(const int64_t *__restrict__ adj_ptr, const int32_t *__restrict__ adj, int64_t nnodes,
+                              int ndn, int64_t *__restrict__ colptr, int32_t *__restrict__ rowval) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nnodes;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = adj_ptr[c];
+        const int64_t deg = adj_ptr[c + 1] - b;
+        const int64_t base = (int64_t)ndn * ndn * b;
+        for (int l = 0; l < ndn; l++) {
+            const int64_t col0 = base + (int64_t)l * ndn * deg;
+            colptr[c * ndn + l] = col0;
+            for (int64_t r = 0; r < deg; r++)
+                for (int k = 0; k < ndn; k++) rowval[col0 + r * ndn + k] = adj[b + r] * ndn + k;
+        }
+        if (c == nnodes - 1) colptr[nnodes * ndn] = base + (int64_t)ndn * ndn * deg;
+    }
+}
+
+int64_t read_i64(feasm_ctx *ctx, const int64_t *d) {
+    int64_t v = 0;
+    FEASM_HIP(hipMemcpyAsync(&v, d, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
+    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    return v;
+}
+
+}  // namespace
+
+void exclusive_scan_i32_to_i64(feasm_ctx *ctx, const int32_t *in, int64_t *out, int64_t n) {
+    size_t bytes = 0;
+    FEASM_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(),
+                                      ctx->stream));
+    void *tmp = ctx->scratch_get(bytes);
+    FEASM_HIP(
+        rocprim::exclusive_scan(tmp, bytes, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), ctx->stream));
+}
+
+void exclusive_scan_i64(feasm_ctx *ctx, const int64_t *in, int64_t *out, int64_t n) {
+    size_t bytes = 0;
+    FEASM_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(),
+                                      ctx->stream));
+    void *tmp = ctx->scratch_get(bytes);
+    FEASM_HIP(
+        rocprim::exclusive_scan(tmp, bytes, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), ctx->stream));
+}
+
+void build_node_to_element(feasm_mesh *m) {
+    feasm_ctx *ctx = m->ctx;
+    const int64_t nent = m->nelem * m->npe;
+    FEASM_REQUIRE(m->nelem < (1ll << 25), FEASM_EUNSUPPORTED, "more than 2^25 elements in one mesh");
+    DevPtr cnt = dev_alloc((m->nnodes + 1) * sizeof(int32_t));
+    FEASM_HIP(hipMemsetAsync(cnt->p, 0, (m->nnodes + 1) * sizeof(int32_t), ctx->stream));
+    hipLaunchKernelGGL(k_n2e_count, dim3(grid_for(nent, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(m->conn),
+                       nent, dp<int32_t>(cnt));
+    m->n2e_ptr = dev_alloc((m->nnodes + 1) * sizeof(int64_t));
+    exclusive_scan_i32_to_i64(ctx, dp<int32_t>(cnt), dp<int64_t>(m->n2e_ptr), m->nnodes + 1);
+    DevPtr cursor = dev_alloc((m->nnodes + 1) * sizeof(int64_t));
+    FEASM_HIP(hipMemcpyAsync(cursor->p, m->n2e_ptr->p, (m->nnodes + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice,
+                             ctx->stream));
+    m->n2e = dev_alloc(nent * sizeof(uint32_t));
+    hipLaunchKernelGGL(k_n2e_fill, dim3(grid_for(nent, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(m->conn),
+                       m->nelem, m->npe, dp<unsigned long long>(cursor), dp<uint32_t>(m->n2e));
+    hipLaunchKernelGGL(k_n2e_sort, dim3(grid_for(m->nnodes, 256)), dim3(256), 0, ctx->stream,
+                       dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e), m->nnodes);
+    FEASM_HIP(hipGetLastError());
+    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn) {
+    feasm_ctx *ctx = m->ctx;
+    if (!m->n2e_ptr) build_node_to_element(m);
+    auto P = std::make_shared<Pattern>();
+    P->ndn = ndn;
+    P->nnodes = m->nnodes;
+    const int64_t nn = m->nnodes;
+    // 2. adjacency
+    DevPtr deg = dev_alloc((nn + 1) * sizeof(int32_t));
+    DevPtr overflow = dev_alloc(sizeof(int32_t));
+    FEASM_HIP(hipMemsetAsync(overflow->p, 0, sizeof(int32_t), ctx->stream));
+    FEASM_HIP(hipMemsetAsync(deg->p, 0, (nn + 1) * sizeof(int32_t), ctx->stream));
+    hipLaunchKernelGGL(k_adj, dim3(grid_for(nn, 64)), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr),
+                       dp<uint32_t>(m->n2e), dp<int32_t>(m->conn), m->npe, nn, (const int64_t *)nullptr,
+                       dp<int32_t>(deg), (int32_t *)nullptr, dp<int32_t>(overflow));
+    FEASM_HIP(hipGetLastError());
+    P->adj_ptr = dev_alloc((nn + 1) * sizeof(int64_t));
+    exclusive_scan_i32_to_i64(ctx, dp<int32_t>(deg), dp<int64_t>(P->adj_ptr), nn + 1);
+    P->nblocks = read_i64(ctx, dp<int64_t>(P->adj_ptr) + nn);
+    int32_t ovf = 0;
+    FEASM_HIP(hipMemcpy(&ovf, overflow->p, sizeof(int32_t), hipMemcpyDeviceToHost));
+    FEASM_REQUIRE(ovf == 0, FEASM_EUNSUPPORTED, "a node has more than 256 distinct neighbours");
+    P->adj = dev_alloc(P->nblocks * sizeof(int32_t));
+    hipLaunchKernelGGL(k_adj, dim3(grid_for(nn, 64)), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr),
+                       dp<uint32_t>(m->n2e), dp<int32_t>(m->conn), m->npe, nn, dp<int64_t>(P->adj_ptr),
+                       (int32_t *)nullptr, dp<int32_t>(P->adj), dp<int32_t>(overflow));
+    P->bcol = dev_alloc(P->nblocks * sizeof(int32_t));
+    hipLaunchKernelGGL(k_bcol, dim3(grid_for(nn, 256)), dim3(256), 0, ctx->stream, dp<int64_t>(P->adj_ptr), nn,
+                       dp<int32_t>(P->bcol));
+    FEASM_HIP(hipGetLastError());
+    // 3. contributions
+    DevPtr ccount = dev_alloc((P->nblocks + 1) * sizeof(int32_t));
+    FEASM_HIP(hipMemsetAsync(ccount->p, 0, (P->nblocks + 1) * sizeof(int32_t), ctx->stream));
+    hipLaunchKernelGGL(k_contrib, dim3(grid_for(P->nblocks, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(P->bcol),
+                       dp<int32_t>(P->adj), P->nblocks, dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e),
+                       dp<int32_t>(m->conn), m->npe, (const int64_t *)nullptr, dp<int32_t>(ccount),
+                       (uint32_t *)nullptr);
+    FEASM_HIP(hipGetLastError());
+    P->cptr = dev_alloc((P->nblocks + 1) * sizeof(int64_t));
+    exclusive_scan_i32_to_i64(ctx, dp<int32_t>(ccount), dp<int64_t>(P->cptr), P->nblocks + 1);
+    P->ncontrib = read_i64(ctx, dp<int64_t>(P->cptr) + P->nblocks);
+    P->contrib = dev_alloc(P->ncontrib * sizeof(uint32_t));
+    hipLaunchKernelGGL(k_contrib, dim3(grid_for(P->nblocks, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(P->bcol),
+                       dp<int32_t>(P->adj), P->nblocks, dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e),
+                       dp<int32_t>(m->conn), m->npe, dp<int64_t>(P->cptr), (int32_t *)nullptr,
+                       dp<uint32_t>(P->contrib));
+    FEASM_HIP(hipGetLastError());
+    // 4. DOF-level CSC
+    P->n = nn * ndn;
+    P->nnz = P->nblocks * ndn * ndn;
+    FEASM_REQUIRE(P->n < (1ll << 31), FEASM_EUNSUPPORTED, "more than 2^31 DOFs");
+    P->colptr = dev_alloc((P->n + 1) * sizeof(int64_t));
+    P->rowval = dev_alloc(P->nnz * sizeof(int32_t));
+    hipLaunchKernelGGL(k_dof_pattern, dim3(grid_for(nn, 256)), dim3(256), 0, ctx->stream, dp<int64_t>(P->adj_ptr),
+                       dp<int32_t>(P->adj), nn, ndn, dp<int64_t>(P->colptr), dp<int32_t>(P->rowval));
+    FEASM_HIP(hipGetLastError());
+    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    return P;
+}
+
+}  // namespace feasm

@@ -1,0 +1,212 @@
+"""Thin object layer over the C ABI: device context, resident meshes, matrices.
+
+These objects own library handles (device memory) and release them when
+garbage-collected or closed.
+"""
+import ctypes
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib as L
+from ._lib import check, lib, ptr
+
+_default_ctx = None
+
+
+class Context:
+    """One HIP device + one stream (feasm_ctx)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().feasm_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().feasm_ctx_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @property
+    def stream(self):
+        return lib().feasm_ctx_stream(self.h)
+
+    def synchronize(self):
+        check(lib().feasm_ctx_synchronize(self.h), self.h)
+
+    def mem_info(self):
+        f, t = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().feasm_mem_info(self.h, ctypes.byref(f), ctypes.byref(t)), self.h)
+        return f.value, t.value
+
+    def set_timing(self, slots=64):
+        """Record HIP events around the kernels of the next `slots` numeric
+        assemblies (0 = off)."""
+        check(lib().feasm_ctx_set_timing(self.h, int(slots)), self.h)
+
+    def timings(self, max_calls=4096):
+        """(n, 3) array of [element_ms, gather_ms, sum] per recorded call."""
+        out = np.zeros(3 * max_calls)
+        n = ctypes.c_int()
+        check(lib().feasm_timings(self.h, ptr(out), max_calls, ctypes.byref(n)), self.h)
+        return out[:3 * n.value].reshape(n.value, 3)
+
+    def reset_timings(self):
+        check(lib().feasm_timings_reset(self.h), self.h)
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class DeviceMatrix:
+    """A CSC matrix held in HBM (feasm_matrix)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().feasm_matrix_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @property
+    def dims(self):
+        m, n, z = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib().feasm_matrix_dims(self.h, ctypes.byref(m), ctypes.byref(n), ctypes.byref(z)))
+        return m.value, n.value, z.value
+
+    @property
+    def nnz(self):
+        return self.dims[2]
+
+    def csc_arrays(self, base=0):
+        m, n, nnz = self.dims
+        colptr = np.empty(n + 1, np.int64)
+        rowval = np.empty(nnz, np.int64)
+        nzval = np.empty(nnz, np.float64)
+        check(lib().feasm_matrix_copy_csc(self.h, base, ptr(colptr), ptr(rowval), ptr(nzval)), self.ctx.h)
+        return colptr, rowval, nzval
+
+    def to_scipy(self):
+        m, n, _ = self.dims
+        colptr, rowval, nzval = self.csc_arrays(0)
+        return sp.csc_matrix((nzval, rowval, colptr), shape=(m, n))
+
+    def device_ptrs(self):
+        c, r, v = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().feasm_matrix_device_csc(self.h, ctypes.byref(c), ctypes.byref(r), ctypes.byref(v)))
+        return c.value, r.value, v.value
+
+
+class DeviceMesh:
+    """Connectivity + coordinates + DOF numbering resident in HBM (feasm_mesh).
+
+    conn: (nelem, npe) 1-based node numbers; xyz: (nnodes, 3)."""
+
+    def __init__(self, ctx, etype, conn, xyz):
+        conn = np.ascontiguousarray(conn, dtype=np.int64)
+        xyz = np.asarray(xyz, dtype=np.float64)
+        self.ctx = ctx
+        self.etype = etype
+        self.nelem, self.npe = conn.shape
+        self.nnodes = xyz.shape[0]
+        xyz_cm = np.asfortranarray(xyz)  # Julia's column-major nnodes x 3
+        h = ctypes.c_void_p()
+        check(lib().feasm_mesh_create(ctx.h, etype, self.nelem, self.npe, ptr(conn), self.nnodes,
+                                      xyz_cm.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h)), ctx.h)
+        self.h = h
+        self.ndn = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().feasm_mesh_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_xyz(self, xyz):
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        check(lib().feasm_mesh_set_xyz(self.h, ptr(x)), self.ctx.h)
+
+    def set_dofs(self, ndn, dofnums=None, nalldofs=None):
+        if dofnums is None:
+            check(lib().feasm_dofs_set_nodemajor(self.h, ndn), self.ctx.h)
+        else:
+            d = np.asfortranarray(np.asarray(dofnums, dtype=np.int64).reshape(self.nnodes, ndn))
+            check(lib().feasm_dofs_set(self.h, ndn, d.ctypes.data_as(ctypes.c_void_p), int(nalldofs)),
+                  self.ctx.h)
+        self.ndn = ndn
+
+    def pattern_info(self):
+        nnz, nb = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().feasm_pattern_info(self.h, ctypes.byref(nnz), ctypes.byref(nb)), self.ctx.h)
+        return nnz.value, nb.value
+
+    def build_pattern(self):
+        check(lib().feasm_pattern_build(self.h), self.ctx.h)
+
+    def bilform(self, form, rule, order, coeff, precision=L.FP64, path="auto"):
+        c = np.ascontiguousarray(np.atleast_1d(coeff), dtype=np.float64).ravel()
+        h = ctypes.c_void_p()
+        fn = lib().feasm_bilform_coo if path == "coo" else lib().feasm_bilform
+        check(fn(self.h, form, rule, order, ptr(c), len(c), precision, ctypes.byref(h)), self.ctx.h)
+        return DeviceMatrix(self.ctx, h)
+
+    def bilform_into(self, mat, form, rule, order, coeff, precision=L.FP64):
+        c = np.ascontiguousarray(np.atleast_1d(coeff), dtype=np.float64).ravel()
+        check(lib().feasm_bilform_into(self.h, form, rule, order, ptr(c), len(c), precision, mat.h), self.ctx.h)
+        return mat
+
+
+def sparse_device(ctx, I0, J0, V, m, n):
+    """Device sparse(I, J, V, m, n) on host arrays (0-based indices)."""
+    import torch  # device staging of host arrays for the device entry point
+
+    dev = torch.device("cuda", ctx.device)
+    tI = torch.as_tensor(np.ascontiguousarray(I0, dtype=np.int32), device=dev)
+    tJ = torch.as_tensor(np.ascontiguousarray(J0, dtype=np.int32), device=dev)
+    tV = torch.as_tensor(np.ascontiguousarray(V, dtype=np.float64), device=dev)
+    torch.cuda.synchronize(dev)
+    h = ctypes.c_void_p()
+    check(lib().feasm_sparse_device(ctx.h, len(V), tI.data_ptr(), tJ.data_ptr(), tV.data_ptr(), m, n,
+                                    ctypes.byref(h)), ctx.h)
+    return DeviceMatrix(ctx, h)
+
+
+# ------------------------------------------------------------------ generators
+def gen_h8block(L_, W, H, nL, nW, nH):
+    ne, nn = ctypes.c_int64(), ctypes.c_int64()
+    check(lib().feasm_gen_h8block(L_, W, H, nL, nW, nH, ctypes.byref(ne), ctypes.byref(nn), None, None))
+    conn = np.empty((ne.value, 8), np.int64)
+    xyz = np.empty((nn.value, 3))
+    check(lib().feasm_gen_h8block(L_, W, H, nL, nW, nH, ctypes.byref(ne), ctypes.byref(nn), ptr(conn), ptr(xyz)))
+    return conn, xyz
+
+
+def gen_t10block(L_, W, H, nL, nW, nH, amp_vertex=0.0, amp_mid=0.0, seed=0):
+    ne, nn = ctypes.c_int64(), ctypes.c_int64()
+    check(lib().feasm_gen_t10block(L_, W, H, nL, nW, nH, amp_vertex, amp_mid, seed, ctypes.byref(ne),
+                                   ctypes.byref(nn), None, None))
+    conn = np.empty((ne.value, 10), np.int64)
+    xyz = np.empty((nn.value, 3))
+    check(lib().feasm_gen_t10block(L_, W, H, nL, nW, nH, amp_vertex, amp_mid, seed, ctypes.byref(ne),
+                                   ctypes.byref(nn), ptr(conn), ptr(xyz)))
+    return conn, xyz
+
+
+def gen_perturb(xyz, lo, hi, amp, seed):
+    x = np.ascontiguousarray(xyz, dtype=np.float64).copy()
+    lo = np.ascontiguousarray(lo, dtype=np.float64)
+    hi = np.ascontiguousarray(hi, dtype=np.float64)
+    check(lib().feasm_gen_perturb(x.shape[0], ptr(x), ptr(lo), ptr(hi), amp, seed))
+    return x

@@ -1,0 +1,314 @@
+"""Host-side mirror of FinEtools.jl's assembly interface, backed by libfeasm.
+
+Names, argument meaning and error behaviour follow the reference so that code
+written against it reads the same (Julia's trailing ``!`` cannot appear in
+Python names: ``startassembly!`` is ``startassembly`` and so on):
+
+  FENodeSet, FESetH8, FESetT10      src/FENodeSetModule.jl:26-32, src/FESetModule.jl:953,1393
+  NodalField, numberdofs, nalldofs  src/NodalFieldModule.jl:20-62, src/FieldModule.jl:102,328-377
+  GaussRule, TetRule                src/IntegRuleModule.jl:206-398, 483-516
+  IntegDomain, FEMMBase             src/IntegDomainModule.jl:43-61, src/FEMMBaseModule.jl:72-84
+  DataCache, DeforModelRed3D        src/DataCacheModule.jl:129-175, src/DeforModelRedModule.jl:64
+  SysmatAssemblerSparse             src/AssemblyModule.jl:88-329
+    startassembly, assemble, makematrix, setnomatrixresult, expectedntriples
+  bilform_diffusion                 src/FEMMBaseModule.jl:1462-1545
+  bilform_lin_elastic               src/FEMMBaseModule.jl:1774-1824
+  bilform_dot                       src/FEMMBaseModule.jl:1335-1376
+  H8block, T10block                 src/MeshHexahedronModule.jl:39-52, src/MeshTetrahedronModule.jl:249-261
+
+Matrices come back as scipy.sparse.csc_matrix (0-based).  A bilform_* call
+with a SysmatAssemblerSparse runs the whole element loop on the GPU (the
+device keeps mesh, DOFs and the CSC pattern resident between calls); other
+calls of the generic protocol buffer triples on the host and build the CSC
+with the device sparse().
+"""
+import ctypes
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib as L
+from ._lib import check, lib, ptr
+from .core import DeviceMatrix, DeviceMesh, default_context, gen_h8block, gen_t10block
+
+
+# ------------------------------------------------------------------ mesh data
+class FENodeSet:
+    def __init__(self, xyz):
+        self.xyz = np.array(xyz, dtype=np.float64, copy=True)
+
+    def count(self):
+        return self.xyz.shape[0]
+
+
+class _FESet:
+    NPE = 0
+    ETYPE = -1
+
+    def __init__(self, conn):
+        conn = np.asarray(conn, dtype=np.int64)
+        if conn.ndim != 2 or conn.shape[1] != self.NPE:
+            raise ValueError("Wrong number of nodes per element")
+        self.conn = np.ascontiguousarray(conn)
+        self.label = np.zeros(conn.shape[0], np.int64)
+
+    def count(self):
+        return self.conn.shape[0]
+
+    def nodesperelem(self):
+        return self.NPE
+
+
+class FESetH8(_FESet):
+    NPE = 8
+    ETYPE = L.H8
+
+
+class FESetT10(_FESet):
+    NPE = 10
+    ETYPE = L.T10
+
+
+def H8block(Length, Width, Height, nL, nW, nH):
+    conn, xyz = gen_h8block(float(Length), float(Width), float(Height), int(nL), int(nW), int(nH))
+    return FENodeSet(xyz), FESetH8(conn)
+
+
+def T10block(Length, Width, Height, nL, nW, nH, orientation="a"):
+    if orientation not in ("a", ":a"):
+        raise ValueError("only orientation :a is generated")
+    conn, xyz = gen_t10block(float(Length), float(Width), float(Height), int(nL), int(nW), int(nH))
+    return FENodeSet(xyz), FESetT10(conn)
+
+
+# ------------------------------------------------------------------ fields
+class NodalField:
+    """values (nnodes x ndn); dofnums filled by numberdofs (node-major, free
+    DOFs only -- essential boundary conditions are not part of this path)."""
+
+    def __init__(self, values):
+        v = np.asarray(values, dtype=np.float64)
+        if v.ndim == 1:
+            v = v.reshape(-1, 1)
+        self.values = v.copy()
+        self.dofnums = np.zeros(v.shape, np.int64)
+        self.nfreedofs = 0
+
+    def ndofs(self):
+        return self.values.shape[1]
+
+    def nents(self):
+        return self.values.shape[0]
+
+
+def numberdofs(u, entperm=None):
+    """numberdofs! (src/FieldModule.jl:356-377) with every DOF free: entities in
+    the order entperm (default 1:n), components fastest."""
+    n, dim = u.values.shape
+    order = np.arange(n) if entperm is None else np.asarray(entperm, dtype=np.int64) - 1
+    nxt = 1
+    for i in order:
+        for j in range(dim):
+            u.dofnums[i, j] = nxt
+            nxt += 1
+    u.nfreedofs = nxt - 1
+    return u
+
+
+def nalldofs(u):
+    return int(u.nfreedofs)
+
+
+# ------------------------------------------------------------------ quadrature + FEMM
+class GaussRule:
+    def __init__(self, dim=1, order=1):
+        if not (1 <= dim <= 3):
+            raise ValueError("Gauss rule of dimension %d not available" % dim)
+        if order not in (1, 2, 3, 4):
+            raise ValueError("Gauss rule of order %d not available" % order)
+        self.dim, self.order, self.npts = dim, order, order ** dim
+        self.kind = L.RULE_GAUSS
+
+
+class TetRule:
+    def __init__(self, npts=1):
+        if npts not in (1, 4, 5):
+            raise ValueError("Unknown number of integration points")
+        self.npts = npts
+        self.order = npts
+        self.kind = L.RULE_TET
+
+
+class IntegDomain:
+    def __init__(self, fes, integration_rule, otherdimension=1.0, axisymmetric=False):
+        if axisymmetric:
+            raise NotImplementedError("axisymmetric domains are outside this path")
+        self.fes = fes
+        self.integration_rule = integration_rule
+
+
+class DataCache:
+    """Constant coefficient (scalar, 3x3 or 6x6), src/DataCacheModule.jl:141-153."""
+
+    def __init__(self, data):
+        self.data = np.asarray(data, dtype=np.float64)
+
+    def size(self):
+        return self.data.shape
+
+
+class DeforModelRed3D:
+    nstressstrain = 6
+
+
+class FEMMBase:
+    """Finite element machine.  Keeps the device mesh of the last (geometry,
+    field) it assembled, so repeated bilforms reuse the resident pattern."""
+
+    def __init__(self, integdomain, mcsys=None, ctx=None):
+        if mcsys is not None:
+            raise NotImplementedError("only the identity material frame CSys(3) is on this path")
+        self.integdomain = integdomain
+        self.ctx = ctx
+        self._mesh = None
+        self._key = None
+
+    def finite_elements(self):
+        return self.integdomain.fes
+
+    def _device_mesh(self, geom, u):
+        ctx = self.ctx or default_context()
+        fes = self.integdomain.fes
+        if geom.values.shape[1] != 3:
+            raise ValueError("geometry must be three-dimensional")
+        key = (id(fes.conn), fes.conn.shape, geom.values.shape)
+        if self._mesh is None or self._key != key or not np.array_equal(self._xyz, geom.values) or \
+                not np.array_equal(self._conn, fes.conn):
+            if self._mesh is not None and self._key == key and np.array_equal(self._conn, fes.conn):
+                self._mesh.set_xyz(geom.values)  # same mesh, moved nodes: keep the pattern
+            else:
+                self._mesh = DeviceMesh(ctx, fes.ETYPE, fes.conn, geom.values)
+                self._conn = fes.conn.copy()
+                self._dofs = None
+            self._xyz = geom.values.copy()
+            self._key = key
+        ndn = u.ndofs()
+        if self._dofs is None or not np.array_equal(self._dofs, u.dofnums):
+            if np.array_equal(u.dofnums.ravel(), np.arange(1, u.dofnums.size + 1)) and \
+                    nalldofs(u) == u.dofnums.size:
+                self._mesh.set_dofs(ndn)
+            else:
+                self._mesh.set_dofs(ndn, u.dofnums, nalldofs(u))
+            self._dofs = u.dofnums.copy()
+        return self._mesh
+
+
+# ------------------------------------------------------------------ assembler
+class SysmatAssemblerSparse:
+    """SysmatAssemblerSparse(z=0.0, nomatrixresult=False) on the device.
+
+    The generic protocol (startassembly / assemble / makematrix) buffers the
+    triples and builds the CSC with the device sparse().  Passed to a bilform_*
+    function it instead runs the GPU element loop (then ``precision`` selects
+    FP64 or the mixed FP32/FP64 arithmetic)."""
+
+    def __init__(self, z=0.0, nomatrixresult=False, ctx=None, precision=L.FP64):
+        if not isinstance(z, float):
+            raise TypeError("only Float64 assemblers are supported")
+        self.ctx = ctx or default_context()
+        h = ctypes.c_void_p()
+        check(lib().feasm_coo_create(self.ctx.h, 1 if nomatrixresult else 0, ctypes.byref(h)), self.ctx.h)
+        self.h = h
+        self._nomatrixresult = bool(nomatrixresult)
+        self.precision = precision
+        self._dims = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().feasm_coo_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def eltype(self):
+        return np.float64
+
+
+def setnomatrixresult(a, flag):
+    a._nomatrixresult = bool(flag)
+    check(lib().feasm_coo_set_nomatrixresult(a.h, 1 if flag else 0), a.ctx.h)
+    return a
+
+
+def expectedntriples(a, elem_mat_nrows, elem_mat_ncols, n_elem_mats):
+    return elem_mat_nrows * elem_mat_ncols * n_elem_mats
+
+
+def startassembly(a, elem_mat_nrows, elem_mat_ncols, n_elem_mats, row_nalldofs, col_nalldofs,
+                  force_init=False):
+    check(lib().feasm_coo_start(a.h, elem_mat_nrows, elem_mat_ncols, n_elem_mats, row_nalldofs, col_nalldofs),
+          a.ctx.h)
+    if a._dims is None:
+        a._dims = (int(row_nalldofs), int(col_nalldofs))
+    return a
+
+
+def assemble(a, mat, dofnums_row, dofnums_col):
+    m = np.asfortranarray(np.asarray(mat, dtype=np.float64))
+    if m.ndim == 1:
+        m = m.reshape(-1, 1)
+    r = np.ascontiguousarray(np.ravel(dofnums_row), dtype=np.int64)
+    c = np.ascontiguousarray(np.ravel(dofnums_col), dtype=np.int64)
+    check(lib().feasm_coo_assemble(a.h, m.ctypes.data_as(ctypes.c_void_p), m.shape[0], m.shape[1], ptr(r), len(r),
+                                   ptr(c), len(c)), a.ctx.h)
+    return a
+
+
+def makematrix(a):
+    h = ctypes.c_void_p()
+    check(lib().feasm_coo_makematrix(a.h, ctypes.byref(h)), a.ctx.h)
+    if not h.value:
+        m, n = a._dims if a._dims else (0, 0)
+        return sp.csc_matrix((m, n))
+    return DeviceMatrix(a.ctx, h).to_scipy()
+
+
+# ------------------------------------------------------------------ bilinear forms
+def _rule(femm):
+    r = femm.integdomain.integration_rule
+    fes = femm.integdomain.fes
+    if isinstance(fes, FESetH8) and not isinstance(r, GaussRule):
+        raise ValueError("H8 elements take a Gauss rule")
+    if isinstance(fes, FESetT10) and not isinstance(r, TetRule):
+        raise ValueError("T10 elements take a tetrahedral rule")
+    return r.kind, r.order
+
+
+def _run(femm, assembler, geom, u, form, coeff, return_device=False):
+    if not isinstance(assembler, SysmatAssemblerSparse):
+        raise TypeError("the GPU bilinear forms take a SysmatAssemblerSparse")
+    mesh = femm._device_mesh(geom, u)
+    kind, order = _rule(femm)
+    mat = mesh.bilform(form, kind, order, coeff, precision=assembler.precision)
+    return mat if return_device else mat.to_scipy()
+
+
+def bilform_diffusion(femm, assembler, geom, u, cf, return_device=False):
+    """Diffusion stiffness: scalar DataCache -> isotropic path, 3x3 -> general."""
+    if cf.data.ndim == 0 or cf.data.size == 1:
+        return _run(femm, assembler, geom, u, L.DIFFUSION_ISO, [float(cf.data.ravel()[0])], return_device)
+    return _run(femm, assembler, geom, u, L.DIFFUSION_MAT, cf.data.reshape(3, 3), return_device)
+
+
+def bilform_lin_elastic(femm, assembler, geom, u, mr, cf, return_device=False):
+    if mr is not DeforModelRed3D:
+        raise NotImplementedError("only DeforModelRed3D is on this path")
+    return _run(femm, assembler, geom, u, L.LIN_ELASTIC, cf.data.reshape(6, 6), return_device)
+
+
+def bilform_dot(femm, assembler, geom, u, cf, m=3, return_device=False):
+    if m != 3:
+        raise ValueError("That is the only acceptable option here.")
+    ndn = u.ndofs()
+    c = np.asarray(cf.data, dtype=np.float64).reshape(ndn, ndn)
+    return _run(femm, assembler, geom, u, L.DOT, c, return_device)

@@ -1,0 +1,213 @@
+/*
+ * feasm.h -- C ABI of the MI355X finite-element assembly library (libfeasm.so).
+ *
+ * The library replaces the FE assembly hot path of FinEtools.jl: element
+ * matrices integrated at quadrature points (IntegDomain/IntegRule Jacobians,
+ * FEMMBase bilinear forms for diffusion, linear elasticity and mass) and
+ * scattered through SysmatAssemblerSparse into a CSC matrix.
+ *
+ * Conventions (those of the reference, so that a Julia `ccall` passes its
+ * arrays without copies):
+ *   - node numbers, DOF numbers and CSC indices are 1-based int64 at the ABI;
+ *   - node coordinates are passed as Julia stores FENodeSet.xyz: an
+ *     nnodes x 3 column-major Float64 matrix (xyz[n + nnodes*k]);
+ *   - connectivity is nelem x npe, passed row-major (conn[e*npe + a]), which
+ *     is what `reduce(vcat, fes.conn)`-style packing of the NTuple vector gives;
+ *   - dofnums is NodalField.dofnums: nnodes x ndn column-major Int64;
+ *   - element matrices are column-major (Julia Matrix{Float64}).
+ * Every entry point returns FEASM_OK (0) or a negative status; the message
+ * text of the last failure on a context is returned by feasm_last_error().
+ * Messages for the error cases the reference raises carry the reference's
+ * own text ("Row degree of freedom < 1", "Wrong size of matrix", ...).
+ *
+ * Threading: a context owns one HIP stream and is not thread-safe; use one
+ * context per host thread.  Device memory is owned by the library handles and
+ * released by the matching *_destroy call.
+ */
+#ifndef FEASM_H
+#define FEASM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define FEASM_OK 0
+#define FEASM_EINVAL -1     /* bad argument                                   */
+#define FEASM_EDOF -2       /* DOF number out of 1..n (assemble!)             */
+#define FEASM_ESIZE -3      /* element matrix size mismatch (assemble!)       */
+#define FEASM_EHIP -4       /* HIP runtime failure                            */
+#define FEASM_ENOMEM -5     /* device memory exhausted / admission refused    */
+#define FEASM_ESTATE -6     /* call out of protocol order                     */
+#define FEASM_EUNSUPPORTED -7
+#define FEASM_ECANCELLED -8 /* serving: request cancelled                     */
+
+/* ---- enumerations ------------------------------------------------------- */
+/* element types: FESetH8 (src/FESetModule.jl:953), FESetT10 (:1393) */
+#define FEASM_H8 0
+#define FEASM_T10 1
+/* bilinear forms (src/FEMMBaseModule.jl):
+ *   DIFFUSION_ISO  bilform_diffusion, scalar DataCache   (:1462-1474, :1508-1535)
+ *   DIFFUSION_MAT  bilform_diffusion, 3x3 DataCache      (:1476-1506)
+ *   LIN_ELASTIC    bilform_lin_elastic, DeforModelRed3D  (:1774-1813)
+ *   DOT            bilform_dot with m = 3                (:1335-1366)        */
+#define FEASM_FORM_DIFFUSION_ISO 0
+#define FEASM_FORM_DIFFUSION_MAT 1
+#define FEASM_FORM_LIN_ELASTIC 2
+#define FEASM_FORM_DOT 3
+/* integration rules: GaussRule(3, order) (src/IntegRuleModule.jl:206-398),
+ * TetRule(npts) (src/IntegRuleModule.jl:483-516) */
+#define FEASM_RULE_GAUSS 0
+#define FEASM_RULE_TET 1
+/* arithmetic: FP64 everywhere, or FP32 element arithmetic with FP64 Jacobian
+ * determinant/inverse and FP64 global summation */
+#define FEASM_FP64 0
+#define FEASM_MIXED 1
+
+typedef struct feasm_ctx feasm_ctx;
+typedef struct feasm_mesh feasm_mesh;
+typedef struct feasm_matrix feasm_matrix;
+typedef struct feasm_coo feasm_coo;
+
+/* ---- context ------------------------------------------------------------ */
+/* No reference counterpart (the reference is host-only Julia): owns the HIP
+ * device selection, one stream and the last-error text. */
+int feasm_ctx_create(int device, feasm_ctx **out);
+int feasm_ctx_destroy(feasm_ctx *ctx);
+const char *feasm_last_error(const feasm_ctx *ctx);
+/* hipStream_t of the context, as an opaque pointer (for event timing). */
+void *feasm_ctx_stream(feasm_ctx *ctx);
+int feasm_ctx_synchronize(feasm_ctx *ctx);
+/* Free and total device memory in bytes (admission control). */
+int feasm_mem_info(feasm_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
+/* Per-kernel timing with HIP events on the context stream.  slots > 0 turns
+ * it on with a ring of `slots` numeric assemblies (each records an event
+ * before the element kernel, between the kernels and after the gather);
+ * 0 turns it off. */
+int feasm_ctx_set_timing(feasm_ctx *ctx, int slots);
+/* Durations (ms) of the recorded numeric assemblies, oldest first:
+ * out[3i] element kernel, out[3i+1] gather kernel, out[3i+2] their sum.
+ * Waits for the recorded events. */
+int feasm_timings(feasm_ctx *ctx, double *out, int max_calls, int *ncalls);
+int feasm_timings_reset(feasm_ctx *ctx);
+
+/* ---- mesh + DOF numbering (device resident) ----------------------------- */
+/* Replaces what FENodeSet/FESetH8/FESetT10 hold (src/FENodeSetModule.jl:26-32,
+ * src/FESetModule.jl:57-93) and the geometry NodalField (src/NodalFieldModule.jl:
+ * 34-44): copies conn (1-based) and xyz to the device. */
+int feasm_mesh_create(feasm_ctx *ctx, int elem_type, int64_t nelem, int npe, const int64_t *conn,
+                      int64_t nnodes, const double *xyz_colmajor, feasm_mesh **out);
+/* Same, from device memory already resident in HBM: conn0 int32 0-based
+ * row-major (nelem x npe), xyz row-major (nnodes x 3).  The mesh copies them. */
+int feasm_mesh_create_device(feasm_ctx *ctx, int elem_type, int64_t nelem, int npe, const int32_t *d_conn0,
+                             int64_t nnodes, const double *d_xyz, feasm_mesh **out);
+int feasm_mesh_destroy(feasm_mesh *mesh);
+/* Update node coordinates in place (row-major nnodes x 3, host memory). */
+int feasm_mesh_set_xyz(feasm_mesh *mesh, const double *xyz_rowmajor);
+/* numberdofs!/gatherdofnums! (src/FieldModule.jl:304-377): dofnums nnodes x ndn
+ * column-major, 1-based; nalldofs (src/FieldModule.jl:102).  When dofnums is
+ * the node-major numbering (node-1)*ndn + comp, assembly uses the device
+ * resident CSC pattern; otherwise it goes through the generic COO path. */
+int feasm_dofs_set(feasm_mesh *mesh, int ndn, const int64_t *dofnums_colmajor, int64_t nalldofs);
+/* Node-major numbering without passing the table (numberdofs! with no EBCs). */
+int feasm_dofs_set_nodemajor(feasm_mesh *mesh, int ndn);
+
+/* ---- bilinear forms (GPU element loop + assembly) ----------------------- */
+/* One call = the whole bilform_* loop followed by makematrix! on a
+ * SysmatAssemblerSparse(0.0): bilform_diffusion / bilform_lin_elastic /
+ * bilform_dot (src/FEMMBaseModule.jl:1335-1366, 1462-1545, 1774-1813).
+ * coeff: DIFFUSION_ISO 1 value (k); DIFFUSION_MAT 9 (3x3 row-major);
+ * LIN_ELASTIC 36 (6x6 row-major D); DOT ndn*ndn (row-major c).
+ * rule/order: FEASM_RULE_GAUSS with order 1..4 (H8), FEASM_RULE_TET with 1, 4
+ * or 5 points (T10).  The first call for a (mesh, ndn) builds the CSC pattern
+ * on the device (symbolic phase); later calls reuse it (numeric phase only).
+ * *out receives a new matrix handle. */
+int feasm_bilform(feasm_mesh *mesh, int form, int rule, int order, const double *coeff, int ncoeff, int precision,
+                  feasm_matrix **out);
+/* Re-assembly into an existing matrix built for the same mesh, DOFs and form
+ * class (values overwritten; pattern reused; no device allocation when the
+ * staging buffers already exist). */
+int feasm_bilform_into(feasm_mesh *mesh, int form, int rule, int order, const double *coeff, int ncoeff,
+                       int precision, feasm_matrix *mat);
+/* Generic path used for non-node-major DOF numberings, and selectable for
+ * testing: element kernels write COO triples in assemble!'s order, then the
+ * device sparse() (radix sort + ordered duplicate summation). */
+int feasm_bilform_coo(feasm_mesh *mesh, int form, int rule, int order, const double *coeff, int ncoeff,
+                      int precision, feasm_matrix **out);
+/* Build (or rebuild) the device CSC pattern for the mesh's DOFs now. */
+int feasm_pattern_build(feasm_mesh *mesh);
+/* Pattern statistics: nnz of the DOF-level CSC and number of node blocks. */
+int feasm_pattern_info(feasm_mesh *mesh, int64_t *nnz, int64_t *nblocks);
+
+/* ---- matrices ----------------------------------------------------------- */
+/* makematrix! result (src/AssemblyModule.jl:304-329): SparseMatrixCSC{Float64,Int}. */
+int feasm_matrix_dims(const feasm_matrix *mat, int64_t *m, int64_t *n, int64_t *nnz);
+/* Copy to caller arrays (colptr n+1, rowval nnz, nzval nnz); base 1 gives the
+ * Julia SparseMatrixCSC convention, base 0 the scipy one. */
+int feasm_matrix_copy_csc(feasm_matrix *mat, int base, int64_t *colptr, int64_t *rowval, double *nzval);
+/* Device pointers: colptr int64 (0-based, n+1), rowval int32 (0-based),
+ * nzval double.  Valid until the matrix is destroyed. */
+int feasm_matrix_device_csc(feasm_matrix *mat, const int64_t **colptr, const int32_t **rowval,
+                            double **nzval);
+int feasm_matrix_destroy(feasm_matrix *mat);
+
+/* ---- generic assembler protocol (SysmatAssemblerSparse) ----------------- */
+/* SysmatAssemblerSparse(0.0, nomatrixresult) (src/AssemblyModule.jl:159-165).
+ * ctx may be NULL for host-side buffering only (makematrix! then fails). */
+int feasm_coo_create(feasm_ctx *ctx, int nomatrixresult, feasm_coo **out);
+/* startassembly! (src/AssemblyModule.jl:209-238): on a fresh assembler the
+ * buffers are sized for elem_nrows*elem_ncols*n_elem_mats triples and the
+ * matrix dimensions recorded; on a used one both are left as they are and
+ * only the buffer pointer is reset.  assemble! grows the buffers by
+ * elem_nrows*elem_ncols*1000 triples when full (_resize_buffers, :61-76). */
+int feasm_coo_start(feasm_coo *coo, int64_t elem_nrows, int64_t elem_ncols, int64_t n_elem_mats,
+                    int64_t row_nalldofs, int64_t col_nalldofs);
+/* assemble! (src/AssemblyModule.jl:250-282): mat is nrows x ncols column-major;
+ * rows/cols are 1-based; serialized column-major.  Errors exactly as the
+ * reference: "Wrong size of matrix", "Column degree of freedom < 1", ... */
+int feasm_coo_assemble(feasm_coo *coo, const double *mat, int64_t mat_nrows, int64_t mat_ncols,
+                       const int64_t *rows, int64_t nrows, const int64_t *cols, int64_t ncols);
+/* Many same-size element matrices at once: mats[e] column-major, rows/cols
+ * nmats x nrows and nmats x ncols row-major. */
+int feasm_coo_assemble_many(feasm_coo *coo, int64_t nmats, const double *mats, const int64_t *rows,
+                            int64_t nrows, const int64_t *cols, int64_t ncols);
+/* setnomatrixresult (src/AssemblyModule.jl:29-33). */
+int feasm_coo_set_nomatrixresult(feasm_coo *coo, int flag);
+/* makematrix! (src/AssemblyModule.jl:304-329): *out = NULL when
+ * nomatrixresult is set (the reference returns spzeros and keeps the
+ * triples); otherwise the device sparse() runs and the pointer resets to 1. */
+int feasm_coo_makematrix(feasm_coo *coo, feasm_matrix **out);
+/* number of triples currently buffered (buffer pointer - 1) */
+int64_t feasm_coo_ntriples(const feasm_coo *coo);
+int feasm_coo_destroy(feasm_coo *coo);
+/* Device sparse(I, J, V, m, n) on device arrays (0-based int32 indices):
+ * stable radix sort by (col, row), duplicates summed left to right in input
+ * order, explicit zeros kept -- SparseArrays.sparse as called by makematrix!. */
+int feasm_sparse_device(feasm_ctx *ctx, int64_t ntrip, const int32_t *d_I, const int32_t *d_J, const double *d_V,
+                        int64_t m, int64_t n, feasm_matrix **out);
+
+/* ---- mesh generators (src/MeshHexahedronModule.jl:39-112,
+ *      src/MeshTetrahedronModule.jl:53-261) ------------------------------- */
+/* H8blockx on the box [0,L]x[0,W]x[0,H] with nL x nW x nH cells: node i along
+ * an axis at coordinate L*i/nL.  Writes 1-based conn (row-major nelem x 8)
+ * and xyz (row-major nnodes x 3); pass NULL arrays to query the sizes. */
+int feasm_gen_h8block(double L, double W, double H, int64_t nL, int64_t nW, int64_t nH, int64_t *nelem,
+                      int64_t *nnodes, int64_t *conn, double *xyz);
+/* T10block(..., orientation = :a): T4blockx + T4toT10 (first-encounter
+ * mid-edge numbering).  amp_vertex / amp_mid: seeded perturbation amplitudes
+ * applied to interior vertex nodes before the mid-edge nodes are placed and
+ * to interior mid-edge nodes afterwards (0 = none). */
+int feasm_gen_t10block(double L, double W, double H, int64_t nL, int64_t nW, int64_t nH, double amp_vertex,
+                       double amp_mid, uint64_t seed, int64_t *nelem, int64_t *nnodes, int64_t *conn, double *xyz);
+/* Seeded perturbation of nodes strictly inside [lo, hi] (row-major xyz):
+ * component c of node n moves by u*amp, u in [-1,1) from SplitMix64 at
+ * counter 3n+c+1 of stream `seed`. */
+int feasm_gen_perturb(int64_t nnodes, double *xyz, const double lo[3], const double hi[3], double amp,
+                      uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEASM_H */

@@ -1,0 +1,256 @@
+"""GPU parity: the HIP assembly against the oracle on the same seeded meshes.
+
+Bar: colptr/rowval bit-exact; FP64 nzval within 1e-12 relative Frobenius;
+mixed precision within 1e-6.  Every case runs through the C ABI.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import feasm
+from feasm import _lib as L
+from feasm import core
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+MIXED_TOL = 1e-6
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_fro(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb else 1.0)
+
+
+def mesh_for(etype, n, amp, seed):
+    if etype == O.H8:
+        conn, xyz = core.gen_h8block(1.0, 1.0, 1.0, *n)
+        if amp:
+            xyz = core.gen_perturb(xyz, [0, 0, 0], [1, 1, 1], amp / max(n), seed)
+    else:
+        h = 1.0 / max(n)
+        conn, xyz = core.gen_t10block(1.0, 1.0, 1.0, *n, amp * h, 0.3 * amp * h, seed)
+    return conn, xyz
+
+
+def rule_for(etype):
+    return (L.RULE_GAUSS, 2) if etype == O.H8 else (L.RULE_TET, 4)
+
+
+D200 = O.iso_D(200e3, 0.3)
+D70 = O.iso_D(70e3, 0.33)
+_rng = np.random.default_rng(5)
+_A = _rng.normal(size=(3, 3))
+KAPPA = _A @ _A.T + np.eye(3)
+_B = _rng.normal(size=(6, 6))
+DANISO = _B @ _B.T + 6 * np.eye(6)
+
+CASES = [
+    # name, etype, cells, amp, form, ndn, coeff
+    ("C1 H8 diffusion 10^3", O.H8, (10, 10, 10), 0.1, L.DIFFUSION_ISO, 1, [1.0]),
+    ("H8 elasticity", O.H8, (6, 5, 4), 0.15, L.LIN_ELASTIC, 3, D200),
+    ("H8 anisotropic elasticity", O.H8, (3, 4, 3), 0.15, L.LIN_ELASTIC, 3, DANISO),
+    ("H8 matrix diffusion", O.H8, (5, 4, 3), 0.1, L.DIFFUSION_MAT, 1, KAPPA),
+    ("H8 mass 3", O.H8, (4, 4, 4), 0.1, L.DOT, 3, 7.8e-9 * np.eye(3)),
+    ("H8 dot 1", O.H8, (4, 3, 5), 0.1, L.DOT, 1, [2.5]),
+    ("T10 elasticity", O.T10, (3, 3, 2), 0.1, L.LIN_ELASTIC, 3, D70),
+    ("T10 mass 3", O.T10, (3, 2, 3), 0.1, L.DOT, 3, 2.7e-9 * np.eye(3)),
+    ("T10 diffusion", O.T10, (3, 3, 3), 0.1, L.DIFFUSION_ISO, 1, [1.7]),
+    ("H8 single element", O.H8, (1, 1, 1), 0.0, L.LIN_ELASTIC, 3, D200),
+]
+
+
+def oracle_csc(etype, form, conn, xyz, ndn, coeff):
+    return O.assemble_csc(etype, form, conn, xyz, ndn, np.ravel(coeff))
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("path", ["resident", "coo"])
+def test_fp64_parity(ctx, case, path):
+    name, etype, n, amp, form, ndn, coeff = case
+    say("case", name, path)
+    conn, xyz = mesh_for(etype, n, amp, 1234)
+    ocp, orv, onz = oracle_csc(etype, form, conn, xyz, ndn, coeff)
+    mesh = core.DeviceMesh(ctx, etype, conn, xyz)
+    mesh.set_dofs(ndn)
+    rule, order = rule_for(etype)
+    M = mesh.bilform(form, rule, order, np.ravel(coeff), precision=L.FP64, path=path)
+    cp, rv, nz = M.csc_arrays(base=1)
+    assert np.array_equal(cp, ocp)
+    assert np.array_equal(rv, orv)
+    err = rel_fro(nz, onz)
+    say("  rel Frobenius", err)
+    assert err <= FP64_TOL
+    if form != L.DOT or np.allclose(np.reshape(coeff, (ndn, ndn)), np.reshape(coeff, (ndn, ndn)).T):
+        import scipy.sparse as sp
+        K = sp.csc_matrix((nz, rv - 1, cp - 1), shape=(len(cp) - 1,) * 2)
+        assert (K - K.T).count_nonzero() == 0  # bit-exact symmetry (test/test_forms.jl:442)
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c[1] == O.T10 or c[4] == L.LIN_ELASTIC],
+                         ids=lambda c: c[0])
+def test_mixed_precision_error(ctx, case):
+    name, etype, n, amp, form, ndn, coeff = case
+    say("mixed", name)
+    conn, xyz = mesh_for(etype, n, amp, 99)
+    ocp, orv, onz = oracle_csc(etype, form, conn, xyz, ndn, coeff)
+    mesh = core.DeviceMesh(ctx, etype, conn, xyz)
+    mesh.set_dofs(ndn)
+    rule, order = rule_for(etype)
+    M = mesh.bilform(form, rule, order, np.ravel(coeff), precision=L.MIXED)
+    cp, rv, nz = M.csc_arrays(base=1)
+    assert np.array_equal(cp, ocp) and np.array_equal(rv, orv)
+    err = rel_fro(nz, onz)
+    say("  mixed rel Frobenius", err)
+    assert err <= MIXED_TOL
+
+
+def test_reassembly_reuses_pattern_and_is_deterministic(ctx):
+    conn, xyz = mesh_for(O.H8, (5, 5, 5), 0.15, 7)
+    mesh = core.DeviceMesh(ctx, O.H8, conn, xyz)
+    mesh.set_dofs(3)
+    M = mesh.bilform(L.LIN_ELASTIC, L.RULE_GAUSS, 2, D200)
+    a = M.csc_arrays(1)
+    mesh.bilform_into(M, L.LIN_ELASTIC, L.RULE_GAUSS, 2, D70)
+    b = M.csc_arrays(1)
+    ocp, orv, onz = oracle_csc(O.H8, L.LIN_ELASTIC, conn, xyz, 3, D70)
+    assert np.array_equal(b[0], ocp) and np.array_equal(b[1], orv)
+    assert rel_fro(b[2], onz) <= FP64_TOL
+    mesh.bilform_into(M, L.LIN_ELASTIC, L.RULE_GAUSS, 2, D200)
+    c = M.csc_arrays(1)
+    assert np.array_equal(a[2], c[2])  # bitwise repeatable (test/test_basics.jl:3016-3051)
+
+
+def test_general_dof_numbering_goes_through_coo(ctx):
+    conn, xyz = mesh_for(O.H8, (3, 3, 3), 0.1, 3)
+    nn = xyz.shape[0]
+    rng = np.random.default_rng(0)
+    perm = rng.permutation(nn)
+    dof = np.zeros((nn, 3), np.int64)
+    nxt = 1
+    for i in perm:  # numberdofs!(u, entperm)
+        dof[i] = [nxt, nxt + 1, nxt + 2]
+        nxt += 3
+    I, J, V = O.bilform_coo(O.H8, O.LIN_ELASTIC, conn, xyz, 3, dof, 3 * nn, D200.ravel())
+    ocp, orv, onz = O.sparse(I, J, V, 3 * nn, 3 * nn)
+    mesh = core.DeviceMesh(ctx, O.H8, conn, xyz)
+    mesh.set_dofs(3, dof, 3 * nn)
+    M = mesh.bilform(L.LIN_ELASTIC, L.RULE_GAUSS, 2, D200)
+    cp, rv, nz = M.csc_arrays(1)
+    assert np.array_equal(cp, ocp) and np.array_equal(rv, orv)
+    assert rel_fro(nz, onz) <= FP64_TOL
+
+
+def test_sparse_device_order_and_zeros(ctx):
+    # left fold in input order: (1e16 + 1) - 1e16 == 0; explicit zeros kept
+    M = core.sparse_device(ctx, np.array([1, 1, 1, 0]), np.array([0, 0, 0, 2]),
+                           np.array([1e16, 1.0, -1e16, 0.0]), 3, 3)
+    cp, rv, nz = M.csc_arrays(1)
+    assert cp.tolist() == [1, 2, 2, 3] and rv.tolist() == [2, 1]
+    assert nz[0] == 0.0 and nz[1] == 0.0
+    rng = np.random.default_rng(2)
+    m, n, t = 101, 77, 50000
+    I = rng.integers(1, m + 1, t)
+    J = rng.integers(1, n + 1, t)
+    V = rng.normal(size=t)
+    ocp, orv, onz = O.sparse(I, J, V, m, n)
+    M = core.sparse_device(ctx, I - 1, J - 1, V, m, n)
+    cp, rv, nz = M.csc_arrays(1)
+    assert np.array_equal(cp, ocp) and np.array_equal(rv, orv)
+    assert np.array_equal(nz, onz)  # same additions in the same order: bitwise
+
+
+def test_finetools_mirror_protocol_kats(ctx):
+    from feasm.finetools import SysmatAssemblerSparse, assemble, makematrix, setnomatrixresult, startassembly
+
+    m1 = np.array([[0.24406, 0.599773, 0.833404, 0.0420141],
+                   [0.786024, 0.00206713, 0.995379, 0.780298],
+                   [0.845816, 0.198459, 0.355149, 0.224996]])
+    m2 = np.array([[0.146618, 0.53471, 0.614342, 0.737833],
+                   [0.479719, 0.41354, 0.00760941, 0.836455],
+                   [0.254868, 0.476189, 0.460794, 0.00919633],
+                   [0.159064, 0.261821, 0.317078, 0.77646],
+                   [0.643538, 0.429817, 0.59788, 0.958909]])
+    # test/test_basics.jl:1577-1613 (massnomat1)
+    a = SysmatAssemblerSparse(0.0, True, ctx=ctx)
+    startassembly(a, 5, 5, 3, 7, 7)
+    assemble(a, m1, [1, 7, 5], [5, 2, 1, 4])
+    assemble(a, m2, [2, 3, 1, 7, 5], [6, 7, 3, 4])
+    A = makematrix(a)
+    assert A.nnz == 0 and A.shape == (7, 7)
+    setnomatrixresult(a, False)
+    A = makematrix(a).toarray()
+    for (i, j, v) in ((1, 1, 0.833404), (5, 1, 0.355149), (7, 6, 0.159064), (3, 7, 0.41354), (7, 7, 0.261821)):
+        assert abs(A[i - 1, j - 1] - v) < 1e-12
+    # test/test_basics.jl:78-133 (mmassembly2ya1)
+    testA = np.array([
+        [2.85928, 1.21875, 0.891063, 0.891614, 2.56958, 0.0, 0.0],
+        [1.21875, 1.15515, 0.716396, 0.0714644, 1.56825, 0.0, 0.0],
+        [0.891063, 0.716396, 0.936979, 0.0, 1.36026, 0.0, 0.0],
+        [0.891614, 0.0714644, 0.0, 0.661253, 0.813892, 0.0, 0.0],
+        [2.56958, 1.56825, 1.36026, 0.813892, 4.15934, 0.0, 0.0],
+        [0.0] * 7, [0.0] * 7])
+    a = SysmatAssemblerSparse(0.0, ctx=ctx)
+    startassembly(a, 5, 5, 3, 7, 7)
+    assemble(a, m1.T @ m1, [5, 2, 1, 4], [5, 2, 1, 4])
+    assemble(a, m2.T @ m2, [2, 3, 1, 5], [2, 3, 1, 5])
+    assert np.abs(makematrix(a).toarray() - testA).max() < 1e-5
+
+
+def test_finetools_mirror_bilforms(ctx):
+    from feasm.finetools import (DataCache, DeforModelRed3D, FEMMBase, GaussRule, H8block, IntegDomain, NodalField,
+                                 SysmatAssemblerSparse, bilform_diffusion, bilform_dot, bilform_lin_elastic,
+                                 numberdofs)
+
+    # test/test_forms.jl:331-367 patch test and test/test_basics.jl:962-1020 Gram
+    W, Lx, t = 11.1, 12.0, 7.32
+    fens, fes = H8block(Lx, W, t, 2, 4, 3)
+    geom = NodalField(fens.xyz)
+    femm = FEMMBase(IntegDomain(fes, GaussRule(3, 2)), ctx=ctx)
+    a_, b_, c_, d_ = (-0.33, 2 / 3, -1.67, 2 / 7)
+    X, Y, Z = fens.xyz.T
+    u = NodalField(np.column_stack([a_ + b_ * X + c_ * Y + d_ * Z, b_ + c_ * X + d_ * Y + a_ * Z,
+                                    c_ + d_ * X + a_ * Y + b_ * Z]))
+    numberdofs(u)
+    mu = 0.13377
+    G = bilform_lin_elastic(femm, SysmatAssemblerSparse(0.0, ctx=ctx), geom, u, DeforModelRed3D,
+                            DataCache(np.diag([2 * mu] * 3 + [mu] * 3)))
+    gradu = np.array([[b_, c_, d_], [c_, d_, a_], [d_, a_, b_]])
+    sym = (gradu + gradu.T) / 2
+    exact = 2 * mu * (W * Lx * t) * (sym ** 2).sum()
+    v = u.values.ravel()
+    assert abs(v @ (G @ v) - exact) / exact <= 1e-5
+    assert (G - G.T).count_nonzero() == 0
+    psi = NodalField(np.ones((fens.count(), 1)))
+    numberdofs(psi)
+    Gm = bilform_dot(femm, SysmatAssemblerSparse(0.0, ctx=ctx), geom, psi, DataCache(np.eye(1)))
+    one = np.ones(fens.count())
+    assert abs(one @ (Gm @ one) - W * Lx * t) / (W * Lx * t) <= 1e-5
+    K = bilform_diffusion(femm, SysmatAssemblerSparse(0.0, ctx=ctx), geom, psi, DataCache(1.0))
+    assert abs(one @ (K @ one)) <= 1e-10
+
+
+def test_errors_surface_with_reference_text(ctx):
+    conn, xyz = mesh_for(O.H8, (2, 2, 2), 0.0, 0)
+    mesh = core.DeviceMesh(ctx, O.H8, conn, xyz)
+    nn = xyz.shape[0]
+    dof = np.arange(1, nn + 1).reshape(nn, 1)
+    dof[4, 0] = 0
+    with pytest.raises(feasm.FeasmError, match="degree of freedom < 1"):
+        mesh.set_dofs(1, dof, nn)
+    mesh.set_dofs(1)
+    with pytest.raises(feasm.FeasmError, match="Wrong dimensions"):
+        mesh.bilform(L.LIN_ELASTIC, L.RULE_GAUSS, 2, D200)
+    with pytest.raises(feasm.FeasmError, match="Gauss"):
+        mesh.bilform(L.DIFFUSION_ISO, L.RULE_TET, 4, [1.0])
+    bad = np.array([[1, 2, 3, 4, 5, 6, 7, nn + 5]], np.int64)
+    with pytest.raises(feasm.FeasmError, match="missing node"):
+        core.DeviceMesh(ctx, O.H8, bad, xyz)
