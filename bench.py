@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--cells", type=int, default=100, help="cells per direction of the H8 block")
     ap.add_argument("--cpu-cells", type=int, default=50, help="cells per direction of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a 1 GiB streaming copy (3 launches) to scale traffic counters")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X; "
@@ -87,6 +89,14 @@ def main():
     n = a.cells
     D = iso_D(200e3, 0.3)
     ctx = feasm.Context(0)
+    copy_gbs = None
+    if a.calibrate:
+        import ctypes
+
+        ms = ctypes.c_double()
+        feasm._lib.check(feasm.lib().feasm_copy_calibration(ctx.h, 1 << 30, 3, ctypes.byref(ms)), ctx.h)
+        copy_gbs = 2 * (1 << 30) / (ms.value * 1e-3) / 1e9
+        log("copy calibration: 1 GiB in %.3f ms = %.0f GB/s (read + write)" % (ms.value, copy_gbs))
     conn, xyz = core.gen_h8block(1.0, 1.0, 1.0, n, n, n)
     xyz = core.gen_perturb(xyz, [0, 0, 0], [1, 1, 1], 0.15 / n, 2)
     ne, nn = conn.shape[0], xyz.shape[0]
@@ -152,6 +162,8 @@ def main():
                      "algorithmic_bytes": bytes_alg},
         "served": None,
     }
+    if copy_gbs is not None:
+        out["copy_gbs_measured"] = copy_gbs
     del M
     mesh.close()
     if not a.no_cpu:

@@ -610,6 +610,30 @@ extern "C" int feasm_sparse_device(feasm_ctx *ctx, int64_t ntrip, const int32_t 
     API_CATCH(ctx)
 }
 
+extern "C" int feasm_sparse(feasm_ctx *ctx, int64_t ntrip, const int64_t *I, const int64_t *J, const double *V,
+                            int64_t m, int64_t n, feasm_matrix **out) {
+    API_TRY
+    FEASM_REQUIRE(ctx && out && (ntrip == 0 || (I && J && V)), FEASM_EINVAL, "null argument");
+    FEASM_REQUIRE(m >= 0 && n >= 0 && m < (1ll << 31) && n < (1ll << 31), FEASM_EUNSUPPORTED,
+                  "more than 2^31 rows or columns");
+    std::vector<int32_t> i0(ntrip), j0(ntrip);
+    for (int64_t t = 0; t < ntrip; t++) {
+        FEASM_REQUIRE(I[t] >= 1 && I[t] <= m, FEASM_EDOF, "row index out of range");
+        FEASM_REQUIRE(J[t] >= 1 && J[t] <= n, FEASM_EDOF, "column index out of range");
+        i0[t] = (int32_t)(I[t] - 1);
+        j0[t] = (int32_t)(J[t] - 1);
+    }
+    DevPtr dI = dev_alloc(std::max<size_t>(ntrip * 4, 4)), dJ = dev_alloc(std::max<size_t>(ntrip * 4, 4)),
+           dV = dev_alloc(std::max<size_t>(ntrip * 8, 8));
+    if (ntrip) {
+        FEASM_HIP(hipMemcpy(dI->p, i0.data(), ntrip * 4, hipMemcpyHostToDevice));
+        FEASM_HIP(hipMemcpy(dJ->p, j0.data(), ntrip * 4, hipMemcpyHostToDevice));
+        FEASM_HIP(hipMemcpy(dV->p, V, ntrip * 8, hipMemcpyHostToDevice));
+    }
+    *out = sparse_device(ctx, ntrip, dp<int32_t>(dI), dp<int32_t>(dJ), dp<double>(dV), m, n);
+    API_CATCH(ctx)
+}
+
 // ======================================================================= generators
 extern "C" int feasm_gen_h8block(double L, double W, double H, int64_t nL, int64_t nW, int64_t nH, int64_t *nelem,
                                  int64_t *nnodes, int64_t *conn, double *xyz) {

@@ -31,6 +31,7 @@ SIGNATURES = {
     "feasm_ctx_set_timing": (I32, [P, I32]),
     "feasm_timings": (I32, [P, P, I32, ctypes.POINTER(ctypes.c_int)]),
     "feasm_timings_reset": (I32, [P]),
+    "feasm_copy_calibration": (I32, [P, I64, I32, ctypes.POINTER(ctypes.c_double)]),
     "feasm_mesh_create": (I32, [P, I32, I64, I32, P, I64, P, PP]),
     "feasm_mesh_create_device": (I32, [P, I32, I64, I32, P, I64, P, PP]),
     "feasm_mesh_destroy": (I32, [P]),
@@ -55,6 +56,7 @@ SIGNATURES = {
     "feasm_coo_ntriples": (I64, [P]),
     "feasm_coo_destroy": (I32, [P]),
     "feasm_sparse_device": (I32, [P, I64, P, P, P, I64, I64, PP]),
+    "feasm_sparse": (I32, [P, I64, P, P, P, I64, I64, PP]),
     "feasm_gen_h8block": (I32, [DBL, DBL, DBL, I64, I64, I64, PI64, PI64, P, P]),
     "feasm_gen_t10block": (I32, [DBL, DBL, DBL, I64, I64, I64, DBL, DBL, U64, PI64, PI64, P, P]),
     "feasm_gen_perturb": (I32, [I64, P, P, P, DBL, U64]),

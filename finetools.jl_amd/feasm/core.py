@@ -168,18 +168,13 @@ class DeviceMesh:
         return mat
 
 
-def sparse_device(ctx, I0, J0, V, m, n):
-    """Device sparse(I, J, V, m, n) on host arrays (0-based indices)."""
-    import torch  # device staging of host arrays for the device entry point
-
-    dev = torch.device("cuda", ctx.device)
-    tI = torch.as_tensor(np.ascontiguousarray(I0, dtype=np.int32), device=dev)
-    tJ = torch.as_tensor(np.ascontiguousarray(J0, dtype=np.int32), device=dev)
-    tV = torch.as_tensor(np.ascontiguousarray(V, dtype=np.float64), device=dev)
-    torch.cuda.synchronize(dev)
+def sparse(ctx, I, J, V, m, n):
+    """Device sparse(I, J, V, m, n) of host triples with 1-based indices."""
+    I = np.ascontiguousarray(I, dtype=np.int64)
+    J = np.ascontiguousarray(J, dtype=np.int64)
+    V = np.ascontiguousarray(V, dtype=np.float64)
     h = ctypes.c_void_p()
-    check(lib().feasm_sparse_device(ctx.h, len(V), tI.data_ptr(), tJ.data_ptr(), tV.data_ptr(), m, n,
-                                    ctypes.byref(h)), ctx.h)
+    check(lib().feasm_sparse(ctx.h, len(V), ptr(I), ptr(J), ptr(V), m, n, ctypes.byref(h)), ctx.h)
     return DeviceMatrix(ctx, h)
 
 

@@ -93,6 +93,11 @@ int feasm_ctx_set_timing(feasm_ctx *ctx, int slots);
 int feasm_timings(feasm_ctx *ctx, double *out, int max_calls, int *ncalls);
 int feasm_timings_reset(feasm_ctx *ctx);
 
+/* Measurement helper (no reference counterpart): `repeats` launches of a
+ * streaming copy of nbytes; *ms_per_copy = average duration.  Used to scale
+ * hardware traffic counters to bytes. */
+int feasm_copy_calibration(feasm_ctx *ctx, int64_t nbytes, int repeats, double *ms_per_copy);
+
 /* ---- mesh + DOF numbering (device resident) ----------------------------- */
 /* Replaces what FENodeSet/FESetH8/FESetT10 hold (src/FENodeSetModule.jl:26-32,
  * src/FESetModule.jl:57-93) and the geometry NodalField (src/NodalFieldModule.jl:
@@ -187,6 +192,12 @@ int feasm_coo_destroy(feasm_coo *coo);
  * order, explicit zeros kept -- SparseArrays.sparse as called by makematrix!. */
 int feasm_sparse_device(feasm_ctx *ctx, int64_t ntrip, const int32_t *d_I, const int32_t *d_J, const double *d_V,
                         int64_t m, int64_t n, feasm_matrix **out);
+
+/* sparse(I, J, V, m, n) on host triples (1-based int64 indices, as Julia's
+ * SparseArrays.sparse takes them at makematrix!, src/AssemblyModule.jl:319-325):
+ * uploads them and runs the device sparse(). */
+int feasm_sparse(feasm_ctx *ctx, int64_t ntrip, const int64_t *I, const int64_t *J, const double *V, int64_t m,
+                 int64_t n, feasm_matrix **out);
 
 /* ---- mesh generators (src/MeshHexahedronModule.jl:39-112,
  *      src/MeshTetrahedronModule.jl:53-261) ------------------------------- */

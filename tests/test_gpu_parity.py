@@ -151,8 +151,7 @@ def test_general_dof_numbering_goes_through_coo(ctx):
 
 def test_sparse_device_order_and_zeros(ctx):
     # left fold in input order: (1e16 + 1) - 1e16 == 0; explicit zeros kept
-    M = core.sparse_device(ctx, np.array([1, 1, 1, 0]), np.array([0, 0, 0, 2]),
-                           np.array([1e16, 1.0, -1e16, 0.0]), 3, 3)
+    M = core.sparse(ctx, np.array([2, 2, 2, 1]), np.array([1, 1, 1, 3]), np.array([1e16, 1.0, -1e16, 0.0]), 3, 3)
     cp, rv, nz = M.csc_arrays(1)
     assert cp.tolist() == [1, 2, 2, 3] and rv.tolist() == [2, 1]
     assert nz[0] == 0.0 and nz[1] == 0.0
@@ -162,7 +161,7 @@ def test_sparse_device_order_and_zeros(ctx):
     J = rng.integers(1, n + 1, t)
     V = rng.normal(size=t)
     ocp, orv, onz = O.sparse(I, J, V, m, n)
-    M = core.sparse_device(ctx, I - 1, J - 1, V, m, n)
+    M = core.sparse(ctx, I, J, V, m, n)
     cp, rv, nz = M.csc_arrays(1)
     assert np.array_equal(cp, ocp) and np.array_equal(rv, orv)
     assert np.array_equal(nz, onz)  # same additions in the same order: bitwise
