@@ -114,7 +114,9 @@ def main():
     ctx.synchronize()
     t_first_numeric = time.perf_counter() - t0
     nnz = M.nnz
-    log("pattern: nnz %d, symbolic %.1f ms, first numeric %.1f ms" % (nnz, 1e3 * t_sym, 1e3 * t_first_numeric))
+    ntiles, nhalo = mesh.tile_info()
+    log("pattern: nnz %d, symbolic %.1f ms, first numeric %.1f ms (tiles %d, halo elements %d)"
+        % (nnz, 1e3 * t_sym, 1e3 * t_first_numeric, ntiles, nhalo))
 
     for _ in range(a.warmup):
         mesh.bilform_into(M, feasm.LIN_ELASTIC, feasm.RULE_GAUSS, 2, D)
@@ -155,10 +157,12 @@ def main():
         "nnz_per_s": nnz / (ms_step * 1e-3),
         "first_assembly_ms": 1e3 * (t_sym + t_first_numeric),
         "symbolic_ms": 1e3 * t_sym,
-        "kernels_ms": {"element": el_ms, "gather": ga_ms},
+        "kernels_ms": {"fused": el_ms} if ga_ms == 0.0 else {"element": el_ms, "gather": ga_ms},
+        "tiles": ntiles,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "element + gather (numeric phase)",
+                     "kernel": "k_fused (fused tile kernel, whole numeric phase)" if ga_ms == 0.0
+                     else "k_element + k_gather (staged numeric phase)",
                      "algorithmic_bytes": bytes_alg},
         "served": None,
     }

@@ -3,10 +3,13 @@
 #include <cmath>
 
 #include "common.hpp"
+#include "fused.hpp"
 
 using namespace feasm;
 
 namespace feasm {
+bool fused_supported(int npe, int nqp);
+FormParams make_form_params(int form, int ndn, const double *coeff, int ncoeff);
 void gen_h8block(double L, double W, double H, int64_t nL, int64_t nW, int64_t nH, int64_t *conn, double *xyz);
 int64_t gen_t10block(double L, double W, double H, int64_t nL, int64_t nW, int64_t nH, double amp_v, double amp_m,
                      uint64_t seed, int64_t *conn10, std::vector<double> &xyz10);
@@ -48,7 +51,9 @@ struct Timer {
     }
 };
 
-FormParams make_form(int form, int ndn, const double *coeff, int ncoeff) {
+}  // namespace
+
+FormParams feasm::make_form_params(int form, int ndn, const double *coeff, int ncoeff) {
     FormParams fp;
     std::memset(&fp, 0, sizeof(fp));
     fp.form = form;
@@ -76,6 +81,8 @@ FormParams make_form(int form, int ndn, const double *coeff, int ncoeff) {
     for (int i = 0; i < ncoeff; i++) fp.coef[i] = coeff[i];
     return fp;
 }
+
+namespace {
 
 void check_rule(const feasm_mesh *m, int rule) {
     if (m->etype == FEASM_H8)
@@ -124,6 +131,21 @@ void *ensure_stage(feasm_mesh *m, int form, int precision) {
 void numeric_into(feasm_mesh *m, const FormParams &fp, int rule, int order, int precision, double *nzval) {
     feasm_ctx *ctx = m->ctx;
     const DevQTable &qt = device_qtable(ctx, m->etype, rule, order);
+    const bool fused = m->numeric_path != 1 && fused_supported(m->npe, qt.nqp);
+    FEASM_REQUIRE(fused || m->numeric_path != 2, FEASM_EUNSUPPORTED,
+                  "no fused kernel for this element type and rule");
+    if (fused) {
+        const TileShape shape = tile_shape_for(ctx, m->npe, qt.nqp, fp.form, precision);
+        auto it = m->tiles.find(shape.key());
+        if (it == m->tiles.end()) it = m->tiles.emplace(shape.key(), build_tiles(m, shape)).first;
+        Timer tm(ctx);
+        tm.mark(0);
+        launch_fused(ctx, m, *m->pattern, *it->second, qt, fp, precision, nzval);
+        tm.mark(1);
+        tm.mark(2);
+        tm.done();
+        return;
+    }
     void *stage = ensure_stage(m, fp.form, precision);
     Timer tm(ctx);
     tm.mark(0);
@@ -369,6 +391,27 @@ extern "C" int feasm_pattern_build(feasm_mesh *m) {
     API_CATCH(m ? m->ctx : nullptr)
 }
 
+extern "C" int feasm_mesh_set_numeric_path(feasm_mesh *m, int path) {
+    API_TRY
+    FEASM_REQUIRE(m, FEASM_EINVAL, "null mesh");
+    FEASM_REQUIRE(path >= 0 && path <= 2, FEASM_EINVAL, "path: 0 automatic, 1 staged, 2 fused");
+    m->numeric_path = path;
+    API_CATCH(m ? m->ctx : nullptr)
+}
+
+extern "C" int feasm_mesh_tile_info(feasm_mesh *m, int64_t *ntiles, int64_t *nhalo) {
+    API_TRY
+    FEASM_REQUIRE(m, FEASM_EINVAL, "null mesh");
+    int64_t t = 0, h = 0;
+    for (auto &kv : m->tiles) {
+        t += kv.second->ntiles;
+        h += kv.second->nhalo;
+    }
+    if (ntiles) *ntiles = t;
+    if (nhalo) *nhalo = h;
+    API_CATCH(m ? m->ctx : nullptr)
+}
+
 extern "C" int feasm_pattern_info(feasm_mesh *m, int64_t *nnz, int64_t *nblocks) {
     API_TRY
     FEASM_REQUIRE(m && m->pattern, FEASM_ESTATE, "no pattern built");
@@ -384,7 +427,7 @@ extern "C" int feasm_bilform(feasm_mesh *m, int form, int rule, int order, const
     FEASM_REQUIRE(m->ndn > 0, FEASM_ESTATE, "set the DOF numbering first");
     FEASM_REQUIRE(precision == FEASM_FP64 || precision == FEASM_MIXED, FEASM_EINVAL, "unknown precision");
     check_rule(m, rule);
-    const FormParams fp = make_form(form, m->ndn, coeff, ncoeff);
+    const FormParams fp = make_form_params(form, m->ndn, coeff, ncoeff);
     if (!m->nodemajor) {
         *out = coo_path(m, fp, rule, order, precision);
         return FEASM_OK;
@@ -413,7 +456,7 @@ extern "C" int feasm_bilform_into(feasm_mesh *m, int form, int rule, int order, 
                   "matrix does not hold this mesh's resident pattern");
     FEASM_REQUIRE(precision == FEASM_FP64 || precision == FEASM_MIXED, FEASM_EINVAL, "unknown precision");
     check_rule(m, rule);
-    const FormParams fp = make_form(form, m->ndn, coeff, ncoeff);
+    const FormParams fp = make_form_params(form, m->ndn, coeff, ncoeff);
     numeric_into(m, fp, rule, order, precision, dp<double>(mat->nzval));
     API_CATCH(m ? m->ctx : nullptr)
 }
@@ -425,7 +468,7 @@ extern "C" int feasm_bilform_coo(feasm_mesh *m, int form, int rule, int order, c
     FEASM_REQUIRE(m->ndn > 0, FEASM_ESTATE, "set the DOF numbering first");
     FEASM_REQUIRE(precision == FEASM_FP64 || precision == FEASM_MIXED, FEASM_EINVAL, "unknown precision");
     check_rule(m, rule);
-    const FormParams fp = make_form(form, m->ndn, coeff, ncoeff);
+    const FormParams fp = make_form_params(form, m->ndn, coeff, ncoeff);
     *out = coo_path(m, fp, rule, order, precision);
     API_CATCH(m ? m->ctx : nullptr)
 }

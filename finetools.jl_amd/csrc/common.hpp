@@ -80,6 +80,10 @@ struct DevQTable {
 
 }  // namespace feasm
 
+namespace feasm {
+struct TileSet;
+}
+
 // ---------------------------------------------------------------- handles
 struct feasm_ctx {
     int device = 0;
@@ -129,6 +133,10 @@ struct feasm_mesh {
     feasm::DevPtr dofs;     // int32 [nnodes*ndn] 0-based DOF numbers (node-major rows)
     std::shared_ptr<Pattern> pattern;
     feasm::DevPtr stage;    // per-element staging of the node-pair blocks
+    // numeric path: 0 = automatic (fused tiles when the element/rule has a
+    // fused kernel), 1 = staged element blocks + ordered gather, 2 = fused
+    int numeric_path = 0;
+    std::map<int64_t, std::shared_ptr<feasm::TileSet>> tiles;  // keyed by TileShape::key()
 };
 
 struct feasm_matrix {

@@ -42,6 +42,8 @@ SIGNATURES = {
     "feasm_bilform_into": (I32, [P, I32, I32, I32, P, I32, I32, P]),
     "feasm_bilform_coo": (I32, [P, I32, I32, I32, P, I32, I32, PP]),
     "feasm_pattern_build": (I32, [P]),
+    "feasm_mesh_set_numeric_path": (I32, [P, I32]),
+    "feasm_mesh_tile_info": (I32, [P, PI64, PI64]),
     "feasm_pattern_info": (I32, [P, PI64, PI64]),
     "feasm_matrix_dims": (I32, [P, PI64, PI64, PI64]),
     "feasm_matrix_copy_csc": (I32, [P, I32, P, P, P]),
@@ -57,6 +59,16 @@ SIGNATURES = {
     "feasm_coo_destroy": (I32, [P]),
     "feasm_sparse_device": (I32, [P, I64, P, P, P, I64, I64, PP]),
     "feasm_sparse": (I32, [P, I64, P, P, P, I64, I64, PP]),
+    "feasm_batch_create": (I32, [P, PP]),
+    "feasm_batch_add": (I32, [P, P, I32, I32, I32, P, I32, I32]),
+    "feasm_batch_prepare": (I32, [P, I32, I32, I32]),
+    "feasm_batch_run": (I32, [P]),
+    "feasm_batch_last_ms": (I32, [P, ctypes.POINTER(ctypes.c_double)]),
+    "feasm_batch_matrix": (I32, [P, I64, PP]),
+    "feasm_batch_stats": (I32, [P, PI64, PI64, PI64, PI64]),
+    "feasm_batch_autotune": (I32, [P, ctypes.c_char_p, ctypes.c_char_p, I32, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]),
+    "feasm_batch_destroy": (I32, [P]),
     "feasm_gen_h8block": (I32, [DBL, DBL, DBL, I64, I64, I64, PI64, PI64, P, P]),
     "feasm_gen_t10block": (I32, [DBL, DBL, DBL, I64, I64, I64, DBL, DBL, U64, PI64, PI64, P, P]),
     "feasm_gen_perturb": (I32, [I64, P, P, P, DBL, U64]),

@@ -68,13 +68,15 @@ def default_context():
 class DeviceMatrix:
     """A CSC matrix held in HBM (feasm_matrix)."""
 
-    def __init__(self, ctx, handle):
+    def __init__(self, ctx, handle, owner=None):
         self.ctx = ctx
         self.h = handle
+        self._owner = owner  # set when another handle owns the matrix (batch results)
 
     def close(self):
         if getattr(self, "h", None):
-            lib().feasm_matrix_destroy(self.h)
+            if self._owner is None:
+                lib().feasm_matrix_destroy(self.h)
             self.h = None
 
     __del__ = close
@@ -152,6 +154,15 @@ class DeviceMesh:
         check(lib().feasm_pattern_info(self.h, ctypes.byref(nnz), ctypes.byref(nb)), self.ctx.h)
         return nnz.value, nb.value
 
+    def set_numeric_path(self, path):
+        """'auto', 'staged' or 'fused'."""
+        check(lib().feasm_mesh_set_numeric_path(self.h, {"auto": 0, "staged": 1, "fused": 2}[path]), self.ctx.h)
+
+    def tile_info(self):
+        t, h = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().feasm_mesh_tile_info(self.h, ctypes.byref(t), ctypes.byref(h)), self.ctx.h)
+        return t.value, h.value
+
     def build_pattern(self):
         check(lib().feasm_pattern_build(self.h), self.ctx.h)
 
@@ -166,6 +177,59 @@ class DeviceMesh:
         c = np.ascontiguousarray(np.atleast_1d(coeff), dtype=np.float64).ravel()
         check(lib().feasm_bilform_into(self.h, form, rule, order, ptr(c), len(c), precision, mat.h), self.ctx.h)
         return mat
+
+
+class Batch:
+    """Many meshes assembled by one persistent launch (feasm_batch)."""
+
+    def __init__(self, ctx):
+        h = ctypes.c_void_p()
+        check(lib().feasm_batch_create(ctx.h, ctypes.byref(h)), ctx.h)
+        self.ctx = ctx
+        self.h = h
+        self.meshes = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().feasm_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def add(self, mesh, form, rule, order, coeff, precision=L.FP64):
+        c = np.ascontiguousarray(np.atleast_1d(coeff), dtype=np.float64).ravel()
+        check(lib().feasm_batch_add(self.h, mesh.h, form, rule, order, ptr(c), len(c), precision), self.ctx.h)
+        self.meshes.append(mesh)
+        return len(self.meshes) - 1
+
+    def prepare(self, use_graph=True, max_nodes=0, budget_kib=0):
+        check(lib().feasm_batch_prepare(self.h, 1 if use_graph else 0, max_nodes, budget_kib), self.ctx.h)
+
+    def run(self):
+        check(lib().feasm_batch_run(self.h), self.ctx.h)
+
+    def last_ms(self):
+        v = ctypes.c_double()
+        check(lib().feasm_batch_last_ms(self.h, ctypes.byref(v)), self.ctx.h)
+        return v.value
+
+    def matrix(self, i):
+        h = ctypes.c_void_p()
+        check(lib().feasm_batch_matrix(self.h, i, ctypes.byref(h)), self.ctx.h)
+        return DeviceMatrix(self.ctx, h, owner=self)
+
+    def stats(self):
+        a, b, c, d = (ctypes.c_int64() for _ in range(4))
+        check(lib().feasm_batch_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)),
+              self.ctx.h)
+        return {"meshes": a.value, "elements": b.value, "nnz": c.value, "tiles": d.value}
+
+    def autotune(self, db_path=None, key=None, reps=3):
+        n, kb, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(lib().feasm_batch_autotune(self.h, db_path.encode() if db_path else None,
+                                         key.encode() if key else None, reps, ctypes.byref(n), ctypes.byref(kb),
+                                         ctypes.byref(ms)), self.ctx.h)
+        return {"max_nodes": n.value, "budget_kib": kb.value, "ms": ms.value}
 
 
 def sparse(ctx, I, J, V, m, n):

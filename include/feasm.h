@@ -70,6 +70,7 @@ typedef struct feasm_ctx feasm_ctx;
 typedef struct feasm_mesh feasm_mesh;
 typedef struct feasm_matrix feasm_matrix;
 typedef struct feasm_coo feasm_coo;
+typedef struct feasm_batch feasm_batch;
 
 /* ---- context ------------------------------------------------------------ */
 /* No reference counterpart (the reference is host-only Julia): owns the HIP
@@ -141,6 +142,14 @@ int feasm_bilform_into(feasm_mesh *mesh, int form, int rule, int order, const do
  * device sparse() (radix sort + ordered duplicate summation). */
 int feasm_bilform_coo(feasm_mesh *mesh, int form, int rule, int order, const double *coeff, int ncoeff,
                       int precision, feasm_matrix **out);
+/* Numeric path of the resident-pattern assembly: 0 automatic (the fused
+ * tile kernel where the element type and rule have one: H8 with GaussRule(3,2),
+ * T10 with TetRule(4)), 1 staged element blocks + ordered gather (every
+ * element/rule), 2 fused (error where unavailable).  Both give the same
+ * pattern; values agree to rounding. */
+int feasm_mesh_set_numeric_path(feasm_mesh *mesh, int path);
+/* Tiles built for the fused kernel so far (summed over forms/precisions). */
+int feasm_mesh_tile_info(feasm_mesh *mesh, int64_t *ntiles, int64_t *nhalo);
 /* Build (or rebuild) the device CSC pattern for the mesh's DOFs now. */
 int feasm_pattern_build(feasm_mesh *mesh);
 /* Pattern statistics: nnz of the DOF-level CSC and number of node blocks. */
@@ -217,6 +226,35 @@ int feasm_gen_t10block(double L, double W, double H, int64_t nL, int64_t nW, int
  * counter 3n+c+1 of stream `seed`. */
 int feasm_gen_perturb(int64_t nnodes, double *xyz, const double lo[3], const double hi[3], double amp,
                       uint64_t seed);
+
+/* ---- batched multi-mesh assembly (no reference counterpart: the
+ *      reference assembles one mesh per bilform_* call) ------------------- */
+/* Many independent meshes (each with its own form, coefficients and
+ * resident pattern) assembled by one persistent fused launch per element
+ * type/arithmetic group, optionally captured in a hipGraph. */
+int feasm_batch_create(feasm_ctx *ctx, feasm_batch **out);
+/* Add a mesh that already has its DOFs (node-major numbering); the form
+ * arguments are those of feasm_bilform.  H8/GaussRule(3,2) and T10/TetRule(4). */
+int feasm_batch_add(feasm_batch *b, feasm_mesh *mesh, int form, int rule, int order, const double *coeff,
+                    int ncoeff, int precision);
+/* Build patterns, tiles and launch tables; use_graph != 0 captures the
+ * launches in a hipGraph replayed by feasm_batch_run.  max_nodes (owned
+ * nodes per tile, <= 64) and budget_kib (LDS per workgroup) tune the tiles;
+ * 0 picks the defaults. */
+int feasm_batch_prepare(feasm_batch *b, int use_graph, int max_nodes, int budget_kib);
+/* Enqueue one batched assembly on the context stream (asynchronous). */
+int feasm_batch_run(feasm_batch *b);
+/* Device time of the last feasm_batch_run (waits for it). */
+int feasm_batch_last_ms(feasm_batch *b, double *ms);
+/* Result matrix of entry i (owned by the batch: do not destroy). */
+int feasm_batch_matrix(feasm_batch *b, int64_t i, feasm_matrix **out);
+int feasm_batch_stats(feasm_batch *b, int64_t *nmeshes, int64_t *nelem, int64_t *nnz, int64_t *ntiles);
+/* Autotune max_nodes/budget over a small grid (timing `reps` runs each),
+ * leaving the batch prepared with the winner.  With db_path and key the
+ * result is cached in (and first looked up from) a text database. */
+int feasm_batch_autotune(feasm_batch *b, const char *db_path, const char *key, int reps, int *best_nodes,
+                         int *best_budget_kib, double *best_ms);
+int feasm_batch_destroy(feasm_batch *b);
 
 #ifdef __cplusplus
 }

@@ -73,7 +73,7 @@ def oracle_csc(etype, form, conn, xyz, ndn, coeff):
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("path", ["resident", "coo"])
+@pytest.mark.parametrize("path", ["fused", "staged", "coo"])
 def test_fp64_parity(ctx, case, path):
     name, etype, n, amp, form, ndn, coeff = case
     say("case", name, path)
@@ -81,6 +81,8 @@ def test_fp64_parity(ctx, case, path):
     ocp, orv, onz = oracle_csc(etype, form, conn, xyz, ndn, coeff)
     mesh = core.DeviceMesh(ctx, etype, conn, xyz)
     mesh.set_dofs(ndn)
+    if path != "coo":
+        mesh.set_numeric_path(path)
     rule, order = rule_for(etype)
     M = mesh.bilform(form, rule, order, np.ravel(coeff), precision=L.FP64, path=path)
     cp, rv, nz = M.csc_arrays(base=1)
@@ -97,13 +99,15 @@ def test_fp64_parity(ctx, case, path):
 
 @pytest.mark.parametrize("case", [c for c in CASES if c[1] == O.T10 or c[4] == L.LIN_ELASTIC],
                          ids=lambda c: c[0])
-def test_mixed_precision_error(ctx, case):
+@pytest.mark.parametrize("path", ["fused", "staged"])
+def test_mixed_precision_error(ctx, case, path):
     name, etype, n, amp, form, ndn, coeff = case
-    say("mixed", name)
+    say("mixed", name, path)
     conn, xyz = mesh_for(etype, n, amp, 99)
     ocp, orv, onz = oracle_csc(etype, form, conn, xyz, ndn, coeff)
     mesh = core.DeviceMesh(ctx, etype, conn, xyz)
     mesh.set_dofs(ndn)
+    mesh.set_numeric_path(path)
     rule, order = rule_for(etype)
     M = mesh.bilform(form, rule, order, np.ravel(coeff), precision=L.MIXED)
     cp, rv, nz = M.csc_arrays(base=1)
@@ -253,3 +257,74 @@ def test_errors_surface_with_reference_text(ctx):
     bad = np.array([[1, 2, 3, 4, 5, 6, 7, nn + 5]], np.int64)
     with pytest.raises(feasm.FeasmError, match="missing node"):
         core.DeviceMesh(ctx, O.H8, bad, xyz)
+
+
+@pytest.mark.parametrize("etype,n,form,ndn,coeff", [
+    (O.H8, (9, 7, 8), L.LIN_ELASTIC, 3, D200),
+    (O.T10, (5, 4, 5), L.LIN_ELASTIC, 3, D70),
+    (O.H8, (12, 11, 10), L.DIFFUSION_ISO, 1, [1.3]),
+    (O.T10, (4, 5, 4), L.DOT, 3, 2.7e-9 * np.eye(3)),
+])
+def test_fused_matches_staged_and_oracle_on_many_tiles(ctx, etype, n, form, ndn, coeff):
+    # meshes large enough for many tiles with halos crossing tile borders
+    say("fused vs staged", etype, n, form)
+    conn, xyz = mesh_for(etype, n, 0.15 if etype == O.H8 else 0.1, 77)
+    mesh = core.DeviceMesh(ctx, etype, conn, xyz)
+    mesh.set_dofs(ndn)
+    rule, order = rule_for(etype)
+    mesh.set_numeric_path("fused")
+    F = mesh.bilform(form, rule, order, np.ravel(coeff))
+    ntiles, nhalo = mesh.tile_info()
+    say("  tiles", ntiles, "halo elements", nhalo, "elements", conn.shape[0])
+    assert ntiles > 4
+    mesh.set_numeric_path("staged")
+    S = mesh.bilform(form, rule, order, np.ravel(coeff))
+    f, s = F.csc_arrays(1), S.csc_arrays(1)
+    assert np.array_equal(f[0], s[0]) and np.array_equal(f[1], s[1])
+    ocp, orv, onz = oracle_csc(etype, form, conn, xyz, ndn, coeff)
+    assert np.array_equal(f[0], ocp) and np.array_equal(f[1], orv)
+    assert rel_fro(f[2], onz) <= FP64_TOL and rel_fro(s[2], onz) <= FP64_TOL
+    import scipy.sparse as sp
+    K = sp.csc_matrix((f[2], f[1] - 1, f[0] - 1), shape=(len(f[0]) - 1,) * 2)
+    assert (K - K.T).count_nonzero() == 0
+
+
+def test_batched_launch_matches_single_and_oracle_with_graph_replays(ctx):
+    """Each mesh's result is the same alone and in the batch (bitwise, same
+    tiles), and every replay of the captured graph gives the eager result."""
+    rng = np.random.default_rng(11)
+    specs = []
+    for i in range(6):
+        n = tuple(int(x) for x in rng.integers(2, 7, 3))
+        if i % 2 == 0:
+            specs.append((n, L.DIFFUSION_ISO, 1, [float(rng.uniform(0.5, 2.0))]))
+        else:
+            specs.append((n, L.LIN_ELASTIC, 3, O.iso_D(float(rng.uniform(1e3, 1e5)), float(rng.uniform(0.2, 0.45)))))
+    for precision in (L.FP64, L.MIXED):
+        say("batch precision", precision)
+        batch = core.Batch(ctx)
+        meshes, singles = [], []
+        for j, (n, form, ndn, coeff) in enumerate(specs):
+            conn, xyz = mesh_for(O.H8, n, 0.1, 100 + j)
+            mesh = core.DeviceMesh(ctx, O.H8, conn, xyz)
+            mesh.set_dofs(ndn)
+            singles.append(mesh.bilform(form, L.RULE_GAUSS, 2, np.ravel(coeff), precision=precision).csc_arrays(1))
+            batch.add(mesh, form, L.RULE_GAUSS, 2, np.ravel(coeff), precision)
+            meshes.append((mesh, conn, xyz, form, ndn, coeff))
+        batch.prepare(use_graph=True)
+        for rep in range(3):
+            batch.run()
+            ctx.synchronize()
+            for j, (mesh, conn, xyz, form, ndn, coeff) in enumerate(meshes):
+                got = batch.matrix(j).csc_arrays(1)
+                assert np.array_equal(got[0], singles[j][0]) and np.array_equal(got[1], singles[j][1])
+                assert np.array_equal(got[2], singles[j][2]), (rep, j)
+            # eager work between replays (graph replays must not depend on it)
+            meshes[0][0].bilform(specs[0][1], L.RULE_GAUSS, 2, np.ravel(specs[0][3]))
+        for j, (mesh, conn, xyz, form, ndn, coeff) in enumerate(meshes):
+            ocp, orv, onz = oracle_csc(O.H8, form, conn, xyz, ndn, coeff)
+            err = rel_fro(batch.matrix(j).csc_arrays(1)[2], onz)
+            assert err <= (FP64_TOL if precision == L.FP64 else MIXED_TOL), (j, err)
+        st = batch.stats()
+        assert st["meshes"] == 6 and st["tiles"] > 0
+        batch.close()
