@@ -67,6 +67,201 @@ def cpu_baseline(n_cells, seed, D, min_seconds=10.0, max_runs=5):
                       "(%.2f s each)" % (n_cells, ne, len(times), t)}
 
 
+def rel_fro(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def bench_t10(feasm, core, n=60, steps=5):
+    """C3: T10 n^3 x 6 tets, E=70e3 nu=0.33, rho=2.7e-9: FP64 and mixed stiffness
+    plus the consistent mass; error of the mixed results against FP64."""
+    ctx = feasm.Context(0)
+    h = 1.0 / n
+    t0 = time.perf_counter()
+    conn, xyz = core.gen_t10block(1.0, 1.0, 1.0, n, n, n, 0.1 * h, 0.03 * h, 3)
+    tgen = time.perf_counter() - t0
+    ne, nn = conn.shape[0], xyz.shape[0]
+    log("T10 mesh: %d elements, %d nodes (generated in %.1f s)" % (ne, nn, tgen))
+    mesh = core.DeviceMesh(ctx, feasm.T10, conn, xyz)
+    del conn, xyz
+    mesh.set_dofs(3)
+    D = iso_D(70e3, 0.33)
+    rho = 2.7e-9
+    out = {"workload": "T10 %dx%dx%d x6 = %d elements, %d nodes, TetRule(4), E=70e3 nu=0.33, rho=2.7e-9" % (n, n, n, ne, nn)}
+    res = {}
+    for name, form, coeff, prec in (("K_fp64", feasm.LIN_ELASTIC, D, feasm.FP64),
+                                    ("K_mixed", feasm.LIN_ELASTIC, D, feasm.MIXED),
+                                    ("M_fp64", feasm.DOT, rho * np.eye(3), feasm.FP64),
+                                    ("M_mixed", feasm.DOT, rho * np.eye(3), feasm.MIXED)):
+        M = mesh.bilform(form, feasm.RULE_TET, 4, coeff, precision=prec)
+        for _ in range(2):
+            mesh.bilform_into(M, form, feasm.RULE_TET, 4, coeff, precision=prec)
+        ctx.set_timing(steps)
+        for _ in range(steps):
+            mesh.bilform_into(M, form, feasm.RULE_TET, 4, coeff, precision=prec)
+        ctx.synchronize()
+        ms = float(np.mean(ctx.timings()[:, 2]))
+        ctx.set_timing(0)
+        nnz = M.nnz
+        res[name] = M.csc_arrays(0)[2]
+        out[name] = {"ms": ms, "elements_per_s": ne / (ms * 1e-3), "nnz_per_s": nnz / (ms * 1e-3),
+                     "achieved_GBps": algorithmic_bytes_numeric(10, ne, nn, nnz) / (ms * 1e-3) / 1e9}
+        out["nnz"] = nnz
+        M.close()
+    out["K_mixed"]["rel_fro_vs_fp64"] = rel_fro(res["K_mixed"], res["K_fp64"])
+    out["M_mixed"]["rel_fro_vs_fp64"] = rel_fro(res["M_mixed"], res["M_fp64"])
+    mesh.close()
+    ctx.close()
+    return out
+
+
+def bench_batched(feasm, core, nmesh=256, seed=4, steps=5, tune_db=None):
+    """C4: 256 H8 blocks, nx,ny,nz ~ U{4..64}; half 1-DOF diffusion k~U[0.5,2],
+    half 3-DOF elasticity E~U[1e3,1e5] nu~U[0.2,0.45]; mixed precision; one
+    persistent launch replayed from a hipGraph; tile size autotuned."""
+    rng = np.random.default_rng(seed)
+    ctx = feasm.Context(0)
+    meshes, specs = [], []
+    t0 = time.perf_counter()
+    ne_tot = 0
+    for i in range(nmesh):
+        nx, ny, nz = (int(v) for v in rng.integers(4, 65, 3))
+        conn, xyz = core.gen_h8block(1.0, 1.0, 1.0, nx, ny, nz)
+        h = 1.0 / max(nx, ny, nz)
+        xyz = core.gen_perturb(xyz, [0, 0, 0], [1, 1, 1], 0.1 * h, 1000 + i)
+        m = core.DeviceMesh(ctx, feasm.H8, conn, xyz)
+        if i % 2 == 0:
+            m.set_dofs(1)
+            specs.append((feasm.DIFFUSION_ISO, [float(rng.uniform(0.5, 2.0))]))
+        else:
+            m.set_dofs(3)
+            specs.append((feasm.LIN_ELASTIC, iso_D(float(rng.uniform(1e3, 1e5)), float(rng.uniform(0.2, 0.45)))))
+        meshes.append(m)
+        ne_tot += conn.shape[0]
+    log("batch: %d meshes, %d elements (setup %.1f s)" % (nmesh, ne_tot, time.perf_counter() - t0))
+    results = {}
+    for label, prec in (("mixed", feasm.MIXED), ("fp64", feasm.FP64)):
+        batch = core.Batch(ctx)
+        for m, (form, coeff) in zip(meshes, specs):
+            batch.add(m, form, feasm.RULE_GAUSS, 2, coeff, precision=prec)
+        t0 = time.perf_counter()
+        tuned = batch.autotune(db_path=tune_db, key="h8-batch256-%s" % label, reps=3)
+        batch.prepare(use_graph=True, max_nodes=tuned["max_nodes"], budget_kib=tuned["budget_kib"])
+        tprep = time.perf_counter() - t0
+        batch.run()
+        ctx.synchronize()
+        times = []
+        for _ in range(steps):
+            batch.run()
+            times.append(batch.last_ms())
+        st = batch.stats()
+        ms = float(np.median(times))
+        results[label] = {"ms": ms, "elements_per_s": st["elements"] / (ms * 1e-3),
+                          "nnz_per_s": st["nnz"] / (ms * 1e-3), "tuned": tuned, "prepare_s": tprep,
+                          "tiles": st["tiles"]}
+        if label == "mixed":
+            vals_mixed = [batch.matrix(i).csc_arrays(0)[2] for i in range(0, nmesh, 16)]
+        else:
+            vals_fp64 = [batch.matrix(i).csc_arrays(0)[2] for i in range(0, nmesh, 16)]
+        results["nnz"] = st["nnz"]
+        batch.close()
+    results["mixed"]["rel_fro_vs_fp64_max"] = max(rel_fro(a, b) for a, b in zip(vals_mixed, vals_fp64))
+    # the plain per-mesh FP64 path, one bilform per mesh, for comparison
+    mats = [m.bilform(form, feasm.RULE_GAUSS, 2, coeff) for m, (form, coeff) in zip(meshes, specs)]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for m, M, (form, coeff) in zip(meshes, mats, specs):
+            m.bilform_into(M, form, feasm.RULE_GAUSS, 2, coeff)
+    ctx.synchronize()
+    ms_plain = 1e3 * (time.perf_counter() - t0) / steps
+    results["plain_fp64_per_mesh"] = {"ms": ms_plain, "elements_per_s": ne_tot / (ms_plain * 1e-3)}
+    results["elements"] = ne_tot
+    results["speedup_vs_plain"] = results["mixed"]["elements_per_s"] / results["plain_fp64_per_mesh"]["elements_per_s"]
+    for M in mats:
+        M.close()
+    for m in meshes:
+        m.close()
+    ctx.close()
+    return results
+
+
+def bench_served(feasm, core, rate=200.0, duration=3.0, seed=5, pool=48, workers=8):
+    """C5: Poisson arrivals (rate/s) of H8/T10 (50/50) block meshes with log-normal
+    element counts (median 5e4, sigma 1.0, clipped to [1e3, 2e6]), diffusion or
+    elasticity; 30 % re-assemble a resident mesh with new material values.
+    Served (continuous batching, mixed precision) vs the plain per-request FP64
+    path (one request at a time, in arrival order)."""
+    rng = np.random.default_rng(seed)
+    # a pool of distinct meshes generated up front (the requests carry them)
+    meshes = []
+    t0 = time.perf_counter()
+    for i in range(pool):
+        ne_target = float(np.clip(np.exp(np.log(5e4) + 1.0 * rng.standard_normal()), 1e3, 2e6))
+        if i % 2 == 0:
+            n = max(1, int(round(ne_target ** (1 / 3))))
+            conn, xyz = core.gen_h8block(1.0, 1.0, 1.0, n, n, n)
+            xyz = core.gen_perturb(xyz, [0, 0, 0], [1, 1, 1], 0.1 / n, 2000 + i)
+            meshes.append((feasm.H8, conn, xyz, (feasm.RULE_GAUSS, 2)))
+        else:
+            n = max(1, int(round((ne_target / 6) ** (1 / 3))))
+            conn, xyz = core.gen_t10block(1.0, 1.0, 1.0, n, n, n, 0.1 / n, 0.03 / n, 2000 + i)
+            meshes.append((feasm.T10, conn, xyz, (feasm.RULE_TET, 4)))
+    log("served: pool of %d meshes generated in %.1f s" % (pool, time.perf_counter() - t0))
+    # the request stream
+    nreq = int(rate * duration)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    stream = []
+    for k in range(nreq):
+        diff = rng.random() < 0.5
+        form = feasm.DIFFUSION_ISO if diff else feasm.LIN_ELASTIC
+        coeff = [float(rng.uniform(0.5, 2.0))] if diff else iso_D(float(rng.uniform(1e3, 1e5)),
+                                                                   float(rng.uniform(0.2, 0.45)))
+        stream.append((arrivals[k], int(rng.integers(0, pool)), form, coeff, rng.random() < 0.3))
+
+    def run(precision, workers_):
+        srv = core.Server(0, workers=workers_, hbm_fraction=0.6)
+        resident = {}  # (pool index, ndn) -> mesh id
+        pending = []
+        ne_done = 0
+        t_start = time.perf_counter()
+        for (t_arr, pi, form, coeff, re) in stream:
+            while time.perf_counter() - t_start < t_arr:
+                time.sleep(0.0002)
+            et, conn, xyz, rule = meshes[pi]
+            ndn = 1 if form == feasm.DIFFUSION_ISO else 3
+            key = (pi, ndn)
+            if re and key in resident:
+                rid = srv.submit_reassembly(resident[key], form, rule[0], rule[1], coeff, precision)
+            else:
+                rid, mid = srv.submit_mesh(et, conn, xyz, ndn, form, rule[0], rule[1], coeff, precision,
+                                           keep=key not in resident)
+                if key not in resident:
+                    resident[key] = mid
+            pending.append((rid, conn.shape[0]))
+        lats = []
+        for rid, ne in pending:
+            st, lat, _ = srv.wait(rid)
+            if st == 2:
+                lats.append(lat)
+                ne_done += ne
+            srv.release(rid)
+        wall = time.perf_counter() - t_start
+        srv.close()
+        lats = np.array(lats)
+        return {"requests": len(stream), "completed": int(len(lats)), "wall_s": wall,
+                "elements_per_s": ne_done / wall, "p50_ms": float(np.percentile(lats, 50)),
+                "p99_ms": float(np.percentile(lats, 99)), "mean_ms": float(lats.mean())}
+
+    served = run(feasm.MIXED, workers)
+    log("served: %s" % served)
+    plain = run(feasm.FP64, 1)
+    log("plain: %s" % plain)
+    return {"workload": "Poisson %.0f req/s for %.1f s, H8/T10 50/50, median 5e4 elements (sigma 1.0), 30%% "
+                        "re-assembly of resident meshes" % (rate, duration),
+            "served_mixed": served, "plain_fp64_sequential": plain,
+            "throughput_ratio": served["elements_per_s"] / plain["elements_per_s"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +270,7 @@ def main():
     ap.add_argument("--cells", type=int, default=100, help="cells per direction of the H8 block")
     ap.add_argument("--cpu-cells", type=int, default=50, help="cells per direction of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--extras", default="", help="comma list of extra workloads: t10,batch,served")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB streaming copy (3 launches) to scale traffic counters")
     a = ap.parse_args()
@@ -170,6 +366,18 @@ def main():
         out["copy_gbs_measured"] = copy_gbs
     del M
     mesh.close()
+    ctx.close()
+    extras = [x for x in a.extras.split(",") if x]
+    if "t10" in extras:
+        log("T10 ...")
+        out["t10"] = bench_t10(feasm, core)
+    if "batch" in extras:
+        log("batched ...")
+        out["batched"] = bench_batched(feasm, core, tune_db=os.path.join(ROOT, "finetools.jl_amd", "feasm",
+                                                                          "autotune.txt"))
+    if "served" in extras:
+        log("served ...")
+        out["served"] = bench_served(feasm, core)
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(a.cpu_cells, 2, D)

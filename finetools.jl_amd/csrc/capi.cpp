@@ -9,6 +9,8 @@ using namespace feasm;
 
 namespace feasm {
 bool fused_supported(int npe, int nqp);
+void numeric_into_ctx(feasm_ctx *ctx, feasm_mesh *m, const FormParams &fp, int rule, int order, int precision,
+                      double *nzval);
 FormParams make_form_params(int form, int ndn, const double *coeff, int ncoeff);
 void gen_h8block(double L, double W, double H, int64_t nL, int64_t nW, int64_t nH, int64_t *conn, double *xyz);
 int64_t gen_t10block(double L, double W, double H, int64_t nL, int64_t nW, int64_t nH, double amp_v, double amp_m,
@@ -128,8 +130,10 @@ void *ensure_stage(feasm_mesh *m, int form, int precision) {
     return m->stage->p;
 }
 
-void numeric_into(feasm_mesh *m, const FormParams &fp, int rule, int order, int precision, double *nzval) {
-    feasm_ctx *ctx = m->ctx;
+}  // namespace
+
+void feasm::numeric_into_ctx(feasm_ctx *ctx, feasm_mesh *m, const FormParams &fp, int rule, int order,
+                             int precision, double *nzval) {
     const DevQTable &qt = device_qtable(ctx, m->etype, rule, order);
     const bool fused = m->numeric_path != 1 && fused_supported(m->npe, qt.nqp);
     FEASM_REQUIRE(fused || m->numeric_path != 2, FEASM_EUNSUPPORTED,
@@ -146,6 +150,7 @@ void numeric_into(feasm_mesh *m, const FormParams &fp, int rule, int order, int 
         tm.done();
         return;
     }
+    build_contributions(m, *m->pattern);
     void *stage = ensure_stage(m, fp.form, precision);
     Timer tm(ctx);
     tm.mark(0);
@@ -154,6 +159,12 @@ void numeric_into(feasm_mesh *m, const FormParams &fp, int rule, int order, int 
     launch_gather(ctx, m, *m->pattern, fp, precision, stage, nzval);
     tm.mark(2);
     tm.done();
+}
+
+namespace {
+
+void numeric_into(feasm_mesh *m, const FormParams &fp, int rule, int order, int precision, double *nzval) {
+    numeric_into_ctx(m->ctx, m, fp, rule, order, precision, nzval);
 }
 
 feasm_matrix *coo_path(feasm_mesh *m, const FormParams &fp, int rule, int order, int precision) {

@@ -157,6 +157,7 @@ const DevQTable &device_qtable(feasm_ctx *ctx, int etype, int rule, int order);
 // pattern.hip
 void build_node_to_element(feasm_mesh *m);
 std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn);
+void build_contributions(feasm_mesh *m, Pattern &p);
 
 // element.hip
 struct FormParams {

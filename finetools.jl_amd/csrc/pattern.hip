@@ -199,6 +199,30 @@ void build_node_to_element(feasm_mesh *m) {
     FEASM_HIP(hipStreamSynchronize(ctx->stream));
 }
 
+void build_contributions(feasm_mesh *m, Pattern &P_) {
+    feasm_ctx *ctx = m->ctx;
+    Pattern *P = &P_;
+    if (P->contrib) return;
+    DevPtr ccount = dev_alloc((P->nblocks + 1) * sizeof(int32_t));
+    FEASM_HIP(hipMemsetAsync(ccount->p, 0, (P->nblocks + 1) * sizeof(int32_t), ctx->stream));
+    hipLaunchKernelGGL(k_contrib, dim3(grid_for(P->nblocks, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(P->bcol),
+                       dp<int32_t>(P->adj), P->nblocks, dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e),
+                       dp<int32_t>(m->conn), m->npe, (const int64_t *)nullptr, dp<int32_t>(ccount),
+                       (uint32_t *)nullptr);
+    FEASM_HIP(hipGetLastError());
+    P->cptr = dev_alloc((P->nblocks + 1) * sizeof(int64_t));
+    exclusive_scan_i32_to_i64(ctx, dp<int32_t>(ccount), dp<int64_t>(P->cptr), P->nblocks + 1);
+    P->ncontrib = read_i64(ctx, dp<int64_t>(P->cptr) + P->nblocks);
+    DevPtr contrib = dev_alloc(std::max<size_t>(P->ncontrib * sizeof(uint32_t), 4));
+    hipLaunchKernelGGL(k_contrib, dim3(grid_for(P->nblocks, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(P->bcol),
+                       dp<int32_t>(P->adj), P->nblocks, dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e),
+                       dp<int32_t>(m->conn), m->npe, dp<int64_t>(P->cptr), (int32_t *)nullptr,
+                       dp<uint32_t>(contrib));
+    FEASM_HIP(hipGetLastError());
+    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    P->contrib = contrib;
+}
+
 std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn) {
     feasm_ctx *ctx = m->ctx;
     if (!m->n2e_ptr) build_node_to_element(m);
@@ -229,29 +253,13 @@ std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn) {
     hipLaunchKernelGGL(k_bcol, dim3(grid_for(nn, 256)), dim3(256), 0, ctx->stream, dp<int64_t>(P->adj_ptr), nn,
                        dp<int32_t>(P->bcol));
     FEASM_HIP(hipGetLastError());
-    // 3. contributions
-    DevPtr ccount = dev_alloc((P->nblocks + 1) * sizeof(int32_t));
-    FEASM_HIP(hipMemsetAsync(ccount->p, 0, (P->nblocks + 1) * sizeof(int32_t), ctx->stream));
-    hipLaunchKernelGGL(k_contrib, dim3(grid_for(P->nblocks, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(P->bcol),
-                       dp<int32_t>(P->adj), P->nblocks, dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e),
-                       dp<int32_t>(m->conn), m->npe, (const int64_t *)nullptr, dp<int32_t>(ccount),
-                       (uint32_t *)nullptr);
-    FEASM_HIP(hipGetLastError());
-    P->cptr = dev_alloc((P->nblocks + 1) * sizeof(int64_t));
-    exclusive_scan_i32_to_i64(ctx, dp<int32_t>(ccount), dp<int64_t>(P->cptr), P->nblocks + 1);
-    P->ncontrib = read_i64(ctx, dp<int64_t>(P->cptr) + P->nblocks);
-    P->contrib = dev_alloc(P->ncontrib * sizeof(uint32_t));
-    hipLaunchKernelGGL(k_contrib, dim3(grid_for(P->nblocks, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(P->bcol),
-                       dp<int32_t>(P->adj), P->nblocks, dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e),
-                       dp<int32_t>(m->conn), m->npe, dp<int64_t>(P->cptr), (int32_t *)nullptr,
-                       dp<uint32_t>(P->contrib));
-    FEASM_HIP(hipGetLastError());
     // 4. DOF-level CSC
     P->n = nn * ndn;
     P->nnz = P->nblocks * ndn * ndn;
     FEASM_REQUIRE(P->n < (1ll << 31), FEASM_EUNSUPPORTED, "more than 2^31 DOFs");
     P->colptr = dev_alloc((P->n + 1) * sizeof(int64_t));
-    P->rowval = dev_alloc(P->nnz * sizeof(int32_t));
+    P->rowval = dev_alloc(std::max<size_t>(P->nnz * sizeof(int32_t), 4));
+    FEASM_HIP(hipMemsetAsync(P->colptr->p, 0, (P->n + 1) * sizeof(int64_t), ctx->stream));
     hipLaunchKernelGGL(k_dof_pattern, dim3(grid_for(nn, 256)), dim3(256), 0, ctx->stream, dp<int64_t>(P->adj_ptr),
                        dp<int32_t>(P->adj), nn, ndn, dp<int64_t>(P->colptr), dp<int32_t>(P->rowval));
     FEASM_HIP(hipGetLastError());

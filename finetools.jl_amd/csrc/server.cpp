@@ -1,0 +1,462 @@
+// Assembly server: continuous batching of assembly requests on one GPU.
+//
+// Requests (a new mesh to assemble, or a re-assembly of a resident mesh with
+// new material values) are queued as they arrive.  A fixed set of worker
+// threads, each with its own HIP stream, take the oldest admissible request
+// as soon as they are free: a request is admitted when its estimated device
+// memory fits the server's HBM budget (resident meshes not in use are
+// evicted, least recently used first, to make room).  Work of different
+// requests therefore overlaps on the device, and a request that arrives while
+// others run starts as soon as a worker and memory free up.
+//
+// Results are deterministic: every kernel sums in a fixed order, so a request
+// gives bitwise the same matrix alone or in any mix.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+
+#include "common.hpp"
+#include "fused.hpp"
+
+namespace feasm {
+FormParams make_form_params(int form, int ndn, const double *coeff, int ncoeff);
+void numeric_into_ctx(feasm_ctx *ctx, feasm_mesh *m, const FormParams &fp, int rule, int order, int precision,
+                      double *nzval);
+}  // namespace feasm
+
+using namespace feasm;
+
+extern "C" int feasm_ctx_create(int device, feasm_ctx **out);
+extern "C" int feasm_ctx_destroy(feasm_ctx *ctx);
+
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Resident {
+    feasm_mesh *mesh = nullptr;
+    size_t bytes = 0;
+    int users = 0;
+    double last_use = 0;
+};
+
+struct Request {
+    int64_t id = 0;
+    bool reassembly = false;
+    // new mesh
+    int etype = 0, npe = 0, ndn = 0;
+    int64_t nelem = 0, nnodes = 0;
+    std::vector<int32_t> conn0;
+    std::vector<double> xyz;
+    bool keep = false;
+    // both
+    int64_t mesh_id = -1;
+    FormParams fp;
+    int rule = 0, order = 0, precision = 0;
+    size_t est = 0;
+    double t_submit = 0, t_start = 0, t_done = 0;
+    int status = 0;  // 0 queued, 1 running, 2 done, < 0 failed (feasm status)
+    std::string error;
+    std::unique_ptr<feasm_matrix> result;
+};
+
+size_t estimate_bytes(int npe, int ndn, int64_t nelem, int64_t nnodes) {
+    const double deg = (npe == 8) ? 27.0 : 26.0;  // upper bound of the mean node degree of block meshes
+    const double nb = deg * (double)nnodes, nnz = nb * ndn * ndn;
+    const double b = 4.0 * npe * nelem * 3  // connectivity, node->element map, tiles
+                     + 48.0 * nnodes        // coordinates, pointers, DOFs
+                     + 8.0 * nb             // adjacency, block columns
+                     + 12.0 * nnz           // rowval + nzval
+                     + 8.0 * nnodes * ndn;  // colptr
+    return (size_t)(1.2 * b) + (8u << 20);
+}
+
+}  // namespace
+
+struct feasm_server {
+    int device = 0;
+    size_t budget = 0;
+    std::atomic<size_t> in_use{0};
+    std::vector<std::thread> workers;
+    std::vector<feasm_ctx *> ctxs;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request *> queue;
+    std::map<int64_t, std::unique_ptr<Request>> requests;
+    std::map<int64_t, Resident> residents;
+    std::map<int64_t, int> pending_keep;  // kept meshes still being assembled: mesh id -> DOFs per node
+    int64_t next_id = 1, next_mesh = 1;
+    bool stop = false;
+    int running = 0;
+    int64_t completed = 0;
+    std::string last_error;
+
+    // pick the oldest admissible request (looking a few past the head so a
+    // large request does not block small ones); called with mu held
+    Request *pick() {
+        const size_t window = std::min<size_t>(queue.size(), 8);
+        for (size_t k = 0; k < window; k++) {
+            Request *r = queue[k];
+            if (r->reassembly) {
+                auto it = residents.find(r->mesh_id);
+                if (it == residents.end()) {
+                    if (pending_keep.count(r->mesh_id)) continue;  // its mesh is still being assembled
+                    queue.erase(queue.begin() + k);                 // evicted or unknown: fail it
+                    return r;
+                }
+                if (it->second.users > 0) continue;  // one worker per mesh at a time
+            }
+            size_t need = r->est;
+            if (in_use + need > budget) evict(need);
+            if (in_use + need <= budget || running == 0) {
+                queue.erase(queue.begin() + k);
+                return r;
+            }
+        }
+        return nullptr;
+    }
+
+    void evict(size_t need) {
+        while (in_use + need > budget) {
+            int64_t victim = -1;
+            double oldest = 1e300;
+            for (auto &kv : residents)
+                if (kv.second.users == 0 && kv.second.last_use < oldest) {
+                    oldest = kv.second.last_use;
+                    victim = kv.first;
+                }
+            if (victim < 0) return;
+            Resident &v = residents[victim];
+            in_use -= v.bytes;
+            feasm_mesh *m = v.mesh;
+            residents.erase(victim);
+            (void)hipStreamSynchronize(m->ctx->stream);
+            delete m;
+        }
+    }
+
+    void work(int w) {
+        feasm_ctx *ctx = ctxs[w];
+        FEASM_HIP(hipSetDevice(device));
+        for (;;) {
+            Request *r = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stop || (r = pick()) != nullptr; });
+                if (!r) return;
+                r->status = 1;
+                r->t_start = now_s();
+                running++;
+                if (!r->reassembly) in_use += r->est;
+                if (r->reassembly && residents.count(r->mesh_id)) residents[r->mesh_id].users++;
+            }
+            run_one(ctx, r);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                running--;
+                completed++;
+                if (r->reassembly) {
+                    auto it = residents.find(r->mesh_id);
+                    if (it != residents.end()) {
+                        it->second.users--;
+                        it->second.last_use = now_s();
+                    }
+                }
+            }
+            cv_work.notify_all();
+            cv_done.notify_all();
+        }
+    }
+
+    void run_one(feasm_ctx *ctx, Request *r) {
+        try {
+            feasm_mesh *m = nullptr;
+            std::unique_ptr<feasm_mesh> owned;
+            if (r->reassembly) {
+                std::lock_guard<std::mutex> lk(mu);
+                auto it = residents.find(r->mesh_id);
+                FEASM_REQUIRE(it != residents.end(), FEASM_ESTATE, "resident mesh was evicted or never existed");
+                m = it->second.mesh;
+            } else {
+                owned.reset(new feasm_mesh);
+                m = owned.get();
+                m->ctx = ctx;
+                m->etype = r->etype;
+                m->npe = r->npe;
+                m->nelem = r->nelem;
+                m->nnodes = r->nnodes;
+                m->conn = dev_alloc(std::max<size_t>(r->conn0.size() * 4, 4));
+                m->xyz = dev_alloc(std::max<size_t>(r->xyz.size() * 8, 8));
+                FEASM_HIP(hipMemcpyAsync(m->conn->p, r->conn0.data(), r->conn0.size() * 4, hipMemcpyHostToDevice,
+                                         ctx->stream));
+                FEASM_HIP(hipMemcpyAsync(m->xyz->p, r->xyz.data(), r->xyz.size() * 8, hipMemcpyHostToDevice,
+                                         ctx->stream));
+                m->ndn = r->ndn;
+                m->nalldofs = r->nnodes * r->ndn;
+                m->nodemajor = true;
+                FEASM_HIP(hipStreamSynchronize(ctx->stream));
+                m->pattern = build_pattern(m, m->ndn);
+            }
+            std::unique_ptr<feasm_matrix> M(new feasm_matrix);
+            M->ctx = ctx;
+            M->m = M->n = m->pattern->n;
+            M->nnz = m->pattern->nnz;
+            M->colptr = m->pattern->colptr;
+            M->rowval = m->pattern->rowval;
+            M->pattern = m->pattern;
+            M->form_class = m->ndn;
+            M->nzval = dev_alloc(std::max<size_t>(M->nnz * 8, 8));
+            numeric_into_ctx(ctx, m, r->fp, r->rule, r->order, r->precision, dp<double>(M->nzval));
+            FEASM_HIP(hipStreamSynchronize(ctx->stream));
+            std::lock_guard<std::mutex> lk(mu);
+            r->result = std::move(M);
+            r->t_done = now_s();
+            r->status = 2;
+            if (!r->reassembly) {
+                in_use -= r->est;
+                pending_keep.erase(r->mesh_id);
+                if (r->keep) {
+                    Resident res;
+                    res.mesh = owned.release();
+                    res.bytes = r->est;
+                    res.last_use = now_s();
+                    in_use += res.bytes;
+                    residents[r->mesh_id] = res;
+                }
+            }
+            r->conn0.clear();
+            r->conn0.shrink_to_fit();
+            r->xyz.clear();
+            r->xyz.shrink_to_fit();
+        } catch (const feasm::Error &e) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!r->reassembly) {
+                in_use -= r->est;
+                pending_keep.erase(r->mesh_id);
+            }
+            r->status = e.code;
+            r->error = e.what();
+            r->t_done = now_s();
+        } catch (const std::exception &e) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!r->reassembly) {
+                in_use -= r->est;
+                pending_keep.erase(r->mesh_id);
+            }
+            r->status = FEASM_EINVAL;
+            r->error = e.what();
+            r->t_done = now_s();
+        }
+    }
+
+    int64_t enqueue(std::unique_ptr<Request> r) {
+        std::lock_guard<std::mutex> lk(mu);
+        r->id = next_id++;
+        r->t_submit = now_s();
+        Request *p = r.get();
+        requests[p->id] = std::move(r);
+        queue.push_back(p);
+        cv_work.notify_one();
+        return p->id;
+    }
+};
+
+namespace {
+thread_local std::string g_srv_error;
+int sfail(feasm_server *s, int code, const char *what) {
+    g_srv_error = what;
+    if (s) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->last_error = what;
+    }
+    return code;
+}
+#define S_TRY try {
+#define S_CATCH(sp)                                  \
+    }                                                \
+    catch (const feasm::Error &e) {                  \
+        return sfail((sp), e.code, e.what());        \
+    }                                                \
+    catch (const std::exception &e) {                \
+        return sfail((sp), FEASM_EINVAL, e.what());  \
+    }                                                \
+    return FEASM_OK;
+}  // namespace
+
+extern "C" int feasm_server_create(int device, int workers, double hbm_fraction, feasm_server **out) {
+    S_TRY
+    FEASM_REQUIRE(out && workers >= 1 && workers <= 16, FEASM_EINVAL, "1..16 workers");
+    FEASM_REQUIRE(hbm_fraction > 0 && hbm_fraction <= 0.95, FEASM_EINVAL, "hbm_fraction in (0, 0.95]");
+    std::unique_ptr<feasm_server> s(new feasm_server);
+    s->device = device;
+    FEASM_HIP(hipSetDevice(device));
+    size_t fr = 0, tot = 0;
+    FEASM_HIP(hipMemGetInfo(&fr, &tot));
+    s->budget = (size_t)(hbm_fraction * (double)fr);
+    for (int w = 0; w < workers; w++) {
+        feasm_ctx *c = nullptr;
+        const int rc = feasm_ctx_create(device, &c);
+        FEASM_REQUIRE(rc == FEASM_OK, rc, "could not create a worker context");
+        s->ctxs.push_back(c);
+    }
+    feasm_server *sp = s.get();
+    for (int w = 0; w < workers; w++) s->workers.emplace_back([sp, w] { sp->work(w); });
+    *out = s.release();
+    S_CATCH(nullptr)
+}
+
+extern "C" int feasm_server_submit_mesh(feasm_server *s, int elem_type, int64_t nelem, int npe, const int64_t *conn,
+                                        int64_t nnodes, const double *xyz_rowmajor, int ndn, int form, int rule,
+                                        int order, const double *coeff, int ncoeff, int precision,
+                                        int keep_resident, int64_t *request_id, int64_t *mesh_id) {
+    S_TRY
+    FEASM_REQUIRE(s && conn && xyz_rowmajor && request_id, FEASM_EINVAL, "null argument");
+    FEASM_REQUIRE(npe == (elem_type == FEASM_H8 ? 8 : 10) && (elem_type == FEASM_H8 || elem_type == FEASM_T10),
+                  FEASM_EINVAL, "element type / nodes per element");
+    FEASM_REQUIRE(nelem > 0 && nnodes > 0 && nnodes < (1ll << 31) && nelem < (1ll << 25), FEASM_EUNSUPPORTED,
+                  "mesh size out of range");
+    FEASM_REQUIRE(precision == FEASM_FP64 || precision == FEASM_MIXED, FEASM_EINVAL, "unknown precision");
+    std::unique_ptr<Request> r(new Request);
+    r->etype = elem_type;
+    r->npe = npe;
+    r->ndn = ndn;
+    r->nelem = nelem;
+    r->nnodes = nnodes;
+    r->conn0.resize(nelem * npe);
+    for (int64_t i = 0; i < nelem * npe; i++) {
+        FEASM_REQUIRE(conn[i] >= 1 && conn[i] <= nnodes, FEASM_EINVAL, "connectivity refers to a missing node");
+        r->conn0[i] = (int32_t)(conn[i] - 1);
+    }
+    r->xyz.assign(xyz_rowmajor, xyz_rowmajor + 3 * nnodes);
+    r->fp = make_form_params(form, ndn, coeff, ncoeff);
+    r->rule = rule;
+    r->order = order;
+    r->precision = precision;
+    r->keep = keep_resident != 0;
+    r->est = estimate_bytes(npe, ndn, nelem, nnodes);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->mesh_id = s->next_mesh++;
+        if (r->keep) s->pending_keep[r->mesh_id] = ndn;
+    }
+    if (mesh_id) *mesh_id = r->mesh_id;
+    *request_id = s->enqueue(std::move(r));
+    S_CATCH(s)
+}
+
+extern "C" int feasm_server_submit_reassembly(feasm_server *s, int64_t mesh_id, int form, int rule, int order,
+                                              const double *coeff, int ncoeff, int precision, int64_t *request_id) {
+    S_TRY
+    FEASM_REQUIRE(s && request_id, FEASM_EINVAL, "null argument");
+    FEASM_REQUIRE(precision == FEASM_FP64 || precision == FEASM_MIXED, FEASM_EINVAL, "unknown precision");
+    int ndn = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->residents.find(mesh_id);
+        if (it != s->residents.end()) {
+            ndn = it->second.mesh->ndn;
+        } else {
+            auto pk = s->pending_keep.find(mesh_id);
+            FEASM_REQUIRE(pk != s->pending_keep.end(), FEASM_ESTATE, "no such resident mesh (not kept or evicted)");
+            ndn = pk->second;
+        }
+    }
+    std::unique_ptr<Request> r(new Request);
+    r->reassembly = true;
+    r->mesh_id = mesh_id;
+    r->fp = make_form_params(form, ndn, coeff, ncoeff);
+    r->rule = rule;
+    r->order = order;
+    r->precision = precision;
+    r->est = 0;
+    *request_id = s->enqueue(std::move(r));
+    S_CATCH(s)
+}
+
+// Wait for a request; status: 2 done, < 0 failed (the feasm status code);
+// latency = completion - submission, queue = start - submission (ms).
+extern "C" int feasm_server_wait(feasm_server *s, int64_t request_id, double timeout_s, int *status,
+                                 double *latency_ms, double *queue_ms) {
+    S_TRY
+    FEASM_REQUIRE(s, FEASM_EINVAL, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->requests.find(request_id);
+    FEASM_REQUIRE(it != s->requests.end(), FEASM_EINVAL, "unknown request");
+    Request *r = it->second.get();
+    const auto deadline = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
+    while (r->status == 0 || r->status == 1) {
+        if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) break;
+    }
+    if (status) *status = r->status;
+    if (latency_ms) *latency_ms = (r->status == 2 || r->status < 0) ? 1e3 * (r->t_done - r->t_submit) : -1;
+    if (queue_ms) *queue_ms = r->t_start > 0 ? 1e3 * (r->t_start - r->t_submit) : -1;
+    if (r->status < 0) {
+        s->last_error = r->error;
+        g_srv_error = r->error;
+    }
+    S_CATCH(s)
+}
+
+// The result matrix of a finished request (owned by the server until
+// feasm_server_release).
+extern "C" int feasm_server_result(feasm_server *s, int64_t request_id, feasm_matrix **out) {
+    S_TRY
+    FEASM_REQUIRE(s && out, FEASM_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->requests.find(request_id);
+    FEASM_REQUIRE(it != s->requests.end(), FEASM_EINVAL, "unknown request");
+    FEASM_REQUIRE(it->second->status == 2, FEASM_ESTATE, it->second->status < 0 ? it->second->error : "not done");
+    *out = it->second->result.get();
+    S_CATCH(s)
+}
+
+extern "C" int feasm_server_release(feasm_server *s, int64_t request_id) {
+    S_TRY
+    FEASM_REQUIRE(s, FEASM_EINVAL, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->requests.find(request_id);
+    FEASM_REQUIRE(it != s->requests.end(), FEASM_EINVAL, "unknown request");
+    FEASM_REQUIRE(it->second->status == 2 || it->second->status < 0, FEASM_ESTATE, "request still pending");
+    s->requests.erase(it);
+    S_CATCH(s)
+}
+
+extern "C" int feasm_server_stats(feasm_server *s, int64_t *queued, int64_t *running, int64_t *completed,
+                                  int64_t *residents, int64_t *bytes_in_use, int64_t *budget) {
+    S_TRY
+    FEASM_REQUIRE(s, FEASM_EINVAL, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (queued) *queued = (int64_t)s->queue.size();
+    if (running) *running = s->running;
+    if (completed) *completed = s->completed;
+    if (residents) *residents = (int64_t)s->residents.size();
+    if (bytes_in_use) *bytes_in_use = (int64_t)s->in_use.load();
+    if (budget) *budget = (int64_t)s->budget;
+    S_CATCH(s)
+}
+
+extern "C" const char *feasm_server_last_error(feasm_server *s) {
+    return s ? s->last_error.c_str() : g_srv_error.c_str();
+}
+
+extern "C" int feasm_server_destroy(feasm_server *s) {
+    if (!s) return FEASM_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->workers) t.join();
+    s->requests.clear();
+    for (auto &kv : s->residents) delete kv.second.mesh;
+    s->residents.clear();
+    for (auto *c : s->ctxs) feasm_ctx_destroy(c);
+    delete s;
+    return FEASM_OK;
+}

@@ -6,10 +6,10 @@ host-side mirror of the reference's interface over that ABI.
 """
 from ._lib import (DIFFUSION_ISO, DIFFUSION_MAT, DOT, FP64, H8, LIN_ELASTIC, MIXED, RULE_GAUSS, RULE_TET, T10,
                    FeasmError, LIB_PATH, lib)
-from .core import Batch, Context, DeviceMatrix, DeviceMesh, default_context, gen_h8block, gen_perturb, gen_t10block
+from .core import Batch, Context, DeviceMatrix, Server, DeviceMesh, default_context, gen_h8block, gen_perturb, gen_t10block
 
 __all__ = [
     "DIFFUSION_ISO", "DIFFUSION_MAT", "DOT", "FP64", "H8", "LIN_ELASTIC", "MIXED", "RULE_GAUSS", "RULE_TET", "T10",
-    "FeasmError", "LIB_PATH", "lib", "Batch", "Context", "DeviceMatrix", "DeviceMesh", "default_context", "gen_h8block",
+    "FeasmError", "LIB_PATH", "lib", "Batch", "Server", "Context", "DeviceMatrix", "DeviceMesh", "default_context", "gen_h8block",
     "gen_perturb", "gen_t10block",
 ]

@@ -69,6 +69,16 @@ SIGNATURES = {
     "feasm_batch_autotune": (I32, [P, ctypes.c_char_p, ctypes.c_char_p, I32, ctypes.POINTER(ctypes.c_int),
                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]),
     "feasm_batch_destroy": (I32, [P]),
+    "feasm_server_create": (I32, [I32, I32, DBL, PP]),
+    "feasm_server_submit_mesh": (I32, [P, I32, I64, I32, P, I64, P, I32, I32, I32, I32, P, I32, I32, I32, PI64, PI64]),
+    "feasm_server_submit_reassembly": (I32, [P, I64, I32, I32, I32, P, I32, I32, PI64]),
+    "feasm_server_wait": (I32, [P, I64, DBL, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double)]),
+    "feasm_server_result": (I32, [P, I64, PP]),
+    "feasm_server_release": (I32, [P, I64]),
+    "feasm_server_stats": (I32, [P, PI64, PI64, PI64, PI64, PI64, PI64]),
+    "feasm_server_last_error": (ctypes.c_char_p, [P]),
+    "feasm_server_destroy": (I32, [P]),
     "feasm_gen_h8block": (I32, [DBL, DBL, DBL, I64, I64, I64, PI64, PI64, P, P]),
     "feasm_gen_t10block": (I32, [DBL, DBL, DBL, I64, I64, I64, DBL, DBL, U64, PI64, PI64, P, P]),
     "feasm_gen_perturb": (I32, [I64, P, P, P, DBL, U64]),

@@ -96,7 +96,8 @@ class DeviceMatrix:
         colptr = np.empty(n + 1, np.int64)
         rowval = np.empty(nnz, np.int64)
         nzval = np.empty(nnz, np.float64)
-        check(lib().feasm_matrix_copy_csc(self.h, base, ptr(colptr), ptr(rowval), ptr(nzval)), self.ctx.h)
+        check(lib().feasm_matrix_copy_csc(self.h, base, ptr(colptr), ptr(rowval), ptr(nzval)),
+              self.ctx.h if self.ctx is not None else None)
         return colptr, rowval, nzval
 
     def to_scipy(self):
@@ -230,6 +231,66 @@ class Batch:
                                          key.encode() if key else None, reps, ctypes.byref(n), ctypes.byref(kb),
                                          ctypes.byref(ms)), self.ctx.h)
         return {"max_nodes": n.value, "budget_kib": kb.value, "ms": ms.value}
+
+
+class Server:
+    """Continuous-batching assembly server (feasm_server)."""
+
+    def __init__(self, device=0, workers=4, hbm_fraction=0.5):
+        h = ctypes.c_void_p()
+        rc = lib().feasm_server_create(device, workers, hbm_fraction, ctypes.byref(h))
+        if rc != L.OK:
+            raise L.FeasmError(rc, lib().feasm_server_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc != L.OK:
+            raise L.FeasmError(rc, lib().feasm_server_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().feasm_server_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def submit_mesh(self, etype, conn, xyz, ndn, form, rule, order, coeff, precision=L.FP64, keep=False):
+        conn = np.ascontiguousarray(conn, dtype=np.int64)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        c = np.ascontiguousarray(np.atleast_1d(coeff), dtype=np.float64).ravel()
+        rid, mid = ctypes.c_int64(), ctypes.c_int64()
+        self._check(lib().feasm_server_submit_mesh(self.h, etype, conn.shape[0], conn.shape[1], ptr(conn),
+                                                   xyz.shape[0], ptr(xyz), ndn, form, rule, order, ptr(c), len(c),
+                                                   precision, 1 if keep else 0, ctypes.byref(rid),
+                                                   ctypes.byref(mid)))
+        return rid.value, mid.value
+
+    def submit_reassembly(self, mesh_id, form, rule, order, coeff, precision=L.FP64):
+        c = np.ascontiguousarray(np.atleast_1d(coeff), dtype=np.float64).ravel()
+        rid = ctypes.c_int64()
+        self._check(lib().feasm_server_submit_reassembly(self.h, mesh_id, form, rule, order, ptr(c), len(c),
+                                                         precision, ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid, timeout=600.0):
+        st, lat, qm = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        self._check(lib().feasm_server_wait(self.h, rid, timeout, ctypes.byref(st), ctypes.byref(lat),
+                                            ctypes.byref(qm)))
+        return st.value, lat.value, qm.value
+
+    def result(self, rid):
+        h = ctypes.c_void_p()
+        self._check(lib().feasm_server_result(self.h, rid, ctypes.byref(h)))
+        return DeviceMatrix(None, h, owner=self)
+
+    def release(self, rid):
+        self._check(lib().feasm_server_release(self.h, rid))
+
+    def stats(self):
+        v = [ctypes.c_int64() for _ in range(6)]
+        self._check(lib().feasm_server_stats(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(["queued", "running", "completed", "residents", "bytes_in_use", "budget"],
+                        [x.value for x in v]))
 
 
 def sparse(ctx, I, J, V, m, n):

@@ -71,6 +71,7 @@ typedef struct feasm_mesh feasm_mesh;
 typedef struct feasm_matrix feasm_matrix;
 typedef struct feasm_coo feasm_coo;
 typedef struct feasm_batch feasm_batch;
+typedef struct feasm_server feasm_server;
 
 /* ---- context ------------------------------------------------------------ */
 /* No reference counterpart (the reference is host-only Julia): owns the HIP
@@ -255,6 +256,37 @@ int feasm_batch_stats(feasm_batch *b, int64_t *nmeshes, int64_t *nelem, int64_t 
 int feasm_batch_autotune(feasm_batch *b, const char *db_path, const char *key, int reps, int *best_nodes,
                          int *best_budget_kib, double *best_ms);
 int feasm_batch_destroy(feasm_batch *b);
+
+/* ---- assembly server (continuous batching; no reference counterpart) --- */
+/* Worker threads (one HIP stream each) take queued requests as soon as a
+ * worker and device memory are free; admission is by an HBM budget of
+ * hbm_fraction of the free memory at creation, evicting least recently used
+ * resident meshes that are not in use.  Results are bitwise independent of
+ * what else runs. */
+int feasm_server_create(int device, int workers, double hbm_fraction, feasm_server **out);
+/* Queue the assembly of a new mesh (arrays are copied; conn 1-based row-major
+ * nelem x npe, xyz row-major nnodes x 3, node-major DOFs with ndn per node).
+ * keep_resident != 0 keeps mesh, pattern and tiles on the device for later
+ * re-assembly under *mesh_id. */
+int feasm_server_submit_mesh(feasm_server *s, int elem_type, int64_t nelem, int npe, const int64_t *conn,
+                             int64_t nnodes, const double *xyz_rowmajor, int ndn, int form, int rule, int order,
+                             const double *coeff, int ncoeff, int precision, int keep_resident, int64_t *request_id,
+                             int64_t *mesh_id);
+/* Queue a re-assembly of a resident mesh with new coefficients (pattern and
+ * tiles reused: numeric phase only). */
+int feasm_server_submit_reassembly(feasm_server *s, int64_t mesh_id, int form, int rule, int order,
+                                   const double *coeff, int ncoeff, int precision, int64_t *request_id);
+/* Wait up to timeout_s; *status 2 = done, < 0 = failed (feasm status);
+ * latency and queueing time in ms from submission. */
+int feasm_server_wait(feasm_server *s, int64_t request_id, double timeout_s, int *status, double *latency_ms,
+                      double *queue_ms);
+/* Result of a finished request; owned by the server until released. */
+int feasm_server_result(feasm_server *s, int64_t request_id, feasm_matrix **out);
+int feasm_server_release(feasm_server *s, int64_t request_id);
+int feasm_server_stats(feasm_server *s, int64_t *queued, int64_t *running, int64_t *completed, int64_t *residents,
+                       int64_t *bytes_in_use, int64_t *budget);
+const char *feasm_server_last_error(feasm_server *s);
+int feasm_server_destroy(feasm_server *s);
 
 #ifdef __cplusplus
 }

@@ -328,3 +328,69 @@ def test_batched_launch_matches_single_and_oracle_with_graph_replays(ctx):
         st = batch.stats()
         assert st["meshes"] == 6 and st["tiles"] > 0
         batch.close()
+
+
+def test_server_results_same_alone_and_in_the_mix():
+    """Continuous batching: each request's matrix is bitwise the same when it
+    runs alone and when it runs mixed with others (and matches the oracle)."""
+    rng = np.random.default_rng(21)
+    reqs = []
+    for i in range(8):
+        if i % 2 == 0:
+            n = tuple(int(x) for x in rng.integers(3, 9, 3))
+            conn, xyz = mesh_for(O.H8, n, 0.1, 300 + i)
+            et, rule = O.H8, (L.RULE_GAUSS, 2)
+        else:
+            n = tuple(int(x) for x in rng.integers(2, 5, 3))
+            conn, xyz = mesh_for(O.T10, n, 0.1, 300 + i)
+            et, rule = O.T10, (L.RULE_TET, 4)
+        if i % 4 < 2:
+            form, ndn, coeff = L.DIFFUSION_ISO, 1, [float(rng.uniform(0.5, 2))]
+        else:
+            form, ndn, coeff = L.LIN_ELASTIC, 3, O.iso_D(float(rng.uniform(1e3, 1e5)), float(rng.uniform(0.2, 0.45)))
+        reqs.append((et, conn, xyz, ndn, form, rule, np.ravel(coeff)))
+    for precision, tol in ((L.FP64, FP64_TOL), (L.MIXED, MIXED_TOL)):
+        say("server precision", precision)
+        alone = []
+        srv = core.Server(0, workers=1, hbm_fraction=0.5)
+        for (et, conn, xyz, ndn, form, rule, coeff) in reqs:
+            rid, _ = srv.submit_mesh(et, conn, xyz, ndn, form, rule[0], rule[1], coeff, precision)
+            st, lat, _ = srv.wait(rid)
+            assert st == 2
+            alone.append(srv.result(rid).csc_arrays(1))
+            srv.release(rid)
+        srv.close()
+        srv = core.Server(0, workers=4, hbm_fraction=0.5)
+        ids = []
+        for k in range(3):  # each request three times, interleaved, some kept resident
+            for j, (et, conn, xyz, ndn, form, rule, coeff) in enumerate(reqs):
+                rid, mid = srv.submit_mesh(et, conn, xyz, ndn, form, rule[0], rule[1], coeff, precision,
+                                           keep=(k == 0))
+                ids.append((j, rid, mid if k == 0 else None))
+        mids = {}
+        for j, rid, mid in ids:
+            st, lat, _ = srv.wait(rid)
+            assert st == 2, (j, st)
+            got = srv.result(rid).csc_arrays(1)
+            for x, y in zip(got, alone[j]):
+                assert np.array_equal(x, y), j
+            srv.release(rid)
+            if mid is not None:
+                mids[j] = mid
+        # re-assembly of resident meshes with the original coefficients
+        re = [(j, srv.submit_reassembly(mids[j], reqs[j][4], reqs[j][5][0], reqs[j][5][1], reqs[j][6], precision))
+              for j in mids]
+        for j, rid in re:
+            st, lat, _ = srv.wait(rid)
+            assert st == 2
+            got = srv.result(rid).csc_arrays(1)
+            assert np.array_equal(got[2], alone[j][2])
+            srv.release(rid)
+        st = srv.stats()
+        assert st["completed"] == len(ids) + len(re) and st["residents"] == len(mids)
+        srv.close()
+        # and against the oracle
+        for j, (et, conn, xyz, ndn, form, rule, coeff) in enumerate(reqs):
+            ocp, orv, onz = oracle_csc(et, form, conn, xyz, ndn, coeff)
+            assert np.array_equal(alone[j][0], ocp) and np.array_equal(alone[j][1], orv)
+            assert rel_fro(alone[j][2], onz) <= tol
