@@ -145,7 +145,8 @@ def bench_batched(feasm, core, nmesh=256, seed=4, steps=5, tune_db=None):
             batch.add(m, form, feasm.RULE_GAUSS, 2, coeff, precision=prec)
         t0 = time.perf_counter()
         tuned = batch.autotune(db_path=tune_db, key="h8-batch256-%s" % label, reps=3)
-        batch.prepare(use_graph=True, max_nodes=tuned["max_nodes"], budget_kib=tuned["budget_kib"])
+        batch.prepare(use_graph=True, max_nodes=tuned["max_nodes"], budget_kib=tuned["budget_kib"],
+                      threads=tuned["threads"])
         tprep = time.perf_counter() - t0
         batch.run()
         ctx.synchronize()
@@ -353,11 +354,11 @@ def main():
         "nnz_per_s": nnz / (ms_step * 1e-3),
         "first_assembly_ms": 1e3 * (t_sym + t_first_numeric),
         "symbolic_ms": 1e3 * t_sym,
-        "kernels_ms": {"fused": el_ms} if ga_ms == 0.0 else {"element": el_ms, "gather": ga_ms},
+        "kernels_ms": {"fused": el_ms} if ntiles > 0 else {"element": el_ms, "gather": ga_ms},
         "tiles": ntiles,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_fused (fused tile kernel, whole numeric phase)" if ga_ms == 0.0
+                     "kernel": "k_fused (fused tile kernel, whole numeric phase)" if ntiles > 0
                      else "k_element + k_gather (staged numeric phase)",
                      "algorithmic_bytes": bytes_alg},
         "served": None,

@@ -24,7 +24,7 @@ FormParams make_form_params(int form, int ndn, const double *coeff, int ncoeff);
 using namespace feasm;
 
 struct feasm_batch {
-    feasm_ctx *ctx = nullptr;
+    CtxRef ctx;
     struct Item {
         feasm_mesh *mesh;
         FormParams fp;
@@ -33,7 +33,7 @@ struct feasm_batch {
     };
     std::vector<Item> items;
     std::vector<BatchLaunch> launches;
-    int max_nodes = 0;
+    int max_nodes = 0, threads = 0;
     size_t budget = 0;
     bool graph_on = false;
     hipGraph_t graph = nullptr;
@@ -80,6 +80,7 @@ TileShape shape_for(feasm_batch *b, const feasm_batch::Item &it) {
     TileShape s = tile_shape_for(b->ctx, it.mesh->npe, qt.nqp, it.fp.form, it.precision);
     if (b->max_nodes > 0) s.max_nodes = std::min(64, b->max_nodes);
     if (b->budget > 0) s.budget = std::min(s.budget, b->budget);
+    if (b->threads == 256 || b->threads == 512) s.threads = b->threads;
     return s;
 }
 
@@ -185,10 +186,12 @@ extern "C" int feasm_batch_add(feasm_batch *b, feasm_mesh *mesh, int form, int r
     B_CATCH(b ? b->ctx : nullptr)
 }
 
-extern "C" int feasm_batch_prepare(feasm_batch *b, int use_graph, int max_nodes, int budget_kib) {
+extern "C" int feasm_batch_prepare(feasm_batch *b, int use_graph, int max_nodes, int budget_kib, int threads) {
     B_TRY
     FEASM_REQUIRE(b, FEASM_EINVAL, "null batch");
+    FEASM_REQUIRE(threads == 0 || threads == 256 || threads == 512, FEASM_EINVAL, "threads: 0, 256 or 512");
     b->max_nodes = max_nodes;
+    b->threads = threads;
     b->budget = budget_kib > 0 ? (size_t)budget_kib * 1024 : 0;
     prepare(b);
     b->graph_on = use_graph != 0;
@@ -252,7 +255,7 @@ extern "C" int feasm_batch_stats(feasm_batch *b, int64_t *nmeshes, int64_t *nele
 // database (one line per key: "key max_nodes budget_kib ms") when db_path is
 // given, and read back from it on later calls for the same key.
 extern "C" int feasm_batch_autotune(feasm_batch *b, const char *db_path, const char *key, int reps, int *best_nodes,
-                                    int *best_budget_kib, double *best_ms) {
+                                    int *best_budget_kib, int *best_threads, double *best_ms) {
     B_TRY
     FEASM_REQUIRE(b && !b->items.empty(), FEASM_ESTATE, "empty batch");
     if (reps < 1) reps = 3;
@@ -262,44 +265,57 @@ extern "C" int feasm_batch_autotune(feasm_batch *b, const char *db_path, const c
         while (std::getline(in, line)) {
             std::istringstream ss(line);
             std::string k;
-            int n = 0, kb = 0;
+            int n = 0, kb = 0, th = 0;
             double ms = 0;
-            if ((ss >> k >> n >> kb >> ms) && k == key) {
+            if ((ss >> k >> n >> kb >> th >> ms) && k == key) {
                 if (best_nodes) *best_nodes = n;
                 if (best_budget_kib) *best_budget_kib = kb;
+                if (best_threads) *best_threads = th;
                 if (best_ms) *best_ms = ms;
+                b->max_nodes = n;
+                b->budget = (size_t)kb * 1024;
+                b->threads = th;
+                prepare(b);
+                if (b->graph_on) capture(b);
+                FEASM_HIP(hipStreamSynchronize(b->ctx->stream));
                 return FEASM_OK;
             }
         }
     }
     const int nodes_c[] = {16, 32, 64};
     const int kib_c[] = {160, 80};
+    const int thr_c[] = {256, 512};
     double best = 1e300;
-    int bn = 64, bk = 160;
-    for (int kb : kib_c)
-        for (int n : nodes_c) {
-            b->max_nodes = n;
-            b->budget = (size_t)kb * 1024;
-            prepare(b);
-            run_direct(b);  // warm-up
-            const double ms = time_runs(b, reps);
-            if (ms < best) {
-                best = ms;
-                bn = n;
-                bk = kb;
+    int bn = 64, bk = 160, bt = 512;
+    for (int th : thr_c)
+        for (int kb : kib_c)
+            for (int n : nodes_c) {
+                b->max_nodes = n;
+                b->budget = (size_t)kb * 1024;
+                b->threads = th;
+                prepare(b);
+                run_direct(b);  // warm-up
+                const double ms = time_runs(b, reps);
+                if (ms < best) {
+                    best = ms;
+                    bn = n;
+                    bk = kb;
+                    bt = th;
+                }
             }
-        }
     b->max_nodes = bn;
     b->budget = (size_t)bk * 1024;
+    b->threads = bt;
     prepare(b);
     if (b->graph_on) capture(b);
     FEASM_HIP(hipStreamSynchronize(b->ctx->stream));
     if (db_path && key) {
         std::ofstream o(db_path, std::ios::app);
-        if (o) o << key << " " << bn << " " << bk << " " << best << "\n";
+        if (o) o << key << " " << bn << " " << bk << " " << bt << " " << best << "\n";
     }
     if (best_nodes) *best_nodes = bn;
     if (best_budget_kib) *best_budget_kib = bk;
+    if (best_threads) *best_threads = bt;
     if (best_ms) *best_ms = best;
     B_CATCH(b ? b->ctx : nullptr)
 }

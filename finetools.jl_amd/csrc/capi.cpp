@@ -139,7 +139,10 @@ void feasm::numeric_into_ctx(feasm_ctx *ctx, feasm_mesh *m, const FormParams &fp
     FEASM_REQUIRE(fused || m->numeric_path != 2, FEASM_EUNSUPPORTED,
                   "no fused kernel for this element type and rule");
     if (fused) {
-        const TileShape shape = tile_shape_for(ctx, m->npe, qt.nqp, fp.form, precision);
+        TileShape shape = tile_shape_for(ctx, m->npe, qt.nqp, fp.form, precision);
+        if (m->tile_nodes > 0) shape.max_nodes = std::min(64, m->tile_nodes);
+        if (m->tile_kib > 0) shape.budget = std::min(shape.budget, (size_t)m->tile_kib * 1024);
+        if (m->tile_threads == 256 || m->tile_threads == 512) shape.threads = m->tile_threads;
         auto it = m->tiles.find(shape.key());
         if (it == m->tiles.end()) it = m->tiles.emplace(shape.key(), build_tiles(m, shape)).first;
         Timer tm(ctx);
@@ -215,14 +218,19 @@ extern "C" int feasm_ctx_create(int device, feasm_ctx **out) {
     API_CATCH(nullptr)
 }
 
-extern "C" int feasm_ctx_destroy(feasm_ctx *ctx) {
-    if (!ctx) return FEASM_OK;
+void feasm_ctx_release(feasm_ctx *ctx) {
+    if (--ctx->refs > 0) return;
     (void)hipStreamSynchronize(ctx->stream);
     ctx->qtables.clear();
     ctx->scratch.reset();
     for (auto e : ctx->ring) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+extern "C" int feasm_ctx_destroy(feasm_ctx *ctx) {
+    if (!ctx) return FEASM_OK;
+    feasm_ctx_release(ctx);
     return FEASM_OK;
 }
 
@@ -410,6 +418,18 @@ extern "C" int feasm_mesh_set_numeric_path(feasm_mesh *m, int path) {
     API_CATCH(m ? m->ctx : nullptr)
 }
 
+extern "C" int feasm_mesh_set_tile_params(feasm_mesh *m, int max_nodes, int budget_kib, int threads) {
+    API_TRY
+    FEASM_REQUIRE(m, FEASM_EINVAL, "null mesh");
+    FEASM_REQUIRE(max_nodes >= 0 && max_nodes <= 64 && budget_kib >= 0 &&
+                      (threads == 0 || threads == 256 || threads == 512),
+                  FEASM_EINVAL, "max_nodes 0..64, budget_kib >= 0, threads 0/256/512");
+    m->tile_nodes = max_nodes;
+    m->tile_kib = budget_kib;
+    m->tile_threads = threads;
+    API_CATCH(m ? m->ctx : nullptr)
+}
+
 extern "C" int feasm_mesh_tile_info(feasm_mesh *m, int64_t *ntiles, int64_t *nhalo) {
     API_TRY
     FEASM_REQUIRE(m, FEASM_EINVAL, "null mesh");
@@ -533,7 +553,7 @@ extern "C" int feasm_matrix_destroy(feasm_matrix *mat) {
 // element matrix arrives from host code) and moved to the device once, at
 // makematrix!, where the device sparse() builds the CSC.
 struct feasm_coo {
-    feasm_ctx *ctx = nullptr;
+    CtxRef ctx;
     int64_t buffer_length = 0, buffer_pointer = 0;  // 1-based pointer as the reference
     int64_t row_nalldofs = 0, col_nalldofs = 0;
     bool nomatrixresult = false;

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -86,6 +87,10 @@ struct TileSet;
 
 // ---------------------------------------------------------------- handles
 struct feasm_ctx {
+    // handles made on a context (meshes, matrices, assemblers, batches) keep
+    // it alive: feasm_ctx_destroy drops the caller's reference and the
+    // context is freed with the last one
+    std::atomic<int> refs{1};
     int device = 0;
     hipStream_t stream = nullptr;
     std::string last_error;
@@ -107,6 +112,32 @@ struct feasm_ctx {
     void *scratch_get(size_t bytes);
 };
 
+// releases the context when the last reference goes (capi.cpp)
+void feasm_ctx_release(feasm_ctx *ctx);
+
+// counted reference to a context
+struct CtxRef {
+    feasm_ctx *p = nullptr;
+    CtxRef() = default;
+    CtxRef(feasm_ctx *c) : p(c) {
+        if (p) p->refs++;
+    }
+    CtxRef(const CtxRef &o) : CtxRef(o.p) {}
+    CtxRef &operator=(feasm_ctx *c) {
+        if (c) c->refs++;
+        feasm_ctx *old = p;
+        p = c;
+        if (old) feasm_ctx_release(old);
+        return *this;
+    }
+    CtxRef &operator=(const CtxRef &o) { return (*this = o.p); }
+    ~CtxRef() {
+        if (p) feasm_ctx_release(p);
+    }
+    operator feasm_ctx *() const { return p; }
+    feasm_ctx *operator->() const { return p; }
+};
+
 struct Pattern {
     int ndn = 0;
     int64_t nnodes = 0, nblocks = 0, ncontrib = 0, n = 0, nnz = 0;
@@ -120,7 +151,7 @@ struct Pattern {
 };
 
 struct feasm_mesh {
-    feasm_ctx *ctx = nullptr;
+    CtxRef ctx;
     int etype = 0, npe = 0;
     int64_t nelem = 0, nnodes = 0;
     feasm::DevPtr conn;     // int32 [nelem*npe] 0-based
@@ -136,11 +167,12 @@ struct feasm_mesh {
     // numeric path: 0 = automatic (fused tiles when the element/rule has a
     // fused kernel), 1 = staged element blocks + ordered gather, 2 = fused
     int numeric_path = 0;
+    int tile_nodes = 0, tile_kib = 0, tile_threads = 0;  // fused-kernel tile parameters (0 = default)
     std::map<int64_t, std::shared_ptr<feasm::TileSet>> tiles;  // keyed by TileShape::key()
 };
 
 struct feasm_matrix {
-    feasm_ctx *ctx = nullptr;
+    CtxRef ctx;
     int64_t m = 0, n = 0, nnz = 0;
     feasm::DevPtr colptr;  // int64 [n+1]
     feasm::DevPtr rowval;  // int32 [nnz]

@@ -30,7 +30,7 @@ namespace feasm {
 
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kMaxThreads = 512;
 
 __device__ constexpr int voigt(int i, int k) {
     return (i == k) ? i : ((i + k == 1) ? 3 : ((i + k == 2) ? 4 : 5));
@@ -45,8 +45,9 @@ struct FusedParams {
 
 // Everything one workgroup needs to process one tile of one mesh.
 struct TileJob {
-    const int32_t *node_ptr, *nodes, *node_cj, *halo_ptr, *halo, *cj_ptr;
+    const int32_t *node_ptr, *nodes, *node_cj, *halo_ptr, *halo, *cj_ptr, *hnode_ptr, *hnodes;
     const uint32_t *cj;
+    const uint16_t *lconn;
     const int32_t *conn;
     const double *xyz;
     const int64_t *adj_ptr;
@@ -60,9 +61,11 @@ __device__ __forceinline__ void fused_tile(const TileShape &shape, const TileJob
                                            unsigned char *smem) {
     constexpr bool GRAD = (FORM != FEASM_FORM_DOT);
     constexpr int SWP = (FORM == FEASM_FORM_LIN_ELASTIC || FORM == FEASM_FORM_DIFFUSION_MAT) ? 9 : 1;
-    constexpr int NSPLIT = 2;
+    constexpr int NSPLIT = 4;
     constexpr int APART = (NPE + NSPLIT - 1) / NSPLIT;
     const int32_t *__restrict__ node_ptr = job.node_ptr;
+    const int32_t *__restrict__ hnodes = job.hnodes;
+    const uint16_t *__restrict__ lconn = job.lconn;
     const int32_t *__restrict__ nodes = job.nodes;
     const int32_t *__restrict__ node_cj = job.node_cj;
     const int32_t *__restrict__ halo_ptr = job.halo_ptr;
@@ -76,12 +79,16 @@ __device__ __forceinline__ void fused_tile(const TileShape &shape, const TileJob
     double *__restrict__ nzval = job.nzval;
     const int64_t tile = job.tile;
     const int tid = threadIdx.x;
+    const int kThreads = blockDim.x;
 
     const int n0 = node_ptr[tile], P = node_ptr[tile + 1] - n0;
     const int h0 = halo_ptr[tile], H = halo_ptr[tile + 1] - h0;
     const int c0 = cj_ptr[tile], CJ = cj_ptr[tile + 1] - c0;
-    const TileShape::Layout L = shape.layout(H, CJ, P);
+    const int x0 = job.hnode_ptr[tile], NH = job.hnode_ptr[tile + 1] - x0;
+    const TileShape::Layout L = shape.layout(H, CJ, P, NH);
     double *sDN = reinterpret_cast<double *>(smem + L.dN);
+    double *sX = reinterpret_cast<double *>(smem + L.X);
+    uint16_t *sLc = reinterpret_cast<uint16_t *>(smem + L.lconn);
     int32_t *sNode = reinterpret_cast<int32_t *>(smem + L.node);
     int32_t *sNcj = reinterpret_cast<int32_t *>(smem + L.ncj);
     int32_t *sBP = reinterpret_cast<int32_t *>(smem + L.bp);
@@ -98,7 +105,11 @@ __device__ __forceinline__ void fused_tile(const TileShape &shape, const TileJob
         sNcj[i] = node_cj[n0 + i];
     }
     if (tid == 0) sNcj[P] = CJ;
-    for (int i = tid; i < H * NPE; i += kThreads) sConn[i] = conn[(int64_t)halo[h0 + i / NPE] * NPE + (i % NPE)];
+    for (int i = tid; i < H * NPE; i += kThreads) {
+        sConn[i] = conn[(int64_t)halo[h0 + i / NPE] * NPE + (i % NPE)];
+        sLc[i] = lconn[(int64_t)h0 * NPE + i];
+    }
+    for (int i = tid; i < NH * 3; i += kThreads) sX[i] = xyz[(int64_t)hnodes[x0 + i / 3] * 3 + (i % 3)];
     for (int i = tid; i < CJ; i += kThreads) sCJ[i] = cjg[c0 + i];
     if (tid < 64) {  // block prefix over the owned columns (P <= 64): one wavefront scan
         int d = 0;
@@ -119,14 +130,14 @@ __device__ __forceinline__ void fused_tile(const TileShape &shape, const TileJob
     // ---- phase 1: Jacobians, inverses and scaled gradients
     for (int task = tid; task < H * NQP; task += kThreads) {
         const int h = task / NQP, q = task % NQP;
-        const int32_t *cn = sConn + h * NPE;
+        const uint16_t *cn = sLc + h * NPE;
         const double *dq = sDN + q * NPE * 3;
         double J[9];
 #pragma unroll
         for (int i = 0; i < 9; i++) J[i] = 0.0;
 #pragma unroll
         for (int a = 0; a < NPE; a++) {
-            const double *x = xyz + (int64_t)cn[a] * 3;
+            const double *x = sX + (int)cn[a] * 3;
             const double x0 = x[0], x1 = x[1], x2 = x[2];
             const double d0 = dq[3 * a], d1 = dq[3 * a + 1], d2 = dq[3 * a + 2];
             J[0] += x0 * d0; J[1] += x0 * d1; J[2] += x0 * d2;
@@ -285,6 +296,11 @@ __device__ __forceinline__ void fused_tile(const TileShape &shape, const TileJob
                             for (int l = 0; l < 3; l++) s += kp.fp.coef[6 * voigt(i, k) + voigt(j, l)] * Mc[3 * k + l];
                         Kc[3 * i + j] = s;
                     }
+                if (m == node) {  // diagonal block: upper triangle mirrored, as complete_lt! does
+                    Kc[3] = Kc[1];
+                    Kc[6] = Kc[2];
+                    Kc[7] = Kc[5];
+                }
 #pragma unroll
                 for (int k = 0; k < 3; k++)
 #pragma unroll
@@ -315,7 +331,7 @@ __device__ __forceinline__ void fused_tile(const TileShape &shape, const TileJob
 }
 
 template <int NPE, int NQP, int FORM, int NDN, typename R>
-__global__ __launch_bounds__(kThreads) void k_fused(TileShape shape, TileJob job, int64_t ntiles, FusedParams kp) {
+__global__ __launch_bounds__(kMaxThreads) void k_fused(TileShape shape, TileJob job, int64_t ntiles, FusedParams kp) {
     extern __shared__ __align__(16) unsigned char smem[];
     // XCD-aware order: consecutive workgroups land on different XCDs, so hand
     // each XCD a contiguous range of (Morton-adjacent) tiles
@@ -337,7 +353,7 @@ struct BatchEntry {
 // range of the global tile list (all meshes' tiles, mesh after mesh); every
 // tile carries its mesh's form, coefficients and arrays.
 template <int NPE, int NQP, typename R>
-__global__ __launch_bounds__(kThreads) void k_fused_batch(const BatchEntry *__restrict__ entries,
+__global__ __launch_bounds__(kMaxThreads) void k_fused_batch(const BatchEntry *__restrict__ entries,
                                                           const int32_t *__restrict__ tile_entry, int64_t ntiles) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int64_t per = (ntiles + 7) / 8;
@@ -361,6 +377,8 @@ __global__ __launch_bounds__(kThreads) void k_fused_batch(const BatchEntry *__re
             default:
                 if (E.kp.fp.ndn == 3)
                     fused_tile<NPE, NQP, FEASM_FORM_DOT, 3, R>(E.shape, job, E.kp, smem);
+                else if (E.kp.fp.ndn == 2)
+                    fused_tile<NPE, NQP, FEASM_FORM_DOT, 2, R>(E.shape, job, E.kp, smem);
                 else
                     fused_tile<NPE, NQP, FEASM_FORM_DOT, 1, R>(E.shape, job, E.kp, smem);
                 break;
@@ -377,6 +395,9 @@ TileJob make_job(const feasm_mesh *m, const Pattern &p, const TileSet &t, double
     j.halo = dp<int32_t>(t.halo);
     j.cj_ptr = dp<int32_t>(t.cj_ptr);
     j.cj = dp<uint32_t>(t.cj);
+    j.hnode_ptr = dp<int32_t>(t.hnode_ptr);
+    j.hnodes = dp<int32_t>(t.hnodes);
+    j.lconn = dp<uint16_t>(t.lconn);
     j.conn = dp<int32_t>(m->conn);
     j.xyz = dp<double>(m->xyz);
     j.adj_ptr = dp<int64_t>(p.adj_ptr);
@@ -395,7 +416,7 @@ void go(feasm_ctx *ctx, const feasm_mesh *m, const Pattern &p, const TileSet &t,
     const int64_t per = (t.ntiles + 7) / 8;
     const unsigned grid = (unsigned)(8 * per);
     TileJob job = make_job(m, p, t, nzval);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), budget, ctx->stream, t.shape, job, t.ntiles, kp);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(t.shape.threads), budget, ctx->stream, t.shape, job, t.ntiles, kp);
     FEASM_HIP(hipGetLastError());
 }
 
@@ -427,7 +448,7 @@ template <int NPE, int NQP, typename R>
 void go_batch(feasm_ctx *ctx, const BatchLaunch &b) {
     auto kern = k_fused_batch<NPE, NQP, R>;
     FEASM_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.budget));
-    hipLaunchKernelGGL(kern, dim3(b.grid), dim3(kThreads), b.budget, ctx->stream,
+    hipLaunchKernelGGL(kern, dim3(b.grid), dim3(b.threads), b.budget, ctx->stream,
                        static_cast<const BatchEntry *>(b.entries->p), dp<int32_t>(b.tile_entry), b.ntiles);
     FEASM_HIP(hipGetLastError());
 }
@@ -453,6 +474,7 @@ void prepare_batch_launch(feasm_ctx *ctx, BatchLaunch &b, const std::vector<Batc
         for (int64_t t = 0; t < it.tiles->ntiles; t++) tile_entry.push_back((int32_t)i);
         b.ntiles += it.tiles->ntiles;
         b.budget = std::max(b.budget, it.tiles->shape.budget);
+        b.threads = it.tiles->shape.threads;
         host.push_back(E);
     }
     b.entries = dev_alloc(std::max<size_t>(host.size() * sizeof(BatchEntry), 16));
@@ -497,6 +519,7 @@ TileShape tile_shape_for(feasm_ctx *ctx, int npe, int nqp, int form, int precisi
     s.max_nodes = 64;
     s.max_halo = 255;
     s.max_cj = 4096;
+    s.threads = 512;
     return s;
 }
 

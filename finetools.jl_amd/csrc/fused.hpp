@@ -14,16 +14,23 @@ struct TileShape {
     int rbytes = 8;    // sizeof of the element arithmetic type
     bool grad = true;  // forms that need spatial gradients (all but DOT)
     int max_nodes = 64, max_halo = 255, max_cj = 1024;
+    int threads = 256;  // workgroup size of the fused kernel (256 or 512)
     size_t budget = 0;  // bytes of LDS available to one workgroup
 
     struct Layout {
-        size_t dN, node, ncj, bp, conn, U, S, cj, P, total;
+        size_t dN, X, lconn, node, ncj, bp, conn, U, S, cj, P, total;
     };
-    __host__ __device__ Layout layout(int64_t H, int64_t CJ, int P) const {
+    // H halo elements, CJ (column, element) pairs, P owned nodes, NH distinct
+    // nodes of the halo
+    __host__ __device__ Layout layout(int64_t H, int64_t CJ, int P, int64_t NH) const {
         Layout L;
         size_t o = 0;
         L.dN = o;
         o = align16(o + (size_t)nqp * npe * 3 * 8);
+        L.X = o;
+        o = align16(o + (size_t)NH * 3 * 8);
+        L.lconn = o;
+        o = align16(o + (size_t)H * npe * 2);
         L.node = o;
         o = align16(o + (size_t)(P + 1) * 4);
         L.ncj = o;
@@ -43,11 +50,13 @@ struct TileShape {
         L.total = o;
         return L;
     }
-    size_t lds_bytes(int64_t H, int64_t CJ, int P) const { return layout(H, CJ, P).total; }
+    size_t lds_bytes(int64_t H, int64_t CJ, int P, int64_t NH) const { return layout(H, CJ, P, NH).total; }
     int64_t key() const {
-        return ((((int64_t)(npe * 64 + nqp) * 16 + swp) * 16 + rbytes * 2 + (grad ? 1 : 0)) * 256 + max_nodes) *
-                   4096 +
-               (int64_t)(budget / 1024);
+        return (((((int64_t)(npe * 64 + nqp) * 16 + swp) * 16 + rbytes * 2 + (grad ? 1 : 0)) * 256 + max_nodes) *
+                    4096 +
+                (int64_t)(budget / 1024)) *
+                   4 +
+               (threads == 512 ? 1 : 0);
     }
 };
 
@@ -62,6 +71,9 @@ struct TileSet {
     DevPtr halo;      // int32 halo elements, ascending within a tile
     DevPtr cj_ptr;    // int32 [ntiles+1]
     DevPtr cj;        // uint32 (owned-local c << 16) | (halo-local e << 8) | local index of c in e
+    DevPtr hnode_ptr; // int32 [ntiles+1]
+    DevPtr hnodes;    // int32 distinct nodes of each tile's halo
+    DevPtr lconn;     // uint16 per halo element, its nodes as indices into the tile's hnodes
 };
 
 std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape);
@@ -77,7 +89,7 @@ struct BatchMesh {
 // a prepared persistent launch over the tiles of several meshes that share
 // the element type, rule size and arithmetic
 struct BatchLaunch {
-    int npe = 8, nqp = 8, rbytes = 8;
+    int npe = 8, nqp = 8, rbytes = 8, threads = 256;
     int64_t ntiles = 0;
     size_t budget = 0;
     unsigned grid = 0;

@@ -156,21 +156,23 @@ struct feasm_server {
                 if (!r->reassembly) in_use += r->est;
                 if (r->reassembly && residents.count(r->mesh_id)) residents[r->mesh_id].users++;
             }
-            run_one(ctx, r);
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                running--;
-                completed++;
-                if (r->reassembly) {
-                    auto it = residents.find(r->mesh_id);
-                    if (it != residents.end()) {
-                        it->second.users--;
-                        it->second.last_use = now_s();
-                    }
-                }
-            }
+            run_one(ctx, r);  // publishes the outcome and the bookkeeping under one lock
             cv_work.notify_all();
             cv_done.notify_all();
+        }
+    }
+
+    // bookkeeping of a finished request; called with mu held, in the same
+    // critical section that publishes its status
+    void finish_locked(Request *r) {
+        running--;
+        completed++;
+        if (r->reassembly) {
+            auto it = residents.find(r->mesh_id);
+            if (it != residents.end()) {
+                it->second.users--;
+                it->second.last_use = now_s();
+            }
         }
     }
 
@@ -234,6 +236,7 @@ struct feasm_server {
             r->conn0.shrink_to_fit();
             r->xyz.clear();
             r->xyz.shrink_to_fit();
+            finish_locked(r);
         } catch (const feasm::Error &e) {
             std::lock_guard<std::mutex> lk(mu);
             if (!r->reassembly) {
@@ -243,6 +246,7 @@ struct feasm_server {
             r->status = e.code;
             r->error = e.what();
             r->t_done = now_s();
+            finish_locked(r);
         } catch (const std::exception &e) {
             std::lock_guard<std::mutex> lk(mu);
             if (!r->reassembly) {
@@ -252,6 +256,7 @@ struct feasm_server {
             r->status = FEASM_EINVAL;
             r->error = e.what();
             r->t_done = now_s();
+            finish_locked(r);
         }
     }
 

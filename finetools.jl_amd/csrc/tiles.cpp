@@ -67,8 +67,11 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape) {
 
     auto T = std::make_shared<TileSet>();
     T->shape = shape;
-    std::vector<int32_t> node_ptr{0}, nodes, halo_ptr{0}, halo, cj_ptr{0}, node_cj;
+    std::vector<int32_t> node_ptr{0}, nodes, halo_ptr{0}, halo, cj_ptr{0}, node_cj, hnode_ptr{0}, hnodes;
     std::vector<uint32_t> cj;
+    std::vector<uint16_t> lconn;
+    std::vector<int32_t> nstamp(nn, -1), nlocal(nn, 0);  // halo-node membership of the current tile
+    int64_t cur_nh = 0;
     nodes.reserve(nn);
     node_cj.reserve(nn + 1);
     std::vector<int32_t> stamp(ne, -1), local(ne, 0);
@@ -91,6 +94,17 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape) {
             }
         }
         halo.insert(halo.end(), cur_halo.begin(), cur_halo.end());
+        // distinct halo nodes, ascending, and the halo connectivity in local indices
+        std::vector<int32_t> hn;
+        for (int32_t e : cur_halo)
+            for (int a = 0; a < npe; a++) hn.push_back(conn[(int64_t)e * npe + a]);
+        std::sort(hn.begin(), hn.end());
+        hn.erase(std::unique(hn.begin(), hn.end()), hn.end());
+        for (size_t i = 0; i < hn.size(); i++) nlocal[hn[i]] = (int32_t)i;
+        for (int32_t e : cur_halo)
+            for (int a = 0; a < npe; a++) lconn.push_back((uint16_t)nlocal[conn[(int64_t)e * npe + a]]);
+        hnodes.insert(hnodes.end(), hn.begin(), hn.end());
+        hnode_ptr.push_back((int32_t)hnodes.size());
         node_ptr.push_back((int32_t)nodes.size());
         halo_ptr.push_back((int32_t)halo.size());
         cj_ptr.push_back((int32_t)cj.size());
@@ -100,29 +114,57 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape) {
         cur_nodes.clear();
         cur_halo.clear();
         cur_cj = 0;
+        cur_nh = 0;
         tile++;
     };
+    std::vector<int32_t> seen;
+    seen.reserve(256);
     for (int64_t i = 0; i < nn; i++) {
         const int32_t n = key[i].second;
         const int64_t deg = n2e_ptr[n + 1] - n2e_ptr[n];
-        int64_t fresh = 0;
-        for (int64_t s = n2e_ptr[n]; s < n2e_ptr[n + 1]; s++)
-            if (stamp[n2e[s] >> 4] != tile) fresh++;
-        const int64_t H = (int64_t)cur_halo.size() + fresh, CJ = cur_cj + deg;
+        // what adding node n would add: new halo elements and new halo nodes
+        auto count_new = [&](int64_t &fresh, int64_t &fresh_nodes) {
+            fresh = 0;
+            fresh_nodes = 0;
+            seen.clear();
+            for (int64_t s = n2e_ptr[n]; s < n2e_ptr[n + 1]; s++) {
+                const int32_t e = (int32_t)(n2e[s] >> 4);
+                if (stamp[e] == tile) continue;
+                fresh++;
+                for (int a = 0; a < npe; a++) {
+                    const int32_t v = conn[(int64_t)e * npe + a];
+                    if (nstamp[v] != tile && std::find(seen.begin(), seen.end(), v) == seen.end()) {
+                        seen.push_back(v);
+                        fresh_nodes++;
+                    }
+                }
+            }
+        };
+        int64_t fresh, fresh_nodes;
+        count_new(fresh, fresh_nodes);
+        const int64_t H = (int64_t)cur_halo.size() + fresh, CJ = cur_cj + deg, NH = cur_nh + fresh_nodes;
         const bool fits = (int64_t)cur_nodes.size() + 1 <= shape.max_nodes && H <= shape.max_halo &&
-                          CJ <= shape.max_cj && shape.lds_bytes(H, CJ, (int)cur_nodes.size() + 1) <= shape.budget;
+                          CJ <= shape.max_cj && NH <= 65535 &&
+                          shape.lds_bytes(H, CJ, (int)cur_nodes.size() + 1, NH) <= shape.budget;
         if (!fits && !cur_nodes.empty()) {
             close_tile();
-            fresh = deg;
+            count_new(fresh, fresh_nodes);
         }
         FEASM_REQUIRE(deg <= shape.max_cj && deg <= shape.max_halo &&
-                          shape.lds_bytes(deg, deg, 1) <= shape.budget,
+                          shape.lds_bytes(deg, deg, 1, fresh_nodes) <= shape.budget,
                       FEASM_EUNSUPPORTED, "a node has too many incident elements for the fused kernel");
         for (int64_t s = n2e_ptr[n]; s < n2e_ptr[n + 1]; s++) {
             const int32_t e = (int32_t)(n2e[s] >> 4);
             if (stamp[e] != tile) {
                 stamp[e] = tile;
                 cur_halo.push_back(e);
+                for (int a = 0; a < npe; a++) {
+                    const int32_t v = conn[(int64_t)e * npe + a];
+                    if (nstamp[v] != tile) {
+                        nstamp[v] = tile;
+                        cur_nh++;
+                    }
+                }
             }
         }
         cur_nodes.push_back(n);
@@ -144,6 +186,9 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape) {
     T->halo = up(halo.data(), halo.size() * 4);
     T->cj_ptr = up(cj_ptr.data(), cj_ptr.size() * 4);
     T->cj = up(cj.data(), cj.size() * 4);
+    T->hnode_ptr = up(hnode_ptr.data(), hnode_ptr.size() * 4);
+    T->hnodes = up(hnodes.data(), hnodes.size() * 4);
+    T->lconn = up(lconn.data(), lconn.size() * 2);
     (void)ctx;
     return T;
 }

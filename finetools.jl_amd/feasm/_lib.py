@@ -44,6 +44,7 @@ SIGNATURES = {
     "feasm_pattern_build": (I32, [P]),
     "feasm_mesh_set_numeric_path": (I32, [P, I32]),
     "feasm_mesh_tile_info": (I32, [P, PI64, PI64]),
+    "feasm_mesh_set_tile_params": (I32, [P, I32, I32, I32]),
     "feasm_pattern_info": (I32, [P, PI64, PI64]),
     "feasm_matrix_dims": (I32, [P, PI64, PI64, PI64]),
     "feasm_matrix_copy_csc": (I32, [P, I32, P, P, P]),
@@ -61,13 +62,14 @@ SIGNATURES = {
     "feasm_sparse": (I32, [P, I64, P, P, P, I64, I64, PP]),
     "feasm_batch_create": (I32, [P, PP]),
     "feasm_batch_add": (I32, [P, P, I32, I32, I32, P, I32, I32]),
-    "feasm_batch_prepare": (I32, [P, I32, I32, I32]),
+    "feasm_batch_prepare": (I32, [P, I32, I32, I32, I32]),
     "feasm_batch_run": (I32, [P]),
     "feasm_batch_last_ms": (I32, [P, ctypes.POINTER(ctypes.c_double)]),
     "feasm_batch_matrix": (I32, [P, I64, PP]),
     "feasm_batch_stats": (I32, [P, PI64, PI64, PI64, PI64]),
     "feasm_batch_autotune": (I32, [P, ctypes.c_char_p, ctypes.c_char_p, I32, ctypes.POINTER(ctypes.c_int),
-                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]),
+                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_double)]),
     "feasm_batch_destroy": (I32, [P]),
     "feasm_server_create": (I32, [I32, I32, DBL, PP]),
     "feasm_server_submit_mesh": (I32, [P, I32, I64, I32, P, I64, P, I32, I32, I32, I32, P, I32, I32, I32, PI64, PI64]),

@@ -159,6 +159,9 @@ class DeviceMesh:
         """'auto', 'staged' or 'fused'."""
         check(lib().feasm_mesh_set_numeric_path(self.h, {"auto": 0, "staged": 1, "fused": 2}[path]), self.ctx.h)
 
+    def set_tile_params(self, max_nodes=0, budget_kib=0, threads=0):
+        check(lib().feasm_mesh_set_tile_params(self.h, max_nodes, budget_kib, threads), self.ctx.h)
+
     def tile_info(self):
         t, h = ctypes.c_int64(), ctypes.c_int64()
         check(lib().feasm_mesh_tile_info(self.h, ctypes.byref(t), ctypes.byref(h)), self.ctx.h)
@@ -203,8 +206,8 @@ class Batch:
         self.meshes.append(mesh)
         return len(self.meshes) - 1
 
-    def prepare(self, use_graph=True, max_nodes=0, budget_kib=0):
-        check(lib().feasm_batch_prepare(self.h, 1 if use_graph else 0, max_nodes, budget_kib), self.ctx.h)
+    def prepare(self, use_graph=True, max_nodes=0, budget_kib=0, threads=0):
+        check(lib().feasm_batch_prepare(self.h, 1 if use_graph else 0, max_nodes, budget_kib, threads), self.ctx.h)
 
     def run(self):
         check(lib().feasm_batch_run(self.h), self.ctx.h)
@@ -226,11 +229,11 @@ class Batch:
         return {"meshes": a.value, "elements": b.value, "nnz": c.value, "tiles": d.value}
 
     def autotune(self, db_path=None, key=None, reps=3):
-        n, kb, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        n, kb, th, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         check(lib().feasm_batch_autotune(self.h, db_path.encode() if db_path else None,
                                          key.encode() if key else None, reps, ctypes.byref(n), ctypes.byref(kb),
-                                         ctypes.byref(ms)), self.ctx.h)
-        return {"max_nodes": n.value, "budget_kib": kb.value, "ms": ms.value}
+                                         ctypes.byref(th), ctypes.byref(ms)), self.ctx.h)
+        return {"max_nodes": n.value, "budget_kib": kb.value, "threads": th.value, "ms": ms.value}
 
 
 class Server:

@@ -149,6 +149,9 @@ int feasm_bilform_coo(feasm_mesh *mesh, int form, int rule, int order, const dou
  * element/rule), 2 fused (error where unavailable).  Both give the same
  * pattern; values agree to rounding. */
 int feasm_mesh_set_numeric_path(feasm_mesh *mesh, int path);
+/* Tile parameters of the fused kernel for this mesh (0 = default):
+ * owned nodes per tile (<= 64), LDS per workgroup in KiB, threads (256/512). */
+int feasm_mesh_set_tile_params(feasm_mesh *mesh, int max_nodes, int budget_kib, int threads);
 /* Tiles built for the fused kernel so far (summed over forms/precisions). */
 int feasm_mesh_tile_info(feasm_mesh *mesh, int64_t *ntiles, int64_t *nhalo);
 /* Build (or rebuild) the device CSC pattern for the mesh's DOFs now. */
@@ -240,9 +243,9 @@ int feasm_batch_add(feasm_batch *b, feasm_mesh *mesh, int form, int rule, int or
                     int ncoeff, int precision);
 /* Build patterns, tiles and launch tables; use_graph != 0 captures the
  * launches in a hipGraph replayed by feasm_batch_run.  max_nodes (owned
- * nodes per tile, <= 64) and budget_kib (LDS per workgroup) tune the tiles;
- * 0 picks the defaults. */
-int feasm_batch_prepare(feasm_batch *b, int use_graph, int max_nodes, int budget_kib);
+ * nodes per tile, <= 64), budget_kib (LDS per workgroup) and threads (256 or
+ * 512 per workgroup) tune the tiles; 0 picks the defaults. */
+int feasm_batch_prepare(feasm_batch *b, int use_graph, int max_nodes, int budget_kib, int threads);
 /* Enqueue one batched assembly on the context stream (asynchronous). */
 int feasm_batch_run(feasm_batch *b);
 /* Device time of the last feasm_batch_run (waits for it). */
@@ -254,7 +257,7 @@ int feasm_batch_stats(feasm_batch *b, int64_t *nmeshes, int64_t *nelem, int64_t 
  * leaving the batch prepared with the winner.  With db_path and key the
  * result is cached in (and first looked up from) a text database. */
 int feasm_batch_autotune(feasm_batch *b, const char *db_path, const char *key, int reps, int *best_nodes,
-                         int *best_budget_kib, double *best_ms);
+                         int *best_budget_kib, int *best_threads, double *best_ms);
 int feasm_batch_destroy(feasm_batch *b);
 
 /* ---- assembly server (continuous batching; no reference counterpart) --- */
