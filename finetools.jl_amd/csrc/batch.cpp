@@ -80,7 +80,7 @@ TileShape shape_for(feasm_batch *b, const feasm_batch::Item &it) {
     TileShape s = tile_shape_for(b->ctx, it.mesh->npe, qt.nqp, it.fp.form, it.precision);
     if (b->max_nodes > 0) s.max_nodes = std::min(64, b->max_nodes);
     if (b->budget > 0) s.budget = std::min(s.budget, b->budget);
-    if (b->threads == 256 || b->threads == 512) s.threads = b->threads;
+    if (b->threads == 256 || b->threads == 512 || b->threads == 1024) s.threads = b->threads;
     return s;
 }
 
@@ -88,8 +88,9 @@ void prepare(feasm_batch *b) {
     feasm_ctx *ctx = b->ctx;
     b->drop_graph();
     b->launches.clear();
-    // group by (npe, nqp, precision)
-    std::map<std::tuple<int, int, int>, std::vector<BatchMesh>> groups;
+    // one launch per (element type, rule size, precision, kernel form, DOFs
+    // per node, rule): the workgroups load the rule's tables once
+    std::map<std::tuple<int, int, int, int, int, int, int>, std::vector<BatchMesh>> groups;
     for (auto &it : b->items) {
         feasm_mesh *m = it.mesh;
         if (!m->pattern || m->pattern->ndn != m->ndn) m->pattern = build_pattern(m, m->ndn);
@@ -108,13 +109,16 @@ void prepare(feasm_batch *b) {
             it.mat->nzval = dev_alloc(std::max<size_t>(it.mat->nnz * 8, 8));
         }
         BatchMesh bm{m, tt->second, it.fp, it.rule, it.order, dp<double>(it.mat->nzval)};
-        groups[std::make_tuple(m->npe, s.nqp, s.rbytes)].push_back(bm);
+        groups[std::make_tuple(m->npe, s.nqp, s.rbytes, fused_kernel_form(it.fp), it.fp.ndn, it.rule, it.order)]
+            .push_back(bm);
     }
     for (auto &kv : groups) {
         BatchLaunch L;
         L.npe = std::get<0>(kv.first);
         L.nqp = std::get<1>(kv.first);
         L.rbytes = std::get<2>(kv.first);
+        L.kform = std::get<3>(kv.first);
+        L.ndn = std::get<4>(kv.first);
         prepare_batch_launch(ctx, L, kv.second);
         b->launches.push_back(std::move(L));
     }
@@ -189,7 +193,8 @@ extern "C" int feasm_batch_add(feasm_batch *b, feasm_mesh *mesh, int form, int r
 extern "C" int feasm_batch_prepare(feasm_batch *b, int use_graph, int max_nodes, int budget_kib, int threads) {
     B_TRY
     FEASM_REQUIRE(b, FEASM_EINVAL, "null batch");
-    FEASM_REQUIRE(threads == 0 || threads == 256 || threads == 512, FEASM_EINVAL, "threads: 0, 256 or 512");
+    FEASM_REQUIRE(threads == 0 || threads == 256 || threads == 512 || threads == 1024, FEASM_EINVAL,
+                  "threads: 0, 256, 512 or 1024");
     b->max_nodes = max_nodes;
     b->threads = threads;
     b->budget = budget_kib > 0 ? (size_t)budget_kib * 1024 : 0;
@@ -282,11 +287,11 @@ extern "C" int feasm_batch_autotune(feasm_batch *b, const char *db_path, const c
             }
         }
     }
-    const int nodes_c[] = {16, 32, 64};
+    const int nodes_c[] = {32, 64};
     const int kib_c[] = {160, 80};
-    const int thr_c[] = {256, 512};
+    const int thr_c[] = {512, 1024};
     double best = 1e300;
-    int bn = 64, bk = 160, bt = 512;
+    int bn = 64, bk = 160, bt = 1024;
     for (int th : thr_c)
         for (int kb : kib_c)
             for (int n : nodes_c) {

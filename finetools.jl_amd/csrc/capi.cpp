@@ -138,16 +138,31 @@ void feasm::numeric_into_ctx(feasm_ctx *ctx, feasm_mesh *m, const FormParams &fp
     const bool fused = m->numeric_path != 1 && fused_supported(m->npe, qt.nqp);
     FEASM_REQUIRE(fused || m->numeric_path != 2, FEASM_EUNSUPPORTED,
                   "no fused kernel for this element type and rule");
+    std::shared_ptr<TileSet> tiles;
     if (fused) {
         TileShape shape = tile_shape_for(ctx, m->npe, qt.nqp, fp.form, precision);
         if (m->tile_nodes > 0) shape.max_nodes = std::min(64, m->tile_nodes);
         if (m->tile_kib > 0) shape.budget = std::min(shape.budget, (size_t)m->tile_kib * 1024);
-        if (m->tile_threads == 256 || m->tile_threads == 512) shape.threads = m->tile_threads;
+        if (m->tile_threads == 256 || m->tile_threads == 512 || m->tile_threads == 1024)
+            shape.threads = m->tile_threads;
         auto it = m->tiles.find(shape.key());
-        if (it == m->tiles.end()) it = m->tiles.emplace(shape.key(), build_tiles(m, shape)).first;
+        if (it == m->tiles.end()) {
+            try {
+                tiles = build_tiles(m, shape);
+            } catch (const Error &e) {
+                // meshes beyond the tile index widths take the staged path
+                // unless the fused one was asked for (a null entry remembers it)
+                if (e.code != FEASM_EUNSUPPORTED || m->numeric_path == 2) throw;
+            }
+            m->tiles.emplace(shape.key(), tiles);
+        } else {
+            tiles = it->second;
+        }
+    }
+    if (tiles) {
         Timer tm(ctx);
         tm.mark(0);
-        launch_fused(ctx, m, *m->pattern, *it->second, qt, fp, precision, nzval);
+        launch_fused(ctx, m, *m->pattern, *tiles, qt, fp, precision, nzval);
         tm.mark(1);
         tm.mark(2);
         tm.done();
@@ -422,8 +437,8 @@ extern "C" int feasm_mesh_set_tile_params(feasm_mesh *m, int max_nodes, int budg
     API_TRY
     FEASM_REQUIRE(m, FEASM_EINVAL, "null mesh");
     FEASM_REQUIRE(max_nodes >= 0 && max_nodes <= 64 && budget_kib >= 0 &&
-                      (threads == 0 || threads == 256 || threads == 512),
-                  FEASM_EINVAL, "max_nodes 0..64, budget_kib >= 0, threads 0/256/512");
+                      (threads == 0 || threads == 256 || threads == 512 || threads == 1024),
+                  FEASM_EINVAL, "max_nodes 0..64, budget_kib >= 0, threads 0/256/512/1024");
     m->tile_nodes = max_nodes;
     m->tile_kib = budget_kib;
     m->tile_threads = threads;
@@ -435,6 +450,7 @@ extern "C" int feasm_mesh_tile_info(feasm_mesh *m, int64_t *ntiles, int64_t *nha
     FEASM_REQUIRE(m, FEASM_EINVAL, "null mesh");
     int64_t t = 0, h = 0;
     for (auto &kv : m->tiles) {
+        if (!kv.second) continue;
         t += kv.second->ntiles;
         h += kv.second->nhalo;
     }

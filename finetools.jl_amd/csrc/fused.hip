@@ -1,26 +1,26 @@
 // Fused numeric assembly: element integration and the global summation in
 // one kernel, nothing staged in HBM.
 //
-// One workgroup per tile (a compact set of owned column nodes plus the halo
-// of elements incident to them; see tiles.cpp):
-//   0  tile metadata, halo connectivity and the parametric gradient table
-//      go to LDS;
+// Each workgroup walks a contiguous range of tiles (a compact set of owned
+// column nodes plus the halo of elements incident to them, with the tile's
+// node blocks listed as tasks; see tiles.cpp).  Per tile:
+//   0  the tile's LDS image (column pointers, task and item lists, halo
+//      connectivity; one contiguous blob laid out by TileShape::layout) and
+//      the coordinates of its halo nodes go to LDS.  These global loads are
+//      issued while the previous tile is in phase 2 and held in registers;
 //   1  one thread per (halo element, quadrature point): J = X' dN, det J by
 //      cofactors of row 1 and the adjugate inverse (src/FESetModule.jl:479-544)
 //      in FP64; the spatial gradients g_a = inv(J)' dN_a are stored scaled by
 //      sqrt(|J w|) (u_a = sqrt(|Jw|) g_a, with the sign of J w kept apart), so
 //      that an element's node-pair matrix is M_ab = sum_q s_q u_a u_b^T;
-//   2  one thread per (owned column c, incident element e, half of the
-//      element's nodes) forms the partial blocks M^e_{a c} (9 values each, or
-//      one for the scalar forms) in registers and leaves them in LDS;
-//   3  one thread per CSC node block (m, c) of the tile's columns adds the
-//      partials of the elements holding both nodes in ascending element
-//      number -- the order in which SparseArrays.sparse folds the duplicates
-//      of the reference's element-major COO (src/AssemblyModule.jl:319-325) --
-//      applies the material (K = D : M for elasticity,
-//      src/DeforModelRedModule.jl:447-472 with src/MatrixUtilityModule.jl:
-//      189-216) and writes the block once.
-// The products u_a[k] u_c[l] are formed symmetrically (a sign flip is exact,
+//   2  one thread per CSC node block (m, c) of the tile's columns forms the
+//      element matrices M^e_{m c} of the elements holding both nodes and adds
+//      them in ascending element number -- the order in which
+//      SparseArrays.sparse folds the duplicates of the reference's
+//      element-major COO (src/AssemblyModule.jl:319-325) -- applies the
+//      material (K = D : M for elasticity, src/DeforModelRedModule.jl:447-472
+//      with src/MatrixUtilityModule.jl:189-216) and writes the block once.
+// The products u_m[k] u_c[l] are formed symmetrically (a sign flip is exact,
 // the product commutes) and every block is contracted in the orientation of
 // its lower node number, so the assembled matrix is exactly symmetric, as the
 // reference's is (test/test_forms.jl:442).
@@ -30,108 +30,231 @@ namespace feasm {
 
 namespace {
 
-constexpr int kMaxThreads = 512;
+constexpr int kMaxThreads = 1024;
 
 __device__ constexpr int voigt(int i, int k) {
     return (i == k) ? i : ((i + k == 1) ? 3 : ((i + k == 2) ? 4 : 5));
 }
 
+// Linear elasticity with a material matrix of cubic symmetry (isotropic
+// materials among them): D00 = D11 = D22, D01 = D02 = D12, D33 = D44 = D55,
+// every other entry zero.  Then K = D01 M + D33 M^T off the diagonal and
+// K_ii = D00 M_ii + D33 (tr M - M_ii), three coefficients instead of 36.
+constexpr int kFormElasticCubic = 16;
+
+bool cubic_symmetry(const double *D) {
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            const double v = D[6 * i + j];
+            if (i < 3 && j < 3) {
+                if (v != (i == j ? D[0] : D[1])) return false;
+            } else if (i == j) {
+                if (v != D[21]) return false;
+            } else if (v != 0.0) {
+                return false;
+            }
+        }
+    return true;
+}
+
 struct FusedParams {
     FormParams fp;
+    int cubic;         // elasticity with a cubic-symmetry D (see kFormElasticCubic)
     const double *N;   // [nqp][npe]
     const double *dN;  // [nqp][npe][3]
     const double *w;   // [nqp]
 };
 
-// Everything one workgroup needs to process one tile of one mesh.
+// Everything one workgroup needs to process the tiles of one mesh.
 struct TileJob {
-    const int32_t *node_ptr, *nodes, *node_cj, *halo_ptr, *halo, *cj_ptr, *hnode_ptr, *hnodes;
-    const uint32_t *cj;
-    const uint16_t *lconn;
-    const int32_t *conn;
+    const TileHdr *hdr;
+    const unsigned char *blob;
+    const int32_t *hnodes;
     const double *xyz;
-    const int64_t *adj_ptr;
-    const int32_t *adj;
     double *nzval;
-    int64_t tile;
 };
 
+// signs of J w at the points of one element (nonzero byte = negative)
+template <int NQP>
+__device__ __forceinline__ uint64_t neg_mask(const uint8_t *s) {
+    if constexpr (NQP == 8) return *reinterpret_cast<const uint64_t *>(s);
+    else if constexpr (NQP == 4) return *reinterpret_cast<const uint32_t *>(s);
+    else {
+        uint64_t m = 0;
+        for (int q = 0; q < NQP; q++) m |= (uint64_t)s[q] << (8 * q);
+        return m;
+    }
+}
+
+// Adds M^e for the node pair (row ar, column ac) of halo element h to M: the
+// sum over the points of s_q u_ar u_ac^T (grad forms; 9 values, or its trace
+// for SWP 1) or of N_ar N_ac J w (DOT).
+template <int FORM>
+constexpr int form_swp() {
+    return (FORM == FEASM_FORM_LIN_ELASTIC || FORM == kFormElasticCubic || FORM == FEASM_FORM_DIFFUSION_MAT) ? 9 : 1;
+}
+
+template <int NPE, int NQP, int FORM, typename R, bool SIGNED, typename A>
+__device__ __forceinline__ void elem_block(uint32_t it, const R *__restrict__ sU, const uint8_t *__restrict__ sSg,
+                                           const double *__restrict__ sN, const R *__restrict__ sW, A *M) {
+    constexpr int US = NQP * NPE * 3 + 1;  // TileShape::ustride()
+    constexpr int SWP = form_swp<FORM>();
+    const int h = (int)(it >> 8), ac = (int)((it >> 4) & 15u), ar = (int)(it & 15u);
+    if constexpr (FORM != FEASM_FORM_DOT) {
+        const R *U = sU + h * US;
+        uint64_t neg = 0;
+        if (SIGNED) neg = neg_mask<NQP>(sSg + h * NQP);
+#pragma unroll
+        for (int q = 0; q < NQP; q++) {
+            const R *uc = U + (q * NPE + ac) * 3, *ur = U + (q * NPE + ar) * 3;
+            const R c0 = uc[0], c1 = uc[1], c2 = uc[2];
+            R r0 = ur[0], r1 = ur[1], r2 = ur[2];
+            if (SIGNED && ((neg >> (8 * q)) & 255u)) {
+                r0 = -r0;
+                r1 = -r1;
+                r2 = -r2;
+            }
+            if (SWP == 9) {
+                M[0] = fma(r0, c0, M[0]); M[1] = fma(r0, c1, M[1]); M[2] = fma(r0, c2, M[2]);
+                M[3] = fma(r1, c0, M[3]); M[4] = fma(r1, c1, M[4]); M[5] = fma(r1, c2, M[5]);
+                M[6] = fma(r2, c0, M[6]); M[7] = fma(r2, c1, M[7]); M[8] = fma(r2, c2, M[8]);
+            } else {
+                M[0] = fma(r0, c0, M[0]);
+                M[0] = fma(r1, c1, M[0]);
+                M[0] = fma(r2, c2, M[0]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NQP; q++) {
+            const R nn = (R)(sN[q * NPE + ar] * sN[q * NPE + ac]);
+            M[0] = fma(nn, sW[h * NQP + q], M[0]);
+        }
+    }
+}
+
+// The element matrices of one block's shared elements, added in ascending
+// element order (items j = 0..k-1); ILP elements (1 or 2) in flight at a
+// time.  In FP64 the products go straight into the FP64 sums (element after
+// element, point after point); in mixed precision each element's matrix is
+// formed in FP32 and added in FP64.
+template <int NPE, int NQP, int FORM, typename R, bool SIGNED, int SWP, int ILP>
+__device__ __forceinline__ void block_sum(int t, int k, const int32_t *__restrict__ sSlot,
+                                          const uint16_t *__restrict__ sItem, const R *__restrict__ sU,
+                                          const uint8_t *__restrict__ sSg, const double *__restrict__ sN,
+                                          const R *__restrict__ sW, double *acc) {
+    if constexpr (sizeof(R) == 8) {
+        for (int j = 0; j < k; j++) elem_block<NPE, NQP, FORM, R, SIGNED>(sItem[sSlot[j] + t], sU, sSg, sN, sW, acc);
+    } else {
+        int j = 0;
+        for (; ILP == 2 && j + 1 < k; j += 2) {
+            const uint32_t i0 = sItem[sSlot[j] + t], i1 = sItem[sSlot[j + 1] + t];
+            R M0[SWP], M1[SWP];
+#pragma unroll
+            for (int i = 0; i < SWP; i++) M0[i] = M1[i] = (R)0;
+            elem_block<NPE, NQP, FORM, R, SIGNED>(i0, sU, sSg, sN, sW, M0);
+            elem_block<NPE, NQP, FORM, R, SIGNED>(i1, sU, sSg, sN, sW, M1);
+#pragma unroll
+            for (int i = 0; i < SWP; i++) acc[i] = (acc[i] + (double)M0[i]) + (double)M1[i];
+        }
+        for (; j < k; j++) {
+            R M0[SWP];
+#pragma unroll
+            for (int i = 0; i < SWP; i++) M0[i] = (R)0;
+            elem_block<NPE, NQP, FORM, R, SIGNED>(sItem[sSlot[j] + t], sU, sSg, sN, sW, M0);
+#pragma unroll
+            for (int i = 0; i < SWP; i++) acc[i] = acc[i] + (double)M0[i];
+        }
+    }
+}
+
+// Global loads of one tile's LDS image, issued ahead and held in registers.
+struct Staged {
+    uint4 c0;    // blob chunk tid (the rest of a larger blob is copied at store time)
+    int32_t hn;  // distinct halo node tid
+    double d;    // material coefficient tid
+};
+
+__device__ __forceinline__ void stage_issue(const TileJob &job, const TileHdr &hd, const double *coef, Staged &st) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nch = hd.nbytes >> 4;  // at least one chunk: every tile owns a column
+    const uint4 *src = reinterpret_cast<const uint4 *>(job.blob + hd.blob);
+    (void)nt;
+    st.c0 = src[min(tid, nch - 1)];
+    st.hn = (hd.NH > 0) ? job.hnodes[hd.x0 + min(tid, hd.NH - 1)] : 0;
+    st.d = coef[min(tid, 35)];
+}
+
+// The staged image into LDS: blob, halo coordinates (a second, dependent
+// load), coefficients; clears the tile's sign flag.
+__device__ __forceinline__ void stage_store(const TileJob &job, const TileHdr &hd, const Staged &st,
+                                            const TileShape::Layout &L, unsigned char *smem, double *sD,
+                                            int *sNeg) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nch = hd.nbytes >> 4, NH = hd.NH;
+    double x0 = 0, x1 = 0, x2 = 0;
+    if (tid < NH) {
+        const double *p = job.xyz + 3 * (int64_t)st.hn;
+        x0 = p[0];
+        x1 = p[1];
+        x2 = p[2];
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(smem + L.cbase);
+    if (tid < nch) dst[tid] = st.c0;
+    const uint4 *src = reinterpret_cast<const uint4 *>(job.blob + hd.blob);
+    for (int i = tid + nt; i < nch; i += nt) dst[i] = src[i];
+    if (tid < 36) sD[tid] = st.d;
+    if (tid == 0) *sNeg = 0;
+    double *sX = reinterpret_cast<double *>(smem + L.X);
+    if (tid < NH) {
+        sX[3 * tid] = x0;
+        sX[3 * tid + 1] = x1;
+        sX[3 * tid + 2] = x2;
+    }
+    for (int i = tid + nt; i < NH; i += nt) {
+        const double *p = job.xyz + 3 * (int64_t)job.hnodes[hd.x0 + i];
+        sX[3 * i] = p[0];
+        sX[3 * i + 1] = p[1];
+        sX[3 * i + 2] = p[2];
+    }
+}
+
+// One tile, phases 1 and 2, with its image already in LDS.  Between the
+// phases the next tile's loads are issued into `st` (when `more`).
 template <int NPE, int NQP, int FORM, int NDN, typename R>
-__device__ __forceinline__ void fused_tile(const TileShape &shape, const TileJob &job, const FusedParams &kp,
-                                           unsigned char *smem) {
+__device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJob &job, const FusedParams &kp,
+                                             const TileHdr &hd, unsigned char *smem, bool more,
+                                             const TileJob &njob, const TileHdr &nhd, const double *ncoef,
+                                             Staged &st) {
     constexpr bool GRAD = (FORM != FEASM_FORM_DOT);
-    constexpr int SWP = (FORM == FEASM_FORM_LIN_ELASTIC || FORM == FEASM_FORM_DIFFUSION_MAT) ? 9 : 1;
-    constexpr int NSPLIT = 4;
-    constexpr int APART = (NPE + NSPLIT - 1) / NSPLIT;
-    const int32_t *__restrict__ node_ptr = job.node_ptr;
-    const int32_t *__restrict__ hnodes = job.hnodes;
-    const uint16_t *__restrict__ lconn = job.lconn;
-    const int32_t *__restrict__ nodes = job.nodes;
-    const int32_t *__restrict__ node_cj = job.node_cj;
-    const int32_t *__restrict__ halo_ptr = job.halo_ptr;
-    const int32_t *__restrict__ halo = job.halo;
-    const int32_t *__restrict__ cj_ptr = job.cj_ptr;
-    const uint32_t *__restrict__ cjg = job.cj;
-    const int32_t *__restrict__ conn = job.conn;
-    const double *__restrict__ xyz = job.xyz;
-    const int64_t *__restrict__ adj_ptr = job.adj_ptr;
-    const int32_t *__restrict__ adj = job.adj;
-    double *__restrict__ nzval = job.nzval;
-    const int64_t tile = job.tile;
+    constexpr int SWP = form_swp<FORM>();
     const int tid = threadIdx.x;
     const int kThreads = blockDim.x;
-
-    const int n0 = node_ptr[tile], P = node_ptr[tile + 1] - n0;
-    const int h0 = halo_ptr[tile], H = halo_ptr[tile + 1] - h0;
-    const int c0 = cj_ptr[tile], CJ = cj_ptr[tile + 1] - c0;
-    const int x0 = job.hnode_ptr[tile], NH = job.hnode_ptr[tile + 1] - x0;
-    const TileShape::Layout L = shape.layout(H, CJ, P, NH);
-    double *sDN = reinterpret_cast<double *>(smem + L.dN);
+    const int P = hd.P, H = hd.H, NH = hd.NH, NT = hd.NT, K = hd.K, NI = hd.NI;
+    constexpr int US = NQP * NPE * 3 + 1;  // TileShape::ustride()
+    // two elements in flight where the registers allow it under 1024 threads
+    constexpr int ILP = (SWP == 9) ? 1 : 2;
+    constexpr int DS = NPE * 3 + 1;  // TileShape::dstride()
+    const TileShape::Layout L = shape.layout(H, P, NH, K, NT, NI);
+    double *sN = reinterpret_cast<double *>(smem + L.tab);
+    double *sDN = sN + NQP * NPE;
+    double *sD = sDN + NQP * DS;  // material coefficients (read as broadcasts)
+    int *sNeg = reinterpret_cast<int *>(sD + 36);  // set when J w < 0 somewhere in the tile
+    uint32_t *sTask = reinterpret_cast<uint32_t *>(smem + L.task);
+    uint16_t *sItem = reinterpret_cast<uint16_t *>(smem + L.item);
     double *sX = reinterpret_cast<double *>(smem + L.X);
     uint16_t *sLc = reinterpret_cast<uint16_t *>(smem + L.lconn);
-    int32_t *sNode = reinterpret_cast<int32_t *>(smem + L.node);
-    int32_t *sNcj = reinterpret_cast<int32_t *>(smem + L.ncj);
-    int32_t *sBP = reinterpret_cast<int32_t *>(smem + L.bp);
-    int32_t *sConn = reinterpret_cast<int32_t *>(smem + L.conn);
+    int64_t *sCb = reinterpret_cast<int64_t *>(smem + L.cbase);
+    int32_t *sCd = reinterpret_cast<int32_t *>(smem + L.cdeg);
+    int32_t *sSlot = reinterpret_cast<int32_t *>(smem + L.slot);
     R *sU = reinterpret_cast<R *>(smem + L.U);
-    R *sS = reinterpret_cast<R *>(smem + L.S);
-    uint32_t *sCJ = reinterpret_cast<uint32_t *>(smem + L.cj);
-    R *sP = reinterpret_cast<R *>(smem + L.P);
-
-    // ---- phase 0: metadata
-    for (int i = tid; i < NQP * NPE * 3; i += kThreads) sDN[i] = kp.dN[i];
-    for (int i = tid; i < P; i += kThreads) {
-        sNode[i] = nodes[n0 + i];
-        sNcj[i] = node_cj[n0 + i];
-    }
-    if (tid == 0) sNcj[P] = CJ;
-    for (int i = tid; i < H * NPE; i += kThreads) {
-        sConn[i] = conn[(int64_t)halo[h0 + i / NPE] * NPE + (i % NPE)];
-        sLc[i] = lconn[(int64_t)h0 * NPE + i];
-    }
-    for (int i = tid; i < NH * 3; i += kThreads) sX[i] = xyz[(int64_t)hnodes[x0 + i / 3] * 3 + (i % 3)];
-    for (int i = tid; i < CJ; i += kThreads) sCJ[i] = cjg[c0 + i];
-    if (tid < 64) {  // block prefix over the owned columns (P <= 64): one wavefront scan
-        int d = 0;
-        if (tid < P) {
-            const int32_t nd = nodes[n0 + tid];
-            d = (int)(adj_ptr[nd + 1] - adj_ptr[nd]);
-        }
-        int s = d;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int v = __shfl_up(s, o, 64);
-            if (tid >= o) s += v;
-        }
-        if (tid < P) sBP[tid + 1] = s;
-        if (tid == 0) sBP[0] = 0;
-    }
-    __syncthreads();
+    uint8_t *sSg = reinterpret_cast<uint8_t *>(smem + L.sgn);
+    R *sW = reinterpret_cast<R *>(smem + L.sgn);
 
     // ---- phase 1: Jacobians, inverses and scaled gradients
     for (int task = tid; task < H * NQP; task += kThreads) {
         const int h = task / NQP, q = task % NQP;
         const uint16_t *cn = sLc + h * NPE;
-        const double *dq = sDN + q * NPE * 3;
         double J[9];
 #pragma unroll
         for (int i = 0; i < 9; i++) J[i] = 0.0;
@@ -139,7 +262,7 @@ __device__ __forceinline__ void fused_tile(const TileShape &shape, const TileJob
         for (int a = 0; a < NPE; a++) {
             const double *x = sX + (int)cn[a] * 3;
             const double x0 = x[0], x1 = x[1], x2 = x[2];
-            const double d0 = dq[3 * a], d1 = dq[3 * a + 1], d2 = dq[3 * a + 2];
+            const double d0 = sDN[q * DS + a * 3], d1 = sDN[q * DS + a * 3 + 1], d2 = sDN[q * DS + a * 3 + 2];
             J[0] += x0 * d0; J[1] += x0 * d1; J[2] += x0 * d2;
             J[3] += x1 * d0; J[4] += x1 * d1; J[5] += x1 * d2;
             J[6] += x2 * d0; J[7] += x2 * d1; J[8] += x2 * d2;
@@ -161,184 +284,170 @@ __device__ __forceinline__ void fused_tile(const TileShape &shape, const TileJob
                                   -(RJ(1, 1) * RJ(3, 2) - RJ(3, 1) * RJ(1, 2)) * invdet,
                                   (RJ(1, 1) * RJ(2, 2) - RJ(2, 1) * RJ(1, 2)) * invdet};
             const R r = (R)sqrt(fabs(wj));
-            sS[h * NQP + q] = (wj < 0.0) ? (R)-1 : (R)1;
+            sSg[h * NQP + q] = (wj < 0.0) ? 1 : 0;
+            if (wj < 0.0) atomicOr(sNeg, 1);
             R ivr[9];
 #pragma unroll
             for (int i = 0; i < 9; i++) ivr[i] = (R)iv[i];
-            R *u = sU + (size_t)(h * NQP + q) * NPE * 3;
+            R *u = sU + (size_t)h * US + q * NPE * 3;
+            // node order rotated by the point, so that the lanes of one
+            // element store into different LDS banks
 #pragma unroll
-            for (int a = 0; a < NPE; a++) {
-                const R d0 = (R)dq[3 * a], d1 = (R)dq[3 * a + 1], d2 = (R)dq[3 * a + 2];
+            for (int i = 0; i < NPE; i++) {
+                const int a = (i + q) % NPE;
+                const double *d = sDN + q * DS + a * 3;
+                const R d0 = (R)d[0], d1 = (R)d[1], d2 = (R)d[2];
                 u[3 * a + 0] = r * (d0 * ivr[0] + d1 * ivr[3] + d2 * ivr[6]);
                 u[3 * a + 1] = r * (d0 * ivr[1] + d1 * ivr[4] + d2 * ivr[7]);
                 u[3 * a + 2] = r * (d0 * ivr[2] + d1 * ivr[5] + d2 * ivr[8]);
             }
         } else {
-            sS[h * NQP + q] = (R)wj;
+            sW[h * NQP + q] = (R)wj;
         }
 #undef RJ
     }
+    // one flag for the tile: J w < 0 somewhere (inverted elements) selects
+    // the path that applies the signs
     __syncthreads();
+    const bool tile_neg = *sNeg != 0;
+    if (more) stage_issue(njob, nhd, ncoef, st);
 
-    // ---- phase 2: partial blocks of every (owned column, incident element)
-    for (int task = tid; task < CJ * NSPLIT; task += kThreads) {
-        const int cj = task / NSPLIT, part = task % NSPLIT;
-        const uint32_t e = sCJ[cj];
-        const int h = (int)((e >> 8) & 255u), ac = (int)(e & 255u);
-        const int a0 = part * APART;
-        R *out = sP + (size_t)cj * NPE * SWP;
-        if (GRAD) {
-            R uc[NQP][3], sq[NQP];
-#pragma unroll
-            for (int q = 0; q < NQP; q++) {
-                const R *u = sU + ((size_t)(h * NQP + q) * NPE + ac) * 3;
-                uc[q][0] = u[0];
-                uc[q][1] = u[1];
-                uc[q][2] = u[2];
-                sq[q] = sS[h * NQP + q];
-            }
-#pragma unroll
-            for (int ai = 0; ai < APART; ai++) {
-                const int a = a0 + ai;
-                if (a >= NPE) break;
-                R M[SWP];
-#pragma unroll
-                for (int i = 0; i < SWP; i++) M[i] = (R)0;
-#pragma unroll
-                for (int q = 0; q < NQP; q++) {
-                    const R *u = sU + ((size_t)(h * NQP + q) * NPE + a) * 3;
-                    const R sa0 = sq[q] * u[0], sa1 = sq[q] * u[1], sa2 = sq[q] * u[2];
-                    if (SWP == 9) {
-                        M[0] = fma(sa0, uc[q][0], M[0]); M[1] = fma(sa0, uc[q][1], M[1]); M[2] = fma(sa0, uc[q][2], M[2]);
-                        M[3] = fma(sa1, uc[q][0], M[3]); M[4] = fma(sa1, uc[q][1], M[4]); M[5] = fma(sa1, uc[q][2], M[5]);
-                        M[6] = fma(sa2, uc[q][0], M[6]); M[7] = fma(sa2, uc[q][1], M[7]); M[8] = fma(sa2, uc[q][2], M[8]);
-                    } else {
-                        M[0] = fma(sa0, uc[q][0], M[0]);
-                        M[0] = fma(sa1, uc[q][1], M[0]);
-                        M[0] = fma(sa2, uc[q][2], M[0]);
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < SWP; i++) out[a * SWP + i] = M[i];
-            }
-        } else {
-#pragma unroll
-            for (int ai = 0; ai < APART; ai++) {
-                const int a = a0 + ai;
-                if (a >= NPE) break;
-                R S = (R)0;
-#pragma unroll
-                for (int q = 0; q < NQP; q++) {
-                    const R nn = (R)(kp.N[q * NPE + a] * kp.N[q * NPE + ac]);
-                    S = fma(nn, sS[h * NQP + q], S);
-                }
-                out[a] = S;
-            }
-        }
-    }
-    __syncthreads();
 
-    // ---- phase 3: ordered sums, material, one write per block
-    const int NB = sBP[P];
-    for (int task = tid; task < NB; task += kThreads) {
-        int lo = 0, hi = P;  // column: last c with sBP[c] <= task
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (sBP[mid] <= task) lo = mid; else hi = mid;
-        }
-        const int c = lo;
-        const int r = task - sBP[c];
-        const int deg = sBP[c + 1] - sBP[c];
-        const int32_t node = sNode[c];
-        const int64_t b0 = adj_ptr[node];
-        const int32_t m = adj[b0 + r];
+    // ---- phase 2: one thread per node block: element matrices of the shared
+    // elements, ordered sum, material, one write
+    double *__restrict__ nzval = job.nzval;
+    for (int t = tid; t < NT; t += kThreads) {
+        const uint32_t th = sTask[t];
+        const int c = (int)(th & 255u), r = (int)((th >> 8) & 255u), k = (int)((th >> 16) & 255u);
+        const bool sw = (th >> 24) & 1u, diag = (th >> 25) & 1u;
         double acc[SWP];
 #pragma unroll
         for (int i = 0; i < SWP; i++) acc[i] = 0.0;
-        bool first = true;
-        for (int cj = sNcj[c]; cj < sNcj[c + 1]; cj++) {
-            const int h = (int)((sCJ[cj] >> 8) & 255u);
-            const int32_t *cn = sConn + h * NPE;
-            int am = -1;
-#pragma unroll
-            for (int a = 0; a < NPE; a++)
-                if (cn[a] == m) am = a;
-            if (am < 0) continue;
-            const R *p = sP + ((size_t)cj * NPE + am) * SWP;
-            if (first) {
-#pragma unroll
-                for (int i = 0; i < SWP; i++) acc[i] = (double)p[i];
-                first = false;
-            } else {
-#pragma unroll
-                for (int i = 0; i < SWP; i++) acc[i] = acc[i] + (double)p[i];
-            }
-        }
+        if (tile_neg)
+            block_sum<NPE, NQP, FORM, R, true, SWP, ILP>(t, k, sSlot, sItem, sU, sSg, sN, sW, acc);
+        else
+            block_sum<NPE, NQP, FORM, R, false, SWP, ILP>(t, k, sSlot, sItem, sU, sSg, sN, sW, acc);
         double out[NDN * NDN];
-        if (FORM == FEASM_FORM_LIN_ELASTIC || FORM == FEASM_FORM_DIFFUSION_MAT) {
+        if constexpr (SWP == 9) {
             // contract in the orientation of the lower node number
-            const bool sw = m > node;
             double Mc[9];
 #pragma unroll
-            for (int k = 0; k < 3; k++)
+            for (int a = 0; a < 3; a++)
 #pragma unroll
-                for (int l = 0; l < 3; l++) Mc[3 * k + l] = sw ? acc[3 * l + k] : acc[3 * k + l];
-            if (FORM == FEASM_FORM_LIN_ELASTIC) {
+                for (int b = 0; b < 3; b++) Mc[3 * a + b] = sw ? acc[3 * b + a] : acc[3 * a + b];
+            if constexpr (FORM == FEASM_FORM_DIFFUSION_MAT) {
+                // the address depends on the task, which keeps the compiler
+                // from holding every coefficient in registers across tasks
+                const double *D = sD + (th >> 31);
+                double s = 0.0;
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++) s += D[3 * a + b] * Mc[3 * a + b];
+                out[0] = s;
+            } else {
                 double Kc[9];
+                if constexpr (FORM == kFormElasticCubic) {
+                    const double d00 = kp.fp.coef[0], d01 = kp.fp.coef[1], d33 = kp.fp.coef[21];
+                    const double tr = (Mc[0] + Mc[4]) + Mc[8];
 #pragma unroll
-                for (int i = 0; i < 3; i++)
+                    for (int i = 0; i < 3; i++)
 #pragma unroll
-                    for (int j = 0; j < 3; j++) {
-                        double s = 0.0;
+                        for (int jj = 0; jj < 3; jj++)
+                            Kc[3 * i + jj] = (i == jj) ? d00 * Mc[4 * i] + d33 * (tr - Mc[4 * i])
+                                                       : d01 * Mc[3 * i + jj] + d33 * Mc[3 * jj + i];
+                } else {
+                    const double *D = sD + (th >> 31);  // see above
 #pragma unroll
-                        for (int k = 0; k < 3; k++)
+                    for (int i = 0; i < 3; i++)
 #pragma unroll
-                            for (int l = 0; l < 3; l++) s += kp.fp.coef[6 * voigt(i, k) + voigt(j, l)] * Mc[3 * k + l];
-                        Kc[3 * i + j] = s;
-                    }
-                if (m == node) {  // diagonal block: upper triangle mirrored, as complete_lt! does
+                        for (int jj = 0; jj < 3; jj++) {
+                            double s = 0.0;
+#pragma unroll
+                            for (int a = 0; a < 3; a++)
+#pragma unroll
+                                for (int b = 0; b < 3; b++) s += D[6 * voigt(i, a) + voigt(jj, b)] * Mc[3 * a + b];
+                            Kc[3 * i + jj] = s;
+                        }
+                }
+                if (diag) {  // diagonal block: upper triangle mirrored, as complete_lt! does
                     Kc[3] = Kc[1];
                     Kc[6] = Kc[2];
                     Kc[7] = Kc[5];
                 }
 #pragma unroll
-                for (int k = 0; k < 3; k++)
+                for (int a = 0; a < 3; a++)
 #pragma unroll
-                    for (int l = 0; l < 3; l++) out[(NDN * k + l) % (NDN * NDN)] = sw ? Kc[3 * l + k] : Kc[3 * k + l];
-            } else {
-                double s = 0.0;
-#pragma unroll
-                for (int k = 0; k < 3; k++)
-#pragma unroll
-                    for (int l = 0; l < 3; l++) s += kp.fp.coef[3 * k + l] * Mc[3 * k + l];
-                out[0] = s;
+                    for (int b = 0; b < 3; b++) out[(NDN * a + b) % (NDN * NDN)] = sw ? Kc[3 * b + a] : Kc[3 * a + b];
             }
-        } else if (FORM == FEASM_FORM_DOT) {
+        } else if constexpr (FORM == FEASM_FORM_DOT) {
+            const double *D = sD + (th >> 31);  // see above
 #pragma unroll
-            for (int k = 0; k < NDN; k++)
+            for (int a = 0; a < NDN; a++)
 #pragma unroll
-                for (int l = 0; l < NDN; l++) out[NDN * k + l] = acc[0] * kp.fp.coef[NDN * k + l];
+                for (int b = 0; b < NDN; b++) out[NDN * a + b] = acc[0] * D[NDN * a + b];
         } else {
             out[0] = acc[0];
         }
-        const int64_t base = (int64_t)NDN * NDN * b0 + (int64_t)r * NDN;
+        const int deg = sCd[c];
+        const int64_t base = (int64_t)NDN * NDN * sCb[c] + (int64_t)r * NDN;
 #pragma unroll
         for (int l = 0; l < NDN; l++)
 #pragma unroll
-            for (int k = 0; k < NDN; k++) nzval[base + (int64_t)l * NDN * deg + k] = out[NDN * k + l];
+            for (int a = 0; a < NDN; a++) nzval[base + (int64_t)l * NDN * deg + a] = out[NDN * a + l];
     }
     __syncthreads();  // LDS is reused by the workgroup's next tile
+}
+
+// The tables of the element family, once per workgroup.
+template <int NPE, int NQP>
+__device__ __forceinline__ void load_tables(const FusedParams &kp, unsigned char *smem) {
+    constexpr int DS = NPE * 3 + 1;
+    double *sN = reinterpret_cast<double *>(smem);  // the layout's tab region starts at 0
+    double *sDN = sN + NQP * NPE;
+    for (int i = threadIdx.x; i < NQP * NPE; i += blockDim.x) sN[i] = kp.N[i];
+    for (int i = threadIdx.x; i < NQP * NPE * 3; i += blockDim.x)
+        sDN[(i / (NPE * 3)) * DS + i % (NPE * 3)] = kp.dN[i];
+}
+
+// XCD-aware persistent walk: consecutive workgroups land on different XCDs,
+// so each XCD takes a contiguous range of the (Morton-ordered) tile list and
+// its workgroups stride through it.
+struct TileRange {
+    int64_t begin, end, step;
+};
+__device__ __forceinline__ TileRange xcd_range(int64_t ntiles) {
+    const int64_t per = (ntiles + 7) / 8;
+    const int64_t xcd = blockIdx.x % 8, slot = blockIdx.x / 8, nslots = gridDim.x / 8;
+    const int64_t b = xcd * per;
+    const int64_t e = (b + per < ntiles) ? b + per : ntiles;
+    return TileRange{b + slot, e, nslots};
 }
 
 template <int NPE, int NQP, int FORM, int NDN, typename R>
 __global__ __launch_bounds__(kMaxThreads) void k_fused(TileShape shape, TileJob job, int64_t ntiles, FusedParams kp) {
     extern __shared__ __align__(16) unsigned char smem[];
-    // XCD-aware order: consecutive workgroups land on different XCDs, so hand
-    // each XCD a contiguous range of (Morton-adjacent) tiles
-    const int64_t per = (ntiles + 7) / 8;
-    job.tile = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (job.tile >= ntiles) return;
-    fused_tile<NPE, NQP, FORM, NDN, R>(shape, job, kp, smem);
+    const TileRange rg = xcd_range(ntiles);
+    if (rg.begin >= rg.end) return;
+    constexpr int DS = NPE * 3 + 1;
+    double *sD = reinterpret_cast<double *>(smem) + NQP * NPE + NQP * DS;
+    int *sNeg = reinterpret_cast<int *>(sD + 36);
+    load_tables<NPE, NQP>(kp, smem);
+    int64_t t = rg.begin;
+    TileHdr hd = job.hdr[t];
+    Staged st;
+    stage_issue(job, hd, kp.fp.coef, st);
+    for (;;) {
+        stage_store(job, hd, st, shape.layout(hd.H, hd.P, hd.NH, hd.K, hd.NT, hd.NI), smem, sD, sNeg);
+        __syncthreads();
+        const int64_t tn = t + rg.step;
+        TileHdr hn = hd;
+        if (tn < rg.end) hn = job.hdr[tn];
+        tile_compute<NPE, NQP, FORM, NDN, R>(shape, job, kp, hd, smem, tn < rg.end, job, hn, kp.fp.coef, st);
+        if (tn >= rg.end) break;
+        t = tn;
+        hd = hn;
+    }
 }
 
 // One entry per mesh of a batched launch.
@@ -349,62 +458,65 @@ struct BatchEntry {
     int64_t tile0;
 };
 
-// Persistent batched launch: the workgroups of each XCD walk a contiguous
-// range of the global tile list (all meshes' tiles, mesh after mesh); every
-// tile carries its mesh's form, coefficients and arrays.
-template <int NPE, int NQP, typename R>
+// Persistent batched launch: as k_fused over the global tile list (all
+// meshes' tiles, mesh after mesh); every tile carries its mesh's
+// coefficients and arrays.  One launch covers the meshes of one element
+// type, rule, arithmetic and form.
+template <int NPE, int NQP, int FORM, int NDN, typename R>
 __global__ __launch_bounds__(kMaxThreads) void k_fused_batch(const BatchEntry *__restrict__ entries,
                                                           const int32_t *__restrict__ tile_entry, int64_t ntiles) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int64_t per = (ntiles + 7) / 8;
-    const int64_t xcd = blockIdx.x % 8, slot = blockIdx.x / 8, nslots = gridDim.x / 8;
-    const int64_t begin = xcd * per;
-    const int64_t end = (begin + per < ntiles) ? begin + per : ntiles;
-    for (int64_t t = begin + slot; t < end; t += nslots) {
-        const BatchEntry &E = entries[tile_entry[t]];
-        TileJob job = E.job;
-        job.tile = t - E.tile0;
-        switch (E.kp.fp.form) {
-            case FEASM_FORM_LIN_ELASTIC:
-                fused_tile<NPE, NQP, FEASM_FORM_LIN_ELASTIC, 3, R>(E.shape, job, E.kp, smem);
-                break;
-            case FEASM_FORM_DIFFUSION_ISO:
-                fused_tile<NPE, NQP, FEASM_FORM_DIFFUSION_ISO, 1, R>(E.shape, job, E.kp, smem);
-                break;
-            case FEASM_FORM_DIFFUSION_MAT:
-                fused_tile<NPE, NQP, FEASM_FORM_DIFFUSION_MAT, 1, R>(E.shape, job, E.kp, smem);
-                break;
-            default:
-                if (E.kp.fp.ndn == 3)
-                    fused_tile<NPE, NQP, FEASM_FORM_DOT, 3, R>(E.shape, job, E.kp, smem);
-                else if (E.kp.fp.ndn == 2)
-                    fused_tile<NPE, NQP, FEASM_FORM_DOT, 2, R>(E.shape, job, E.kp, smem);
-                else
-                    fused_tile<NPE, NQP, FEASM_FORM_DOT, 1, R>(E.shape, job, E.kp, smem);
-                break;
+    const TileRange rg = xcd_range(ntiles);
+    if (rg.begin >= rg.end) return;
+    constexpr int DS = NPE * 3 + 1;
+    double *sD = reinterpret_cast<double *>(smem) + NQP * NPE + NQP * DS;
+    int *sNeg = reinterpret_cast<int *>(sD + 36);
+    load_tables<NPE, NQP>(entries[tile_entry[rg.begin]].kp, smem);
+    int64_t t = rg.begin;
+    const BatchEntry *E = &entries[tile_entry[t]];
+    TileHdr hd = E->job.hdr[t - E->tile0];
+    Staged st;
+    stage_issue(E->job, hd, E->kp.fp.coef, st);
+    for (;;) {
+        stage_store(E->job, hd, st, E->shape.layout(hd.H, hd.P, hd.NH, hd.K, hd.NT, hd.NI), smem, sD, sNeg);
+        __syncthreads();
+        const int64_t tn = t + rg.step;
+        const BatchEntry *En = E;
+        TileHdr hn = hd;
+        if (tn < rg.end) {
+            En = &entries[tile_entry[tn]];
+            hn = En->job.hdr[tn - En->tile0];
         }
+        tile_compute<NPE, NQP, FORM, NDN, R>(E->shape, E->job, E->kp, hd, smem, tn < rg.end, En->job, hn,
+                                             En->kp.fp.coef, st);
+        if (tn >= rg.end) break;
+        t = tn;
+        E = En;
+        hd = hn;
     }
 }
 
-TileJob make_job(const feasm_mesh *m, const Pattern &p, const TileSet &t, double *nzval) {
+TileJob make_job(const feasm_mesh *m, const TileSet &t, double *nzval) {
     TileJob j;
-    j.node_ptr = dp<int32_t>(t.node_ptr);
-    j.nodes = dp<int32_t>(t.nodes);
-    j.node_cj = dp<int32_t>(t.node_cj);
-    j.halo_ptr = dp<int32_t>(t.halo_ptr);
-    j.halo = dp<int32_t>(t.halo);
-    j.cj_ptr = dp<int32_t>(t.cj_ptr);
-    j.cj = dp<uint32_t>(t.cj);
-    j.hnode_ptr = dp<int32_t>(t.hnode_ptr);
+    j.hdr = static_cast<const TileHdr *>(t.hdr->p);
+    j.blob = static_cast<const unsigned char *>(t.blob->p);
     j.hnodes = dp<int32_t>(t.hnodes);
-    j.lconn = dp<uint16_t>(t.lconn);
-    j.conn = dp<int32_t>(m->conn);
     j.xyz = dp<double>(m->xyz);
-    j.adj_ptr = dp<int64_t>(p.adj_ptr);
-    j.adj = dp<int32_t>(p.adj);
     j.nzval = nzval;
-    j.tile = 0;
     return j;
+}
+
+// workgroups of a persistent launch: as many as stay resident (LDS-bound),
+// a multiple of the 8 XCDs, and no more than the tiles need
+unsigned persistent_grid(feasm_ctx *ctx, size_t budget, int64_t ntiles) {
+    int cus = 0;
+    FEASM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const size_t lds_cu = 160 * 1024;
+    const int per_cu = (int)std::max<size_t>(1, lds_cu / std::max<size_t>(budget, 1));
+    int64_t grid = (int64_t)cus * per_cu;
+    grid = std::max<int64_t>(8, (grid / 8) * 8);
+    const int64_t need = 8 * ((ntiles + 7) / 8);
+    return (unsigned)std::min(grid, std::max<int64_t>(need, 8));
 }
 
 template <int NPE, int NQP, int FORM, int NDN, typename R>
@@ -413,9 +525,8 @@ void go(feasm_ctx *ctx, const feasm_mesh *m, const Pattern &p, const TileSet &t,
     auto kern = k_fused<NPE, NQP, FORM, NDN, R>;
     const int budget = (int)t.shape.budget;
     FEASM_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, budget));
-    const int64_t per = (t.ntiles + 7) / 8;
-    const unsigned grid = (unsigned)(8 * per);
-    TileJob job = make_job(m, p, t, nzval);
+    const unsigned grid = persistent_grid(ctx, t.shape.budget, t.ntiles);
+    TileJob job = make_job(m, t, nzval);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(t.shape.threads), budget, ctx->stream, t.shape, job, t.ntiles, kp);
     FEASM_HIP(hipGetLastError());
 }
@@ -424,7 +535,9 @@ template <int NPE, int NQP, typename R>
 void by_form(feasm_ctx *ctx, const feasm_mesh *m, const Pattern &p, const TileSet &t, const FusedParams &kp,
              double *nzval) {
     switch (kp.fp.form) {
-        case FEASM_FORM_LIN_ELASTIC: return go<NPE, NQP, FEASM_FORM_LIN_ELASTIC, 3, R>(ctx, m, p, t, kp, nzval);
+        case FEASM_FORM_LIN_ELASTIC:
+            if (kp.cubic) return go<NPE, NQP, kFormElasticCubic, 3, R>(ctx, m, p, t, kp, nzval);
+            return go<NPE, NQP, FEASM_FORM_LIN_ELASTIC, 3, R>(ctx, m, p, t, kp, nzval);
         case FEASM_FORM_DIFFUSION_ISO: return go<NPE, NQP, FEASM_FORM_DIFFUSION_ISO, 1, R>(ctx, m, p, t, kp, nzval);
         case FEASM_FORM_DIFFUSION_MAT: return go<NPE, NQP, FEASM_FORM_DIFFUSION_MAT, 1, R>(ctx, m, p, t, kp, nzval);
         case FEASM_FORM_DOT:
@@ -444,9 +557,9 @@ void dispatch(feasm_ctx *ctx, const feasm_mesh *m, const Pattern &p, const TileS
     throw Error(FEASM_EUNSUPPORTED, "the fused kernel covers H8 with 2x2x2 Gauss and T10 with 4-point rules");
 }
 
-template <int NPE, int NQP, typename R>
+template <int NPE, int NQP, int FORM, int NDN, typename R>
 void go_batch(feasm_ctx *ctx, const BatchLaunch &b) {
-    auto kern = k_fused_batch<NPE, NQP, R>;
+    auto kern = k_fused_batch<NPE, NQP, FORM, NDN, R>;
     FEASM_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.budget));
     hipLaunchKernelGGL(kern, dim3(b.grid), dim3(b.threads), b.budget, ctx->stream,
                        static_cast<const BatchEntry *>(b.entries->p), dp<int32_t>(b.tile_entry), b.ntiles);
@@ -463,8 +576,9 @@ void prepare_batch_launch(feasm_ctx *ctx, BatchLaunch &b, const std::vector<Batc
     for (size_t i = 0; i < items.size(); i++) {
         const BatchMesh &it = items[i];
         BatchEntry E;
-        E.job = make_job(it.mesh, *it.mesh->pattern, *it.tiles, it.nzval);
+        E.job = make_job(it.mesh, *it.tiles, it.nzval);
         E.kp.fp = it.fp;
+        E.kp.cubic = (it.fp.form == FEASM_FORM_LIN_ELASTIC && cubic_symmetry(it.fp.coef)) ? 1 : 0;
         const DevQTable &qt = device_qtable(ctx, it.mesh->etype, it.rule, it.order);
         E.kp.N = qt.N();
         E.kp.dN = qt.dN();
@@ -481,25 +595,43 @@ void prepare_batch_launch(feasm_ctx *ctx, BatchLaunch &b, const std::vector<Batc
     b.tile_entry = dev_alloc(std::max<size_t>(tile_entry.size() * 4, 4));
     FEASM_HIP(hipMemcpy(b.entries->p, host.data(), host.size() * sizeof(BatchEntry), hipMemcpyHostToDevice));
     FEASM_HIP(hipMemcpy(b.tile_entry->p, tile_entry.data(), tile_entry.size() * 4, hipMemcpyHostToDevice));
-    int cus = 0;
-    FEASM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    const size_t lds_cu = 160 * 1024;
-    const int per_cu = (int)std::max<size_t>(1, lds_cu / std::max<size_t>(b.budget, 1));
-    int64_t grid = (int64_t)cus * per_cu;
-    grid = std::max<int64_t>(8, (grid / 8) * 8);
-    const int64_t need = 8 * ((b.ntiles + 7) / 8);
-    b.grid = (unsigned)std::min(grid, std::max<int64_t>(need, 8));
+    b.grid = persistent_grid(ctx, b.budget, b.ntiles);
+}
+
+namespace {
+
+template <int NPE, int NQP, typename R>
+void batch_by_form(feasm_ctx *ctx, const BatchLaunch &b) {
+    switch (b.kform) {
+        case FEASM_FORM_LIN_ELASTIC: return go_batch<NPE, NQP, FEASM_FORM_LIN_ELASTIC, 3, R>(ctx, b);
+        case kFormElasticCubic: return go_batch<NPE, NQP, kFormElasticCubic, 3, R>(ctx, b);
+        case FEASM_FORM_DIFFUSION_ISO: return go_batch<NPE, NQP, FEASM_FORM_DIFFUSION_ISO, 1, R>(ctx, b);
+        case FEASM_FORM_DIFFUSION_MAT: return go_batch<NPE, NQP, FEASM_FORM_DIFFUSION_MAT, 1, R>(ctx, b);
+        case FEASM_FORM_DOT:
+            if (b.ndn == 1) return go_batch<NPE, NQP, FEASM_FORM_DOT, 1, R>(ctx, b);
+            if (b.ndn == 2) return go_batch<NPE, NQP, FEASM_FORM_DOT, 2, R>(ctx, b);
+            if (b.ndn == 3) return go_batch<NPE, NQP, FEASM_FORM_DOT, 3, R>(ctx, b);
+            break;
+    }
+    throw Error(FEASM_EUNSUPPORTED, "unsupported form in the batched launch");
+}
+
+}  // namespace
+
+int fused_kernel_form(const FormParams &fp) {
+    if (fp.form == FEASM_FORM_LIN_ELASTIC && cubic_symmetry(fp.coef)) return kFormElasticCubic;
+    return fp.form;
 }
 
 void launch_batch(feasm_ctx *ctx, const BatchLaunch &b) {
     if (b.ntiles == 0) return;
     if (b.npe == 8 && b.nqp == 8) {
-        if (b.rbytes == 4) return go_batch<8, 8, float>(ctx, b);
-        return go_batch<8, 8, double>(ctx, b);
+        if (b.rbytes == 4) return batch_by_form<8, 8, float>(ctx, b);
+        return batch_by_form<8, 8, double>(ctx, b);
     }
     if (b.npe == 10 && b.nqp == 4) {
-        if (b.rbytes == 4) return go_batch<10, 4, float>(ctx, b);
-        return go_batch<10, 4, double>(ctx, b);
+        if (b.rbytes == 4) return batch_by_form<10, 4, float>(ctx, b);
+        return batch_by_form<10, 4, double>(ctx, b);
     }
     throw Error(FEASM_EUNSUPPORTED, "batched launch covers H8 2x2x2 Gauss and T10 4-point rules");
 }
@@ -510,7 +642,6 @@ TileShape tile_shape_for(feasm_ctx *ctx, int npe, int nqp, int form, int precisi
     TileShape s;
     s.npe = npe;
     s.nqp = nqp;
-    s.swp = (form == FEASM_FORM_LIN_ELASTIC || form == FEASM_FORM_DIFFUSION_MAT) ? 9 : 1;
     s.rbytes = (precision == FEASM_MIXED) ? 4 : 8;
     s.grad = (form != FEASM_FORM_DOT);
     int lds = 0;
@@ -518,8 +649,7 @@ TileShape tile_shape_for(feasm_ctx *ctx, int npe, int nqp, int form, int precisi
     s.budget = (size_t)std::min(lds, 160 * 1024);
     s.max_nodes = 64;
     s.max_halo = 255;
-    s.max_cj = 4096;
-    s.threads = 512;
+    s.threads = 1024;
     return s;
 }
 
@@ -528,6 +658,7 @@ void launch_fused(feasm_ctx *ctx, const feasm_mesh *m, const Pattern &p, const T
     if (t.ntiles == 0) return;
     FusedParams kp;
     kp.fp = fp;
+    kp.cubic = (fp.form == FEASM_FORM_LIN_ELASTIC && cubic_symmetry(fp.coef)) ? 1 : 0;
     kp.N = qt.N();
     kp.dN = qt.dN();
     kp.w = qt.w();

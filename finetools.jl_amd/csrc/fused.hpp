@@ -10,70 +10,88 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(
 // same function sizes the tiles on the host and carves LDS on the device.
 struct TileShape {
     int npe = 8, nqp = 8;
-    int swp = 9;       // values per (element, node pair) partial: 9 (3x3 M) or 1
     int rbytes = 8;    // sizeof of the element arithmetic type
     bool grad = true;  // forms that need spatial gradients (all but DOT)
-    int max_nodes = 64, max_halo = 255, max_cj = 1024;
-    int threads = 256;  // workgroup size of the fused kernel (256 or 512)
+    int max_nodes = 64, max_halo = 255;
+    int threads = 256;  // workgroup size of the fused kernel (256, 512 or 1024)
     size_t budget = 0;  // bytes of LDS available to one workgroup
 
+    // doubles between consecutive halo elements' scaled-gradient records:
+    // nqp*npe*3 plus one, so that records of different elements start in
+    // different LDS banks
+    __host__ __device__ int ustride() const { return nqp * npe * 3 + 1; }
+
     struct Layout {
-        size_t dN, X, lconn, node, ncj, bp, conn, U, S, cj, P, total;
+        size_t tab, X, lconn, cbase, cdeg, slot, task, item, U, sgn, total;
     };
-    // H halo elements, CJ (column, element) pairs, P owned nodes, NH distinct
-    // nodes of the halo
-    __host__ __device__ Layout layout(int64_t H, int64_t CJ, int P, int64_t NH) const {
+    // doubles between the gradient-table rows of consecutive points: npe*3
+    // plus one, for the same reason
+    __host__ __device__ int dstride() const { return npe * 3 + 1; }
+    // H halo elements, P owned nodes, NH distinct nodes of the halo, K the
+    // largest number of elements shared by the two nodes of one block, NT
+    // block tasks and NI (block, element) items
+    // The tile's metadata [cbase .. lconn] is one contiguous blob, stored in
+    // HBM exactly as it sits in LDS (offsets relative to cbase).
+    __host__ __device__ Layout layout(int64_t H, int P, int64_t NH, int K, int64_t NT, int64_t NI) const {
         Layout L;
         size_t o = 0;
-        L.dN = o;
-        o = align16(o + (size_t)nqp * npe * 3 * 8);
-        L.X = o;
-        o = align16(o + (size_t)NH * 3 * 8);
+        L.tab = o;  // N [nqp][npe], dN [nqp][dstride], the 36 material coefficients (FP64), a flag
+        o = align16(o + ((size_t)nqp * (npe + dstride()) + 36) * 8 + 4);
+        L.cbase = o;
+        o = align16(o + (size_t)P * 8);
+        L.cdeg = o;
+        o = align16(o + (size_t)P * 4);
+        L.slot = o;
+        o = align16(o + (size_t)(K + 1) * 4);
+        L.task = o;
+        o = align16(o + (size_t)NT * 4);
+        L.item = o;
+        o = align16(o + (size_t)NI * 2);
         L.lconn = o;
         o = align16(o + (size_t)H * npe * 2);
-        L.node = o;
-        o = align16(o + (size_t)(P + 1) * 4);
-        L.ncj = o;
-        o = align16(o + (size_t)(P + 1) * 4);
-        L.bp = o;
-        o = align16(o + (size_t)(P + 1) * 4);
-        L.conn = o;
-        o = align16(o + (size_t)H * npe * 4);
+        L.X = o;  // end of the blob
+        o = align16(o + (size_t)NH * 3 * 8);
         L.U = o;
-        o = align16(o + (grad ? (size_t)H * nqp * npe * 3 * rbytes : 0));
-        L.S = o;
-        o = align16(o + (size_t)H * nqp * rbytes);
-        L.cj = o;
-        o = align16(o + (size_t)CJ * 4);
-        L.P = o;
-        o = align16(o + (size_t)CJ * npe * swp * rbytes);
+        o = align16(o + (grad ? (size_t)H * ustride() * rbytes : 0));
+        L.sgn = o;  // grad forms: one byte per (element, point), 1 where J w < 0; DOT: J w
+        o = align16(o + (grad ? (size_t)H * nqp : (size_t)H * nqp * rbytes));
         L.total = o;
         return L;
     }
-    size_t lds_bytes(int64_t H, int64_t CJ, int P, int64_t NH) const { return layout(H, CJ, P, NH).total; }
+    size_t lds_bytes(int64_t H, int P, int64_t NH, int K, int64_t NT, int64_t NI) const {
+        return layout(H, P, NH, K, NT, NI).total;
+    }
     int64_t key() const {
-        return (((((int64_t)(npe * 64 + nqp) * 16 + swp) * 16 + rbytes * 2 + (grad ? 1 : 0)) * 256 + max_nodes) *
-                    4096 +
+        return ((((int64_t)(npe * 64 + nqp) * 16 + rbytes * 2 + (grad ? 1 : 0)) * 256 + max_nodes) * 4096 +
                 (int64_t)(budget / 1024)) *
-                   4 +
-               (threads == 512 ? 1 : 0);
+                   2048 +
+               threads;
     }
 };
 
+// Per tile: its sizes and where its blob and halo node list start.
+struct TileHdr {
+    int32_t P, H, NH, NT;  // owned columns, halo elements, distinct halo nodes, block tasks
+    int32_t K, NI;         // most elements shared by one block's nodes; (block, element) items
+    int32_t x0;            // halo nodes: hnodes [x0, x0+NH)
+    int32_t nbytes;        // blob size (a multiple of 16)
+    int64_t blob;          // blob offset in bytes (16-aligned)
+};
+
+// Blob contents (see TileShape::layout): int64 cbase[P] (first node block of
+// each owned column), int32 cdeg[P] (its node blocks), int32 slot[K+1] (items
+// of slot j for task t at slot[j] + t), uint32 task[NT] (c | r << 8 | k << 16 |
+// (row > column) << 24 | (row == column) << 25, sorted by k descending),
+// uint16 item[NI] (h << 8 | index of the column node in e << 4 | index of the
+// row node), uint16 lconn[H*npe] (halo connectivity as indices into the
+// tile's hnodes).
 struct TileSet {
     TileShape shape;
-    int64_t ntiles = 0, nhalo = 0;
-    int max_nodes = 0, max_halo = 0, max_cj = 0;
-    DevPtr node_ptr;  // int32 [ntiles+1]
-    DevPtr nodes;     // int32 owned nodes, tile after tile
-    DevPtr node_cj;   // int32 per owned node: first cj entry (tile-relative)
-    DevPtr halo_ptr;  // int32 [ntiles+1]
-    DevPtr halo;      // int32 halo elements, ascending within a tile
-    DevPtr cj_ptr;    // int32 [ntiles+1]
-    DevPtr cj;        // uint32 (owned-local c << 16) | (halo-local e << 8) | local index of c in e
-    DevPtr hnode_ptr; // int32 [ntiles+1]
-    DevPtr hnodes;    // int32 distinct nodes of each tile's halo
-    DevPtr lconn;     // uint16 per halo element, its nodes as indices into the tile's hnodes
+    int64_t ntiles = 0, nhalo = 0, ntasks = 0, nitems = 0, blob_bytes = 0;
+    int max_nodes = 0, max_halo = 0, max_k = 0;
+    DevPtr hdr;     // TileHdr [ntiles]
+    DevPtr blob;    // the tiles' blobs
+    DevPtr hnodes;  // int32 distinct nodes of each tile's halo, ascending
 };
 
 std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape);
@@ -90,12 +108,16 @@ struct BatchMesh {
 // the element type, rule size and arithmetic
 struct BatchLaunch {
     int npe = 8, nqp = 8, rbytes = 8, threads = 256;
+    int kform = 0, ndn = 1;  // kernel form (fused_kernel_form) and DOFs per node
     int64_t ntiles = 0;
     size_t budget = 0;
     unsigned grid = 0;
     DevPtr entries, tile_entry;
 };
 void prepare_batch_launch(feasm_ctx *ctx, BatchLaunch &b, const std::vector<BatchMesh> &items);
+// the fused kernel's form id for these parameters (forms with a specialised
+// kernel map to their own id)
+int fused_kernel_form(const FormParams &fp);
 void launch_batch(feasm_ctx *ctx, const BatchLaunch &b);
 TileShape tile_shape_for(feasm_ctx *ctx, int npe, int nqp, int form, int precision);
 void launch_fused(feasm_ctx *ctx, const feasm_mesh *m, const Pattern &p, const TileSet &t, const DevQTable &qt,

@@ -4,12 +4,24 @@
 // order of the node coordinates so that it is spatially compact, together
 // with its halo: every element incident to an owned node, in ascending
 // element number.  One workgroup integrates the halo elements into LDS and
-// produces every CSC block of the owned columns, summing the element
-// contributions of each block in ascending element order.  Tiles are grown
-// greedily along the Morton order until the next node would overflow the
-// LDS budget the kernel has for this form and precision.
+// produces every CSC node block of the owned columns.
+//
+// The work of a tile is listed as block tasks: one per node block (m, c) of
+// an owned column c, carrying the elements that hold both nodes in ascending
+// element number (the order in which the reference's sparse() folds the
+// duplicates of its element-major COO, src/AssemblyModule.jl:319-325), each
+// with the local indices of the two nodes in the element.  Tasks are sorted
+// by their element count, so that the lanes of one wavefront run loops of the
+// same length, and their items are stored slot-major, so that lane t reads
+// item j of its task at slot[j] + t: one coalesced load per slot.
+//
+// Tiles are grown along the Morton order until the next node would overflow
+// the LDS budget; the cut is then moved back to the coarsest Morton boundary
+// in the second half of the tile, which keeps tiles box-shaped.
 #include <algorithm>
+#include <cmath>
 #include <numeric>
+#include <thread>
 
 #include "common.hpp"
 #include "fused.hpp"
@@ -28,168 +40,322 @@ uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
     return v;
 }
 
-}  // namespace
+struct HostMesh {
+    int npe = 0;
+    int64_t nn = 0, ne = 0;
+    std::vector<int32_t> conn;
+    std::vector<double> xyz;
+    std::vector<int64_t> n2e_ptr, adj_ptr;
+    std::vector<uint32_t> n2e;
+    std::vector<int32_t> adj;
+};
 
-std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape) {
-    feasm_ctx *ctx = m->ctx;
-    const int64_t nn = m->nnodes, ne = m->nelem;
-    const int npe = m->npe;
-    std::vector<int32_t> conn(ne * npe);
-    std::vector<double> xyz(nn * 3);
-    std::vector<int64_t> n2e_ptr(nn + 1);
-    std::vector<uint32_t> n2e(ne * npe);
-    FEASM_HIP(hipMemcpy(conn.data(), m->conn->p, conn.size() * 4, hipMemcpyDeviceToHost));
-    FEASM_HIP(hipMemcpy(xyz.data(), m->xyz->p, xyz.size() * 8, hipMemcpyDeviceToHost));
-    FEASM_HIP(hipMemcpy(n2e_ptr.data(), m->n2e_ptr->p, (nn + 1) * 8, hipMemcpyDeviceToHost));
-    FEASM_HIP(hipMemcpy(n2e.data(), m->n2e->p, n2e.size() * 4, hipMemcpyDeviceToHost));
-
-    // Morton order of the nodes
+// Node order along a Z-curve of a lattice whose spacing is the mean node
+// spacing, so that runs of the order are nearly box-shaped.
+std::vector<std::pair<uint64_t, int32_t>> morton_order(const HostMesh &h) {
+    const int64_t nn = h.nn;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int64_t n = 0; n < nn; n++)
         for (int k = 0; k < 3; k++) {
-            lo[k] = std::min(lo[k], xyz[3 * n + k]);
-            hi[k] = std::max(hi[k], xyz[3 * n + k]);
+            lo[k] = std::min(lo[k], h.xyz[3 * n + k]);
+            hi[k] = std::max(hi[k], h.xyz[3 * n + k]);
         }
-    double span = 0;
-    for (int k = 0; k < 3; k++) span = std::max(span, hi[k] - lo[k]);
-    if (!(span > 0)) span = 1;
+    double vol = 1, span = 0;
+    int dims = 0;
+    for (int k = 0; k < 3; k++) {
+        span = std::max(span, hi[k] - lo[k]);
+        if (hi[k] - lo[k] > 0) {
+            vol *= hi[k] - lo[k];
+            dims++;
+        }
+    }
+    double step = (dims > 0 && nn > 1) ? std::pow(vol / (double)nn, 1.0 / dims) : 1.0;
+    if (!(step > 0) || span / step > 2e6) step = (span > 0 ? span : 1.0) / 2e6;
     std::vector<std::pair<uint64_t, int32_t>> key(nn);
     for (int64_t n = 0; n < nn; n++) {
         uint64_t code = 0;
         for (int k = 0; k < 3; k++) {
-            double t = (xyz[3 * n + k] - lo[k]) / span;
-            uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0));
+            const double t = std::floor((h.xyz[3 * n + k] - lo[k]) / step + 0.5);
+            const uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, t));
             code |= spread3(q) << k;
         }
         key[n] = {code, (int32_t)n};
     }
     std::sort(key.begin(), key.end());
+    return key;
+}
+
+// Everything one tile contributes to the tile arrays.
+struct TileOut {
+    int P = 0, H = 0, NH = 0, K = 0;
+    std::vector<int64_t> cbase;
+    std::vector<int32_t> cdeg, hnodes;
+    std::vector<uint16_t> lconn, item;
+    std::vector<uint32_t> task;
+    std::vector<int32_t> slot;
+};
+
+void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
+    const int npe = h.npe;
+    std::vector<int32_t> halo;
+    for (int c = 0; c < P; c++)
+        for (int64_t s = h.n2e_ptr[owned[c]]; s < h.n2e_ptr[owned[c] + 1]; s++) halo.push_back((int32_t)(h.n2e[s] >> 4));
+    std::sort(halo.begin(), halo.end());
+    halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+    std::vector<int32_t> hn;
+    for (int32_t e : halo)
+        for (int a = 0; a < npe; a++) hn.push_back(h.conn[(int64_t)e * npe + a]);
+    std::sort(hn.begin(), hn.end());
+    hn.erase(std::unique(hn.begin(), hn.end()), hn.end());
+    o.P = P;
+    o.H = (int)halo.size();
+    o.NH = (int)hn.size();
+    FEASM_REQUIRE(o.H <= 255 && o.NH <= 65535 && npe <= 15 && P <= 255, FEASM_EUNSUPPORTED,
+                  "tile exceeds the fused kernel's index widths");
+    o.hnodes = hn;
+    o.lconn.resize((size_t)o.H * npe);
+    for (int i = 0; i < o.H; i++)
+        for (int a = 0; a < npe; a++) {
+            const int32_t v = h.conn[(int64_t)halo[i] * npe + a];
+            o.lconn[(size_t)i * npe + a] = (uint16_t)(std::lower_bound(hn.begin(), hn.end(), v) - hn.begin());
+        }
+    // block tasks: items of (column c, row position r) in ascending element
+    struct Task {
+        int c, r;
+        bool sw, diag;
+        std::vector<uint16_t> it;
+    };
+    std::vector<Task> tasks;
+    o.cbase.resize(P);
+    o.cdeg.resize(P);
+    for (int c = 0; c < P; c++) {
+        const int32_t node = owned[c];
+        const int64_t b0 = h.adj_ptr[node], deg = h.adj_ptr[node + 1] - b0;
+        FEASM_REQUIRE(deg <= 255, FEASM_EUNSUPPORTED, "a node has more than 255 neighbours");
+        o.cbase[c] = b0;
+        o.cdeg[c] = (int32_t)deg;
+        const int32_t *rows = h.adj.data() + b0;
+        const size_t tb = tasks.size();
+        for (int r = 0; r < deg; r++) tasks.push_back(Task{c, r, rows[r] > node, rows[r] == node, {}});
+        for (int64_t s = h.n2e_ptr[node]; s < h.n2e_ptr[node + 1]; s++) {
+            const int32_t e = (int32_t)(h.n2e[s] >> 4);
+            const int ac = (int)(h.n2e[s] & 15u);
+            const int hl = (int)(std::lower_bound(halo.begin(), halo.end(), e) - halo.begin());
+            for (int a = 0; a < npe; a++) {
+                const int32_t v = h.conn[(int64_t)e * npe + a];
+                const int r = (int)(std::lower_bound(rows, rows + deg, v) - rows);
+                FEASM_REQUIRE(r < deg && rows[r] == v, FEASM_ESTATE, "pattern misses an element node pair");
+                tasks[tb + r].it.push_back((uint16_t)((hl << 8) | (ac << 4) | a));
+            }
+        }
+    }
+    std::stable_sort(tasks.begin(), tasks.end(),
+                     [](const Task &x, const Task &y) { return x.it.size() > y.it.size(); });
+    const int K = tasks.empty() ? 0 : (int)tasks.front().it.size();
+    FEASM_REQUIRE(K <= 255, FEASM_EUNSUPPORTED, "a node block is shared by more than 255 elements");
+    o.K = K;
+    o.task.resize(tasks.size());
+    for (size_t t = 0; t < tasks.size(); t++) {
+        const Task &T = tasks[t];
+        o.task[t] = (uint32_t)T.c | ((uint32_t)T.r << 8) | ((uint32_t)T.it.size() << 16) |
+                    ((uint32_t)T.sw << 24) | ((uint32_t)T.diag << 25);
+    }
+    // slot-major items: slot j holds item j of every task with more than j items
+    o.slot.assign(K + 1, 0);
+    for (int j = 0; j < K; j++) {
+        int nj = 0;
+        while (nj < (int)tasks.size() && (int)tasks[nj].it.size() > j) nj++;
+        o.slot[j + 1] = o.slot[j] + nj;
+        for (int t = 0; t < nj; t++) o.item.push_back(tasks[t].it[j]);
+    }
+}
+
+}  // namespace
+
+std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape) {
+    const Pattern &p = *m->pattern;
+    HostMesh h;
+    h.npe = m->npe;
+    h.nn = m->nnodes;
+    h.ne = m->nelem;
+    h.conn.resize(h.ne * h.npe);
+    h.xyz.resize(h.nn * 3);
+    h.n2e_ptr.resize(h.nn + 1);
+    h.n2e.resize(h.ne * h.npe);
+    h.adj_ptr.resize(h.nn + 1);
+    FEASM_HIP(hipMemcpy(h.conn.data(), m->conn->p, h.conn.size() * 4, hipMemcpyDeviceToHost));
+    FEASM_HIP(hipMemcpy(h.xyz.data(), m->xyz->p, h.xyz.size() * 8, hipMemcpyDeviceToHost));
+    FEASM_HIP(hipMemcpy(h.n2e_ptr.data(), m->n2e_ptr->p, (h.nn + 1) * 8, hipMemcpyDeviceToHost));
+    FEASM_HIP(hipMemcpy(h.n2e.data(), m->n2e->p, h.n2e.size() * 4, hipMemcpyDeviceToHost));
+    FEASM_HIP(hipMemcpy(h.adj_ptr.data(), p.adj_ptr->p, (h.nn + 1) * 8, hipMemcpyDeviceToHost));
+    h.adj.resize(h.adj_ptr[h.nn]);
+    FEASM_HIP(hipMemcpy(h.adj.data(), p.adj->p, h.adj.size() * 4, hipMemcpyDeviceToHost));
+
+    const auto key = morton_order(h);
+    const int64_t nn = h.nn;
+    const int npe = h.npe;
+
+    // ---- partition the Morton order into tiles
+    std::vector<int32_t> estamp(h.ne, -1), nstamp(nn, -1);
+    int32_t epoch = 0;
+    int64_t H = 0, NH = 0, K = 0, NT = 0, NI = 0;
+    auto add_node = [&](int32_t n) {
+        const int64_t d = h.n2e_ptr[n + 1] - h.n2e_ptr[n];
+        K = std::max(K, d);
+        NT += h.adj_ptr[n + 1] - h.adj_ptr[n];
+        NI += d * npe;
+        for (int64_t s = h.n2e_ptr[n]; s < h.n2e_ptr[n + 1]; s++) {
+            const int32_t e = (int32_t)(h.n2e[s] >> 4);
+            if (estamp[e] == epoch) continue;
+            estamp[e] = epoch;
+            H++;
+            for (int a = 0; a < npe; a++) {
+                const int32_t v = h.conn[(int64_t)e * npe + a];
+                if (nstamp[v] != epoch) {
+                    nstamp[v] = epoch;
+                    NH++;
+                }
+            }
+        }
+    };
+    // what the tile would hold with node n added (state left unchanged)
+    std::vector<int32_t> fresh_nodes;
+    auto probe = [&](int32_t n, int64_t &h2, int64_t &nh2, int64_t &k2) {
+        h2 = H;
+        nh2 = NH;
+        k2 = std::max(K, h.n2e_ptr[n + 1] - h.n2e_ptr[n]);
+        fresh_nodes.clear();
+        for (int64_t s = h.n2e_ptr[n]; s < h.n2e_ptr[n + 1]; s++) {
+            const int32_t e = (int32_t)(h.n2e[s] >> 4);
+            if (estamp[e] == epoch) continue;
+            h2++;
+            for (int a = 0; a < npe; a++) {
+                const int32_t v = h.conn[(int64_t)e * npe + a];
+                if (nstamp[v] != epoch && std::find(fresh_nodes.begin(), fresh_nodes.end(), v) == fresh_nodes.end()) {
+                    fresh_nodes.push_back(v);
+                    nh2++;
+                }
+            }
+        }
+    };
+    auto fits = [&](int64_t P, int64_t h2, int64_t nh2, int64_t k2, int64_t nt2, int64_t ni2) {
+        return P <= shape.max_nodes && P <= 255 && h2 <= shape.max_halo && h2 <= 255 && nh2 <= 65535 &&
+               k2 <= 255 && shape.lds_bytes(h2, (int)P, nh2, (int)k2, nt2, ni2) <= shape.budget;
+    };
+    auto level = [&](int64_t j) {  // coarseness of the Morton boundary before sorted position j
+        const uint64_t x = key[j - 1].first ^ key[j].first;
+        return x ? 64 - __builtin_clzll(x) : 0;
+    };
+    std::vector<int64_t> starts;
+    int64_t s = 0;
+    while (s < nn) {
+        epoch++;
+        H = NH = K = NT = NI = 0;
+        int64_t i = s;
+        for (; i < nn; i++) {
+            const int32_t n = key[i].second;
+            int64_t h2, nh2, k2;
+            probe(n, h2, nh2, k2);
+            const int64_t nt2 = NT + (h.adj_ptr[n + 1] - h.adj_ptr[n]);
+            const int64_t ni2 = NI + (h.n2e_ptr[n + 1] - h.n2e_ptr[n]) * npe;
+            if (!fits(i - s + 1, h2, nh2, k2, nt2, ni2)) break;
+            add_node(key[i].second);
+        }
+        FEASM_REQUIRE(i > s, FEASM_EUNSUPPORTED, "a node has too many incident elements for the fused kernel");
+        int64_t cut = i;
+        if (i < nn && i - s >= 4) {  // move the cut back to the coarsest boundary in the second half
+            int best = -1;
+            for (int64_t j = i; j >= s + (i - s + 1) / 2; j--) {
+                const int l = level(j);
+                if (l > best) {
+                    best = l;
+                    cut = j;
+                }
+            }
+        }
+        starts.push_back(s);
+        s = cut;
+    }
+    starts.push_back(nn);
+    const int64_t ntiles = (int64_t)starts.size() - 1;
+
+    // ---- per-tile arrays, built in parallel
+    std::vector<int32_t> order(nn);
+    for (int64_t i = 0; i < nn; i++) order[i] = key[i].second;
+    std::vector<TileOut> outs(ntiles);
+    const int nthreads = (int)std::max<int64_t>(1, std::min<int64_t>(16, std::min<int64_t>(
+                                                          std::thread::hardware_concurrency(), ntiles / 64 + 1)));
+    std::vector<std::thread> pool;
+    std::vector<std::string> errs(nthreads);
+    std::vector<int> codes(nthreads, 0);
+    for (int w = 0; w < nthreads; w++)
+        pool.emplace_back([&, w]() {
+            try {
+                for (int64_t t = w; t < ntiles; t += nthreads)
+                    make_tile(h, order.data() + starts[t], (int)(starts[t + 1] - starts[t]), outs[t]);
+            } catch (const Error &e) {
+                codes[w] = e.code;
+                errs[w] = e.what();
+            }
+        });
+    for (auto &th : pool) th.join();
+    for (int w = 0; w < nthreads; w++)
+        if (codes[w]) throw Error(codes[w], errs[w]);
 
     auto T = std::make_shared<TileSet>();
     T->shape = shape;
-    std::vector<int32_t> node_ptr{0}, nodes, halo_ptr{0}, halo, cj_ptr{0}, node_cj, hnode_ptr{0}, hnodes;
-    std::vector<uint32_t> cj;
-    std::vector<uint16_t> lconn;
-    std::vector<int32_t> nstamp(nn, -1), nlocal(nn, 0);  // halo-node membership of the current tile
-    int64_t cur_nh = 0;
-    nodes.reserve(nn);
-    node_cj.reserve(nn + 1);
-    std::vector<int32_t> stamp(ne, -1), local(ne, 0);
-    std::vector<int32_t> cur_nodes, cur_halo;
-    int64_t cur_cj = 0;
-    int32_t tile = 0;
-    auto close_tile = [&]() {
-        if (cur_nodes.empty()) return;
-        std::sort(cur_halo.begin(), cur_halo.end());
-        for (size_t i = 0; i < cur_halo.size(); i++) local[cur_halo[i]] = (int32_t)i;
-        int32_t cstart = 0;
-        for (size_t c = 0; c < cur_nodes.size(); c++) {
-            const int32_t n = cur_nodes[c];
-            nodes.push_back(n);
-            node_cj.push_back(cstart);
-            for (int64_t s = n2e_ptr[n]; s < n2e_ptr[n + 1]; s++) {  // ascending element
-                const uint32_t e = n2e[s] >> 4, a = n2e[s] & 15u;
-                cj.push_back(((uint32_t)c << 16) | ((uint32_t)local[e] << 8) | a);
-                cstart++;
-            }
-        }
-        halo.insert(halo.end(), cur_halo.begin(), cur_halo.end());
-        // distinct halo nodes, ascending, and the halo connectivity in local indices
-        std::vector<int32_t> hn;
-        for (int32_t e : cur_halo)
-            for (int a = 0; a < npe; a++) hn.push_back(conn[(int64_t)e * npe + a]);
-        std::sort(hn.begin(), hn.end());
-        hn.erase(std::unique(hn.begin(), hn.end()), hn.end());
-        for (size_t i = 0; i < hn.size(); i++) nlocal[hn[i]] = (int32_t)i;
-        for (int32_t e : cur_halo)
-            for (int a = 0; a < npe; a++) lconn.push_back((uint16_t)nlocal[conn[(int64_t)e * npe + a]]);
-        hnodes.insert(hnodes.end(), hn.begin(), hn.end());
-        hnode_ptr.push_back((int32_t)hnodes.size());
-        node_ptr.push_back((int32_t)nodes.size());
-        halo_ptr.push_back((int32_t)halo.size());
-        cj_ptr.push_back((int32_t)cj.size());
-        T->max_nodes = std::max<int>(T->max_nodes, (int)cur_nodes.size());
-        T->max_halo = std::max<int>(T->max_halo, (int)cur_halo.size());
-        T->max_cj = std::max<int>(T->max_cj, (int)cur_cj);
-        cur_nodes.clear();
-        cur_halo.clear();
-        cur_cj = 0;
-        cur_nh = 0;
-        tile++;
-    };
-    std::vector<int32_t> seen;
-    seen.reserve(256);
-    for (int64_t i = 0; i < nn; i++) {
-        const int32_t n = key[i].second;
-        const int64_t deg = n2e_ptr[n + 1] - n2e_ptr[n];
-        // what adding node n would add: new halo elements and new halo nodes
-        auto count_new = [&](int64_t &fresh, int64_t &fresh_nodes) {
-            fresh = 0;
-            fresh_nodes = 0;
-            seen.clear();
-            for (int64_t s = n2e_ptr[n]; s < n2e_ptr[n + 1]; s++) {
-                const int32_t e = (int32_t)(n2e[s] >> 4);
-                if (stamp[e] == tile) continue;
-                fresh++;
-                for (int a = 0; a < npe; a++) {
-                    const int32_t v = conn[(int64_t)e * npe + a];
-                    if (nstamp[v] != tile && std::find(seen.begin(), seen.end(), v) == seen.end()) {
-                        seen.push_back(v);
-                        fresh_nodes++;
-                    }
-                }
-            }
-        };
-        int64_t fresh, fresh_nodes;
-        count_new(fresh, fresh_nodes);
-        const int64_t H = (int64_t)cur_halo.size() + fresh, CJ = cur_cj + deg, NH = cur_nh + fresh_nodes;
-        const bool fits = (int64_t)cur_nodes.size() + 1 <= shape.max_nodes && H <= shape.max_halo &&
-                          CJ <= shape.max_cj && NH <= 65535 &&
-                          shape.lds_bytes(H, CJ, (int)cur_nodes.size() + 1, NH) <= shape.budget;
-        if (!fits && !cur_nodes.empty()) {
-            close_tile();
-            count_new(fresh, fresh_nodes);
-        }
-        FEASM_REQUIRE(deg <= shape.max_cj && deg <= shape.max_halo &&
-                          shape.lds_bytes(deg, deg, 1, fresh_nodes) <= shape.budget,
-                      FEASM_EUNSUPPORTED, "a node has too many incident elements for the fused kernel");
-        for (int64_t s = n2e_ptr[n]; s < n2e_ptr[n + 1]; s++) {
-            const int32_t e = (int32_t)(n2e[s] >> 4);
-            if (stamp[e] != tile) {
-                stamp[e] = tile;
-                cur_halo.push_back(e);
-                for (int a = 0; a < npe; a++) {
-                    const int32_t v = conn[(int64_t)e * npe + a];
-                    if (nstamp[v] != tile) {
-                        nstamp[v] = tile;
-                        cur_nh++;
-                    }
-                }
-            }
-        }
-        cur_nodes.push_back(n);
-        cur_cj += deg;
+    T->ntiles = ntiles;
+    std::vector<TileHdr> hdr(ntiles);
+    int64_t nX = 0, nB = 0;
+    for (int64_t t = 0; t < ntiles; t++) {
+        const TileOut &o = outs[t];
+        const TileShape::Layout L =
+            shape.layout(o.H, o.P, o.NH, o.K, (int64_t)o.task.size(), (int64_t)o.item.size());
+        TileHdr &d = hdr[t];
+        d.P = o.P;
+        d.H = o.H;
+        d.NH = o.NH;
+        d.NT = (int32_t)o.task.size();
+        d.K = o.K;
+        d.NI = (int32_t)o.item.size();
+        d.x0 = (int32_t)nX;
+        d.nbytes = (int32_t)(L.X - L.cbase);
+        d.blob = nB;
+        nX += o.NH;
+        nB += d.nbytes;
+        T->nhalo += o.H;
+        T->ntasks += d.NT;
+        T->nitems += d.NI;
+        T->max_nodes = std::max(T->max_nodes, o.P);
+        T->max_halo = std::max(T->max_halo, o.H);
+        T->max_k = std::max(T->max_k, o.K);
     }
-    close_tile();
-    T->ntiles = tile;
-    T->nhalo = (int64_t)halo.size();
-    node_cj.push_back(0);  // keeps the array non-empty
-    auto up = [&](const void *src, size_t bytes) {
-        DevPtr d = dev_alloc(std::max<size_t>(bytes, 4));
-        if (bytes) FEASM_HIP(hipMemcpy(d->p, src, bytes, hipMemcpyHostToDevice));
-        return d;
-    };
-    T->node_ptr = up(node_ptr.data(), node_ptr.size() * 4);
-    T->nodes = up(nodes.data(), nodes.size() * 4);
-    T->node_cj = up(node_cj.data(), node_cj.size() * 4);
-    T->halo_ptr = up(halo_ptr.data(), halo_ptr.size() * 4);
-    T->halo = up(halo.data(), halo.size() * 4);
-    T->cj_ptr = up(cj_ptr.data(), cj_ptr.size() * 4);
-    T->cj = up(cj.data(), cj.size() * 4);
-    T->hnode_ptr = up(hnode_ptr.data(), hnode_ptr.size() * 4);
-    T->hnodes = up(hnodes.data(), hnodes.size() * 4);
-    T->lconn = up(lconn.data(), lconn.size() * 2);
-    (void)ctx;
+    FEASM_REQUIRE(nX < INT32_MAX, FEASM_EUNSUPPORTED, "mesh too large for 32-bit tile offsets");
+    T->blob_bytes = nB;
+    std::vector<unsigned char> blob(std::max<int64_t>(nB, 16), 0);
+    std::vector<int32_t> hn(std::max<int64_t>(nX, 1));
+    for (int64_t t = 0; t < ntiles; t++) {
+        const TileOut &o = outs[t];
+        const TileHdr &d = hdr[t];
+        const TileShape::Layout L = shape.layout(d.H, d.P, d.NH, d.K, d.NT, d.NI);
+        unsigned char *b = blob.data() + d.blob;
+        auto put = [&](size_t off, const void *src, size_t bytes) {
+            if (bytes) std::memcpy(b + (off - L.cbase), src, bytes);
+        };
+        put(L.cbase, o.cbase.data(), o.cbase.size() * 8);
+        put(L.cdeg, o.cdeg.data(), o.cdeg.size() * 4);
+        put(L.slot, o.slot.data(), o.slot.size() * 4);
+        put(L.task, o.task.data(), o.task.size() * 4);
+        put(L.item, o.item.data(), o.item.size() * 2);
+        put(L.lconn, o.lconn.data(), o.lconn.size() * 2);
+        std::copy(o.hnodes.begin(), o.hnodes.end(), hn.begin() + d.x0);
+    }
+    T->blob = dev_alloc(blob.size());
+    FEASM_HIP(hipMemcpy(T->blob->p, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    T->hnodes = dev_alloc(hn.size() * 4);
+    FEASM_HIP(hipMemcpy(T->hnodes->p, hn.data(), hn.size() * 4, hipMemcpyHostToDevice));
+    T->hdr = dev_alloc(std::max<size_t>(hdr.size() * sizeof(TileHdr), 16));
+    FEASM_HIP(hipMemcpy(T->hdr->p, hdr.data(), hdr.size() * sizeof(TileHdr), hipMemcpyHostToDevice));
     return T;
 }
 
