@@ -139,16 +139,15 @@ __device__ __forceinline__ void elem_block(uint32_t it, const R *__restrict__ sU
 // element, point after point); in mixed precision each element's matrix is
 // formed in FP32 and added in FP64.
 template <int NPE, int NQP, int FORM, typename R, bool SIGNED, int SWP, int ILP>
-__device__ __forceinline__ void block_sum(int t, int k, const int32_t *__restrict__ sSlot,
-                                          const uint16_t *__restrict__ sItem, const R *__restrict__ sU,
+__device__ __forceinline__ void block_sum(int k, const uint16_t *__restrict__ items, const R *__restrict__ sU,
                                           const uint8_t *__restrict__ sSg, const double *__restrict__ sN,
                                           const R *__restrict__ sW, double *acc) {
     if constexpr (sizeof(R) == 8) {
-        for (int j = 0; j < k; j++) elem_block<NPE, NQP, FORM, R, SIGNED>(sItem[sSlot[j] + t], sU, sSg, sN, sW, acc);
+        for (int j = 0; j < k; j++) elem_block<NPE, NQP, FORM, R, SIGNED>(items[j], sU, sSg, sN, sW, acc);
     } else {
         int j = 0;
         for (; ILP == 2 && j + 1 < k; j += 2) {
-            const uint32_t i0 = sItem[sSlot[j] + t], i1 = sItem[sSlot[j + 1] + t];
+            const uint32_t i0 = items[j], i1 = items[j + 1];
             R M0[SWP], M1[SWP];
 #pragma unroll
             for (int i = 0; i < SWP; i++) M0[i] = M1[i] = (R)0;
@@ -161,7 +160,7 @@ __device__ __forceinline__ void block_sum(int t, int k, const int32_t *__restric
             R M0[SWP];
 #pragma unroll
             for (int i = 0; i < SWP; i++) M0[i] = (R)0;
-            elem_block<NPE, NQP, FORM, R, SIGNED>(sItem[sSlot[j] + t], sU, sSg, sN, sW, M0);
+            elem_block<NPE, NQP, FORM, R, SIGNED>(items[j], sU, sSg, sN, sW, M0);
 #pragma unroll
             for (int i = 0; i < SWP; i++) acc[i] = acc[i] + (double)M0[i];
         }
@@ -246,7 +245,8 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
     uint16_t *sLc = reinterpret_cast<uint16_t *>(smem + L.lconn);
     int64_t *sCb = reinterpret_cast<int64_t *>(smem + L.cbase);
     int32_t *sCd = reinterpret_cast<int32_t *>(smem + L.cdeg);
-    int32_t *sSlot = reinterpret_cast<int32_t *>(smem + L.slot);
+    uint16_t *sToff = reinterpret_cast<uint16_t *>(smem + L.toff);
+    uint16_t *sPair = reinterpret_cast<uint16_t *>(smem + L.tpair);
     R *sU = reinterpret_cast<R *>(smem + L.U);
     uint8_t *sSg = reinterpret_cast<uint8_t *>(smem + L.sgn);
     R *sW = reinterpret_cast<R *>(smem + L.sgn);
@@ -318,15 +318,16 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
     double *__restrict__ nzval = job.nzval;
     for (int t = tid; t < NT; t += kThreads) {
         const uint32_t th = sTask[t];
+        if (th == kNullTask) continue;
         const int c = (int)(th & 255u), r = (int)((th >> 8) & 255u), k = (int)((th >> 16) & 255u);
         const bool sw = (th >> 24) & 1u, diag = (th >> 25) & 1u;
         double acc[SWP];
 #pragma unroll
         for (int i = 0; i < SWP; i++) acc[i] = 0.0;
         if (tile_neg)
-            block_sum<NPE, NQP, FORM, R, true, SWP, ILP>(t, k, sSlot, sItem, sU, sSg, sN, sW, acc);
+            block_sum<NPE, NQP, FORM, R, true, SWP, ILP>(k, sItem + sToff[t], sU, sSg, sN, sW, acc);
         else
-            block_sum<NPE, NQP, FORM, R, false, SWP, ILP>(t, k, sSlot, sItem, sU, sSg, sN, sW, acc);
+            block_sum<NPE, NQP, FORM, R, false, SWP, ILP>(k, sItem + sToff[t], sU, sSg, sN, sW, acc);
         double out[NDN * NDN];
         if constexpr (SWP == 9) {
             // contract in the orientation of the lower node number
@@ -395,6 +396,20 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
         for (int l = 0; l < NDN; l++)
 #pragma unroll
             for (int a = 0; a < NDN; a++) nzval[base + (int64_t)l * NDN * deg + a] = out[NDN * a + l];
+        const uint32_t pr = sPair[t];
+        if (pr != kNoPair) {  // the transposed block (row c, column m), in its own column
+            const int c2 = (int)(pr & 255u), r2 = (int)(pr >> 8);
+            const int deg2 = sCd[c2];
+            const int64_t base2 = (int64_t)NDN * NDN * sCb[c2] + (int64_t)r2 * NDN;
+            // K_cm = K_mc^T; the DOT forms scale a scalar by the same matrix
+            // in both blocks
+#pragma unroll
+            for (int l = 0; l < NDN; l++)
+#pragma unroll
+                for (int a = 0; a < NDN; a++)
+                    nzval[base2 + (int64_t)l * NDN * deg2 + a] =
+                        (FORM == FEASM_FORM_DOT) ? out[NDN * a + l] : out[NDN * l + a];
+        }
     }
     __syncthreads();  // LDS is reused by the workgroup's next tile
 }

@@ -22,7 +22,7 @@ struct TileShape {
     __host__ __device__ int ustride() const { return nqp * npe * 3 + 1; }
 
     struct Layout {
-        size_t tab, X, lconn, cbase, cdeg, slot, task, item, U, sgn, total;
+        size_t tab, X, lconn, cbase, cdeg, task, toff, tpair, item, U, sgn, total;
     };
     // doubles between the gradient-table rows of consecutive points: npe*3
     // plus one, for the same reason
@@ -41,10 +41,13 @@ struct TileShape {
         o = align16(o + (size_t)P * 8);
         L.cdeg = o;
         o = align16(o + (size_t)P * 4);
-        L.slot = o;
-        o = align16(o + (size_t)(K + 1) * 4);
+        (void)K;
         L.task = o;
         o = align16(o + (size_t)NT * 4);
+        L.toff = o;
+        o = align16(o + (size_t)NT * 2);
+        L.tpair = o;
+        o = align16(o + (size_t)NT * 2);
         L.item = o;
         o = align16(o + (size_t)NI * 2);
         L.lconn = o;
@@ -79,12 +82,20 @@ struct TileHdr {
 };
 
 // Blob contents (see TileShape::layout): int64 cbase[P] (first node block of
-// each owned column), int32 cdeg[P] (its node blocks), int32 slot[K+1] (items
-// of slot j for task t at slot[j] + t), uint32 task[NT] (c | r << 8 | k << 16 |
-// (row > column) << 24 | (row == column) << 25, sorted by k descending),
-// uint16 item[NI] (h << 8 | index of the column node in e << 4 | index of the
-// row node), uint16 lconn[H*npe] (halo connectivity as indices into the
-// tile's hnodes).
+// each owned column), int32 cdeg[P] (its node blocks), uint32 task[NT] (c |
+// r << 8 | k << 16 | (row > column) << 24 | (row == column) << 25, or
+// kNullTask), uint16 toff[NT] (first item of the task), uint16 tpair[NT] (c2 |
+// r2 << 8: the task also writes the transposed block, row r2 of owned
+// column c2; kNoPair when not), uint16 item[NI]
+// (h << 8 | index of the column node in e << 4 | index of the row node; a
+// task's k items in ascending element), uint16 lconn[H*npe] (halo
+// connectivity as indices into the tile's hnodes).
+constexpr uint32_t kNullTask = 0xffffffffu;
+constexpr uint16_t kNoPair = 0xffff;
+// null tasks a tile's wavefront schedule may add (the tile sizing reserves
+// room for them)
+constexpr int kPadTasks = 320;
+
 struct TileSet {
     TileShape shape;
     int64_t ntiles = 0, nhalo = 0, ntasks = 0, nitems = 0, blob_bytes = 0;

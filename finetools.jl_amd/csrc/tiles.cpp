@@ -12,8 +12,8 @@
 // duplicates of its element-major COO, src/AssemblyModule.jl:319-325), each
 // with the local indices of the two nodes in the element.  Tasks are sorted
 // by their element count, so that the lanes of one wavefront run loops of the
-// same length, and their items are stored slot-major, so that lane t reads
-// item j of its task at slot[j] + t: one coalesced load per slot.
+// same length, and the wavefronts are dealt to the SIMDs so that each gets a
+// similar share of the work.
 //
 // Tiles are grown along the Morton order until the next node would overflow
 // the LDS budget; the cut is then moved back to the coarsest Morton boundary
@@ -90,9 +90,8 @@ struct TileOut {
     int P = 0, H = 0, NH = 0, K = 0;
     std::vector<int64_t> cbase;
     std::vector<int32_t> cdeg, hnodes;
-    std::vector<uint16_t> lconn, item;
+    std::vector<uint16_t> lconn, item, toff, tpair;
     std::vector<uint32_t> task;
-    std::vector<int32_t> slot;
 };
 
 void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
@@ -124,8 +123,11 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
         int c, r;
         bool sw, diag;
         std::vector<uint16_t> it;
+        int c2 = -1, r2 = 0;  // the transposed block, when its column is owned too
+        bool drop = false;    // computed by its transposed block's task
     };
     std::vector<Task> tasks;
+    std::vector<size_t> tfirst(P);
     o.cbase.resize(P);
     o.cdeg.resize(P);
     for (int c = 0; c < P; c++) {
@@ -136,6 +138,7 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
         o.cdeg[c] = (int32_t)deg;
         const int32_t *rows = h.adj.data() + b0;
         const size_t tb = tasks.size();
+        tfirst[c] = tb;
         for (int r = 0; r < deg; r++) tasks.push_back(Task{c, r, rows[r] > node, rows[r] == node, {}});
         for (int64_t s = h.n2e_ptr[node]; s < h.n2e_ptr[node + 1]; s++) {
             const int32_t e = (int32_t)(h.n2e[s] >> 4);
@@ -149,24 +152,88 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
             }
         }
     }
+    // A block whose row node is owned too has its transpose in this tile:
+    // one task computes both (the element lists are the same, and so are the
+    // products, in the same order).
+    {
+        std::vector<std::pair<int32_t, int>> own(P);
+        for (int c = 0; c < P; c++) own[c] = {owned[c], c};
+        std::sort(own.begin(), own.end());
+        for (Task &T : tasks) {
+            if (T.diag || T.drop) continue;
+            const int32_t m = h.adj[o.cbase[T.c] + T.r];
+            auto f = std::lower_bound(own.begin(), own.end(), std::make_pair(m, -1));
+            if (f == own.end() || f->first != m || f->second < T.c) continue;
+            const int c2 = f->second;
+            const int32_t *rows2 = h.adj.data() + o.cbase[c2];
+            const int r2 = (int)(std::lower_bound(rows2, rows2 + o.cdeg[c2], owned[T.c]) - rows2);
+            FEASM_REQUIRE(r2 < o.cdeg[c2] && rows2[r2] == owned[T.c], FEASM_ESTATE, "pattern is not symmetric");
+            T.c2 = c2;
+            T.r2 = r2;
+            tasks[tfirst[c2] + r2].drop = true;
+        }
+        tasks.erase(std::remove_if(tasks.begin(), tasks.end(), [](const Task &T) { return T.drop; }), tasks.end());
+    }
     std::stable_sort(tasks.begin(), tasks.end(),
                      [](const Task &x, const Task &y) { return x.it.size() > y.it.size(); });
     const int K = tasks.empty() ? 0 : (int)tasks.front().it.size();
     FEASM_REQUIRE(K <= 255, FEASM_EUNSUPPORTED, "a node block is shared by more than 255 elements");
     o.K = K;
-    o.task.resize(tasks.size());
-    for (size_t t = 0; t < tasks.size(); t++) {
-        const Task &T = tasks[t];
-        o.task[t] = (uint32_t)T.c | ((uint32_t)T.r << 8) | ((uint32_t)T.it.size() << 16) |
-                    ((uint32_t)T.sw << 24) | ((uint32_t)T.diag << 25);
+    // Wavefront schedule: 64 consecutive tasks run as one wavefront, and the
+    // wavefronts of a workgroup go to the 4 SIMDs in turn.  Chunks of 64
+    // tasks of equal element count (costed by that count plus the material
+    // and store work) are dealt to the SIMDs longest first, each to the least
+    // loaded one; chunk slots a SIMD does not use are filled with null tasks.
+    constexpr int W = 64, NS = 4;
+    std::vector<std::vector<int>> queue(NS);
+    {
+        std::vector<double> load(NS, 0.0);
+        for (size_t c0 = 0; c0 < tasks.size(); c0 += W) {
+            const double cost = (double)tasks[c0].it.size() + 2.0;
+            const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            queue[q].push_back((int)c0);
+            load[q] += cost;
+        }
     }
-    // slot-major items: slot j holds item j of every task with more than j items
-    o.slot.assign(K + 1, 0);
-    for (int j = 0; j < K; j++) {
-        int nj = 0;
-        while (nj < (int)tasks.size() && (int)tasks[nj].it.size() > j) nj++;
-        o.slot[j + 1] = o.slot[j] + nj;
-        for (int t = 0; t < nj; t++) o.item.push_back(tasks[t].it[j]);
+    size_t rounds = 0;
+    for (auto &q : queue) rounds = std::max(rounds, q.size());
+    if (rounds * NS * W > tasks.size() + kPadTasks) {  // too much padding: plain order
+        for (auto &q : queue) q.clear();
+        int i = 0;
+        for (size_t c0 = 0; c0 < tasks.size(); c0 += W, i++) queue[i % NS].push_back((int)c0);
+        rounds = 0;
+        for (auto &q : queue) rounds = std::max(rounds, q.size());
+    }
+    o.task.clear();
+    o.toff.clear();
+    o.tpair.clear();
+    o.item.clear();
+    for (size_t rr = 0; rr < rounds; rr++)
+        for (int q = 0; q < NS; q++) {
+            const int c0 = rr < queue[q].size() ? queue[q][rr] : -1;
+            for (int l = 0; l < W; l++) {
+                const size_t t = (size_t)c0 + l;
+                if (c0 < 0 || t >= tasks.size()) {
+                    o.task.push_back(kNullTask);
+                    o.toff.push_back(0);
+                    o.tpair.push_back(kNoPair);
+                    continue;
+                }
+                const Task &T = tasks[t];
+                FEASM_REQUIRE(o.item.size() + T.it.size() <= 65535, FEASM_EUNSUPPORTED,
+                              "tile holds too many (block, element) items");
+                o.task.push_back((uint32_t)T.c | ((uint32_t)T.r << 8) | ((uint32_t)T.it.size() << 16) |
+                                 ((uint32_t)T.sw << 24) | ((uint32_t)T.diag << 25));
+                o.toff.push_back((uint16_t)o.item.size());
+                o.tpair.push_back(T.c2 < 0 ? kNoPair : (uint16_t)(T.c2 | (T.r2 << 8)));
+                o.item.insert(o.item.end(), T.it.begin(), T.it.end());
+            }
+        }
+    // trailing null tasks are not needed
+    while (!o.task.empty() && o.task.back() == kNullTask) {
+        o.task.pop_back();
+        o.toff.pop_back();
+        o.tpair.pop_back();
     }
 }
 
@@ -256,7 +323,7 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape) {
             const int32_t n = key[i].second;
             int64_t h2, nh2, k2;
             probe(n, h2, nh2, k2);
-            const int64_t nt2 = NT + (h.adj_ptr[n + 1] - h.adj_ptr[n]);
+            const int64_t nt2 = NT + (h.adj_ptr[n + 1] - h.adj_ptr[n]) + kPadTasks;
             const int64_t ni2 = NI + (h.n2e_ptr[n + 1] - h.n2e_ptr[n]) * npe;
             if (!fits(i - s + 1, h2, nh2, k2, nt2, ni2)) break;
             add_node(key[i].second);
@@ -344,7 +411,8 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape) {
         };
         put(L.cbase, o.cbase.data(), o.cbase.size() * 8);
         put(L.cdeg, o.cdeg.data(), o.cdeg.size() * 4);
-        put(L.slot, o.slot.data(), o.slot.size() * 4);
+        put(L.toff, o.toff.data(), o.toff.size() * 2);
+        put(L.tpair, o.tpair.data(), o.tpair.size() * 2);
         put(L.task, o.task.data(), o.task.size() * 4);
         put(L.item, o.item.data(), o.item.size() * 2);
         put(L.lconn, o.lconn.data(), o.lconn.size() * 2);
