@@ -67,6 +67,22 @@ def cpu_baseline(n_cells, seed, D, min_seconds=10.0, max_runs=5):
                       "(%.2f s each)" % (n_cells, ne, len(times), t)}
 
 
+PROFILE = os.path.join(ROOT, "profiles", "k_fused_h8_elasticity_100cube.md")
+
+
+def profiled_traffic(path=PROFILE):
+    """HBM bytes per launch of the fused kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a
+    streaming copy); None when the summary is absent."""
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def rel_fro(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
@@ -357,7 +373,8 @@ def main():
         "kernels_ms": {"fused": el_ms} if ntiles > 0 else {"element": el_ms, "gather": ga_ms},
         "tiles": ntiles,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic() if (ntiles > 0 and n == 100) else None,
                      "kernel": "k_fused (fused tile kernel, whole numeric phase)" if ntiles > 0
                      else "k_element + k_gather (staged numeric phase)",
                      "algorithmic_bytes": bytes_alg},
