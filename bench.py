@@ -235,14 +235,14 @@ def bench_served(feasm, core, rate=200.0, duration=3.0, seed=5, pool=48, workers
                                                                    float(rng.uniform(0.2, 0.45)))
         stream.append((arrivals[k], int(rng.integers(0, pool)), form, coeff, rng.random() < 0.3))
 
-    def run(precision, workers_):
+    def run(precision, workers_, paced=True):
         srv = core.Server(0, workers=workers_, hbm_fraction=0.6)
         resident = {}  # (pool index, ndn) -> mesh id
         pending = []
         ne_done = 0
         t_start = time.perf_counter()
         for (t_arr, pi, form, coeff, re) in stream:
-            while time.perf_counter() - t_start < t_arr:
+            while paced and time.perf_counter() - t_start < t_arr:
                 time.sleep(0.0002)
             et, conn, xyz, rule = meshes[pi]
             ndn = 1 if form == feasm.DIFFUSION_ISO else 3
@@ -273,10 +273,17 @@ def bench_served(feasm, core, rate=200.0, duration=3.0, seed=5, pool=48, workers
     log("served: %s" % served)
     plain = run(feasm.FP64, 1)
     log("plain: %s" % plain)
+    # the same request stream submitted at once: throughput at saturation
+    served_sat = run(feasm.MIXED, workers, paced=False)
+    log("served, saturated: %s" % served_sat)
+    plain_sat = run(feasm.FP64, 1, paced=False)
+    log("plain, saturated: %s" % plain_sat)
     return {"workload": "Poisson %.0f req/s for %.1f s, H8/T10 50/50, median 5e4 elements (sigma 1.0), 30%% "
                         "re-assembly of resident meshes" % (rate, duration),
             "served_mixed": served, "plain_fp64_sequential": plain,
-            "throughput_ratio": served["elements_per_s"] / plain["elements_per_s"]}
+            "throughput_ratio": served["elements_per_s"] / plain["elements_per_s"],
+            "saturated": {"served_mixed": served_sat, "plain_fp64_sequential": plain_sat,
+                          "throughput_ratio": served_sat["elements_per_s"] / plain_sat["elements_per_s"]}}
 
 
 def main():

@@ -10,7 +10,9 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <atomic>
 #include <fstream>
+#include <thread>
 #include <sstream>
 
 #include "common.hpp"
@@ -94,9 +96,49 @@ void prepare(feasm_batch *b) {
     for (auto &it : b->items) {
         feasm_mesh *m = it.mesh;
         if (!m->pattern || m->pattern->ndn != m->ndn) m->pattern = build_pattern(m, m->ndn);
+    }
+    // tiles of the meshes that lack them, built concurrently (one host thread
+    // per mesh at a time)
+    {
+        std::vector<std::pair<feasm_mesh *, TileShape>> todo;
+        for (auto &it : b->items) {
+            const TileShape s = shape_for(b, it);
+            bool dup = false;
+            for (auto &t : todo) dup |= (t.first == it.mesh && t.second.key() == s.key());
+            if (!dup && !it.mesh->tiles.count(s.key())) todo.emplace_back(it.mesh, s);
+        }
+        std::vector<std::shared_ptr<TileSet>> built(todo.size());
+        std::vector<std::string> errs(todo.size());
+        std::vector<int> codes(todo.size(), 0);
+        std::atomic<size_t> next{0};
+        const int nth = (int)std::max<size_t>(1, std::min<size_t>(todo.size(), std::max(1u, std::thread::hardware_concurrency())));
+        std::vector<std::thread> pool;
+        for (int w = 0; w < nth; w++)
+            pool.emplace_back([&]() {
+                const bool dev_ok = hipSetDevice(ctx->device) == hipSuccess;
+                for (size_t i; (i = next++) < todo.size();) {
+                    try {
+                        FEASM_REQUIRE(dev_ok, FEASM_EHIP, "hipSetDevice failed in a tile-building thread");
+                        built[i] = build_tiles(todo[i].first, todo[i].second, 1);
+                    } catch (const Error &e) {
+                        codes[i] = e.code;
+                        errs[i] = e.what();
+                    } catch (const std::exception &e) {
+                        codes[i] = FEASM_ENOMEM;
+                        errs[i] = e.what();
+                    }
+                }
+            });
+        for (auto &th : pool) th.join();
+        for (size_t i = 0; i < todo.size(); i++) {
+            if (codes[i]) throw Error(codes[i], errs[i]);
+            todo[i].first->tiles.emplace(todo[i].second.key(), built[i]);
+        }
+    }
+    for (auto &it : b->items) {
+        feasm_mesh *m = it.mesh;
         const TileShape s = shape_for(b, it);
         auto tt = m->tiles.find(s.key());
-        if (tt == m->tiles.end()) tt = m->tiles.emplace(s.key(), build_tiles(m, s)).first;
         if (!it.mat) {
             it.mat.reset(new feasm_matrix);
             it.mat->ctx = ctx;

@@ -168,6 +168,7 @@ struct feasm_mesh {
     // fused kernel), 1 = staged element blocks + ordered gather, 2 = fused
     int numeric_path = 0;
     int tile_nodes = 0, tile_kib = 0, tile_threads = 0;  // fused-kernel tile parameters (0 = default)
+    int64_t assemblies = 0;  // numeric assemblies run by the assembly server on this mesh
     std::map<int64_t, std::shared_ptr<feasm::TileSet>> tiles;  // keyed by TileShape::key()
 };
 

@@ -232,7 +232,7 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
     const int P = hd.P, H = hd.H, NH = hd.NH, NT = hd.NT, K = hd.K, NI = hd.NI;
     constexpr int US = NQP * NPE * 3 + 1;  // TileShape::ustride()
     // two elements in flight where the registers allow it under 1024 threads
-    constexpr int ILP = (SWP == 9) ? 1 : 2;
+    constexpr int ILP = 2;
     constexpr int DS = NPE * 3 + 1;  // TileShape::dstride()
     const TileShape::Layout L = shape.layout(H, P, NH, K, NT, NI);
     double *sN = reinterpret_cast<double *>(smem + L.tab);

@@ -105,7 +105,8 @@ struct TileSet {
     DevPtr hnodes;  // int32 distinct nodes of each tile's halo, ascending
 };
 
-std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape);
+// max_threads bounds the host threads building the per-tile arrays
+std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int max_threads = 16);
 
 // one mesh of a batched launch
 struct BatchMesh {

@@ -66,14 +66,23 @@ struct Request {
     std::unique_ptr<feasm_matrix> result;
 };
 
-size_t estimate_bytes(int npe, int ndn, int64_t nelem, int64_t nnodes) {
+// Device bytes a request may hold: pattern and values always, plus the staged
+// element blocks and contribution lists (every mesh's first assemblies) or,
+// for a resident mesh re-assembled often enough, the fused kernel's tiles.
+constexpr int kTileAfter = 4;
+
+size_t estimate_bytes(int npe, int ndn, int64_t nelem, int64_t nnodes, bool keep, int precision) {
     const double deg = (npe == 8) ? 27.0 : 26.0;  // upper bound of the mean node degree of block meshes
     const double nb = deg * (double)nnodes, nnz = nb * ndn * ndn;
-    const double b = 4.0 * npe * nelem * 3  // connectivity, node->element map, tiles
-                     + 48.0 * nnodes        // coordinates, pointers, DOFs
-                     + 8.0 * nb             // adjacency, block columns
-                     + 12.0 * nnz           // rowval + nzval
-                     + 8.0 * nnodes * ndn;  // colptr
+    const double npairs = npe * (npe + 1) / 2.0;
+    const double rb = (precision == FEASM_MIXED) ? 4.0 : 8.0;
+    double b = 4.0 * npe * nelem * 3  // connectivity, node->element map
+               + 48.0 * nnodes        // coordinates, pointers, DOFs
+               + 8.0 * nb             // adjacency, block columns
+               + 12.0 * nnz           // rowval + nzval
+               + 8.0 * nnodes * ndn;  // colptr
+    const double staged = npairs * 9.0 * rb * nelem + 4.0 * npe * npe * nelem + 8.0 * nb;  // stage, contributions
+    b += keep ? std::max(staged, 600.0 * nnodes) : staged;
     return (size_t)(1.2 * b) + (8u << 20);
 }
 
@@ -214,6 +223,17 @@ struct feasm_server {
             M->pattern = m->pattern;
             M->form_class = m->ndn;
             M->nzval = dev_alloc(std::max<size_t>(M->nnz * 8, 8));
+            // The fused kernel needs tiles built on the host, which pays off
+            // only over repeated assemblies of one mesh: a mesh's first
+            // kTileAfter assemblies take the staged path (all on the device);
+            // later ones build the tiles once and drop the staged buffers.
+            m->numeric_path = (m->assemblies < kTileAfter) ? 1 : 0;
+            if (m->assemblies == kTileAfter) {
+                m->stage.reset();
+                m->pattern->cptr.reset();
+                m->pattern->contrib.reset();
+            }
+            m->assemblies++;
             numeric_into_ctx(ctx, m, r->fp, r->rule, r->order, r->precision, dp<double>(M->nzval));
             FEASM_HIP(hipStreamSynchronize(ctx->stream));
             std::lock_guard<std::mutex> lk(mu);
@@ -344,7 +364,7 @@ extern "C" int feasm_server_submit_mesh(feasm_server *s, int elem_type, int64_t 
     r->order = order;
     r->precision = precision;
     r->keep = keep_resident != 0;
-    r->est = estimate_bytes(npe, ndn, nelem, nnodes);
+    r->est = estimate_bytes(npe, ndn, nelem, nnodes, keep_resident != 0, precision);
     {
         std::lock_guard<std::mutex> lk(s->mu);
         r->mesh_id = s->next_mesh++;

@@ -239,7 +239,7 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
 
 }  // namespace
 
-std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape) {
+std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int max_threads) {
     const Pattern &p = *m->pattern;
     HostMesh h;
     h.npe = m->npe;
@@ -285,21 +285,23 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape) {
             }
         }
     };
-    // what the tile would hold with node n added (state left unchanged)
-    std::vector<int32_t> fresh_nodes;
+    // what the tile would hold with node n added (tile state left unchanged;
+    // pstamp marks the nodes this probe has counted)
+    std::vector<int64_t> pstamp(nn, -1);
+    int64_t pepoch = 0;
     auto probe = [&](int32_t n, int64_t &h2, int64_t &nh2, int64_t &k2) {
         h2 = H;
         nh2 = NH;
         k2 = std::max(K, h.n2e_ptr[n + 1] - h.n2e_ptr[n]);
-        fresh_nodes.clear();
+        pepoch++;
         for (int64_t s = h.n2e_ptr[n]; s < h.n2e_ptr[n + 1]; s++) {
             const int32_t e = (int32_t)(h.n2e[s] >> 4);
             if (estamp[e] == epoch) continue;
             h2++;
             for (int a = 0; a < npe; a++) {
                 const int32_t v = h.conn[(int64_t)e * npe + a];
-                if (nstamp[v] != epoch && std::find(fresh_nodes.begin(), fresh_nodes.end(), v) == fresh_nodes.end()) {
-                    fresh_nodes.push_back(v);
+                if (nstamp[v] != epoch && pstamp[v] != pepoch) {
+                    pstamp[v] = pepoch;
                     nh2++;
                 }
             }
@@ -350,8 +352,9 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape) {
     std::vector<int32_t> order(nn);
     for (int64_t i = 0; i < nn; i++) order[i] = key[i].second;
     std::vector<TileOut> outs(ntiles);
-    const int nthreads = (int)std::max<int64_t>(1, std::min<int64_t>(16, std::min<int64_t>(
-                                                          std::thread::hardware_concurrency(), ntiles / 64 + 1)));
+    const int nthreads = (int)std::max<int64_t>(
+        1, std::min<int64_t>(std::max(1, max_threads),
+                             std::min<int64_t>(std::thread::hardware_concurrency(), ntiles / 64 + 1)));
     std::vector<std::thread> pool;
     std::vector<std::string> errs(nthreads);
     std::vector<int> codes(nthreads, 0);
@@ -362,6 +365,9 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape) {
                     make_tile(h, order.data() + starts[t], (int)(starts[t + 1] - starts[t]), outs[t]);
             } catch (const Error &e) {
                 codes[w] = e.code;
+                errs[w] = e.what();
+            } catch (const std::exception &e) {
+                codes[w] = FEASM_ENOMEM;
                 errs[w] = e.what();
             }
         });
