@@ -87,6 +87,34 @@ def rel_fro(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def bench_vec(feasm, core, n=100, steps=10):
+    """Load vector (linform_dot, m = 3, + makevector!) of a constant body force
+    on the C2 mesh (H8 100^3, 3 DOF/node): device kernel time per assembly and
+    its algorithmic bytes (int32 connectivity and FP64 coordinates read once,
+    FP64 vector written once)."""
+    ctx = feasm.Context(0)
+    conn, xyz = core.gen_h8block(1.0, 1.0, 1.0, n, n, n)
+    xyz = core.gen_perturb(xyz, [0, 0, 0], [1, 1, 1], 0.15 / n, 2)
+    ne, nn = conn.shape[0], xyz.shape[0]
+    mesh = core.DeviceMesh(ctx, feasm.H8, conn, xyz)
+    mesh.set_dofs(3)
+    f = [0.0, -9.81, 2.0]
+    Fv = mesh.linform_dot(feasm.RULE_GAUSS, 2, f, nn * 3)
+    ctx.set_timing(steps)
+    for _ in range(steps):
+        mesh.linform_dot(feasm.RULE_GAUSS, 2, f, nn * 3)
+    ctx.synchronize()
+    ms = float(np.mean(ctx.timings()[:, 0]))
+    ctx.set_timing(0)
+    bytes_alg = 4 * 8 * ne + 24 * nn + 8 * 3 * nn
+    out = {"workload": "H8 %d^3 body load, 3 DOF/node, 2x2x2 Gauss" % n, "kernels_ms": ms,
+           "elements_per_s": ne / (ms * 1e-3), "achieved_GBps": bytes_alg / (ms * 1e-3) / 1e9,
+           "algorithmic_bytes": bytes_alg, "total_load": [float(v) for v in Fv.reshape(-1, 3).sum(axis=0)]}
+    mesh.close()
+    ctx.close()
+    return out
+
+
 def bench_t10(feasm, core, n=60, steps=5):
     """C3: T10 n^3 x 6 tets, E=70e3 nu=0.33, rho=2.7e-9: FP64 and mixed stiffness
     plus the consistent mass; error of the mixed results against FP64."""
@@ -294,7 +322,7 @@ def main():
     ap.add_argument("--cells", type=int, default=100, help="cells per direction of the H8 block")
     ap.add_argument("--cpu-cells", type=int, default=50, help="cells per direction of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extras", default="", help="comma list of extra workloads: t10,batch,served")
+    ap.add_argument("--extras", default="", help="comma list of extra workloads: vec,t10,batch,served")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB streaming copy (3 launches) to scale traffic counters")
     a = ap.parse_args()
@@ -393,6 +421,9 @@ def main():
     mesh.close()
     ctx.close()
     extras = [x for x in a.extras.split(",") if x]
+    if "vec" in extras:
+        log("load vector ...")
+        out["vector"] = bench_vec(feasm, core)
     if "t10" in extras:
         log("T10 ...")
         out["t10"] = bench_t10(feasm, core)

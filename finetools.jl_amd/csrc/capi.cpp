@@ -385,6 +385,13 @@ static void set_dofs(feasm_mesh *m, int ndn, const std::vector<int64_t> &rowmajo
     m->ndn = ndn;
     m->nalldofs = nalldofs;
     m->nodemajor = nm;
+    std::vector<bool> seen(std::max<int64_t>(nalldofs, 0), false);
+    m->dofs_unique = true;
+    for (int32_t d : d0)
+        if (d >= 0 && d < nalldofs) {
+            if (seen[d]) m->dofs_unique = false;
+            seen[d] = true;
+        }
 }
 
 extern "C" int feasm_dofs_set(feasm_mesh *m, int ndn, const int64_t *dofnums_colmajor, int64_t nalldofs) {
@@ -412,6 +419,48 @@ extern "C" int feasm_dofs_set_nodemajor(feasm_mesh *m, int ndn) {
     std::vector<int64_t> rm(m->nnodes * ndn);
     for (int64_t i = 0; i < (int64_t)rm.size(); i++) rm[i] = i + 1;
     set_dofs(m, ndn, rm, m->nnodes * ndn, true);
+    API_CATCH(m ? m->ctx : nullptr)
+}
+
+// ======================================================================= load vectors
+namespace {
+
+void linform_into(feasm_mesh *m, int rule, int order, const double *force, int nforce, double *F) {
+    FEASM_REQUIRE(m && force && F, FEASM_EINVAL, "null argument");
+    FEASM_REQUIRE(m->ndn > 0, FEASM_ESTATE, "set the DOF numbering first");
+    FEASM_REQUIRE(nforce == m->ndn, FEASM_EINVAL, "the force needs one value per DOF of a node");
+    FEASM_REQUIRE(m->dofs_unique, FEASM_EUNSUPPORTED, "load vectors need distinct DOF numbers per (node, component)");
+    check_rule(m, rule);
+    feasm_ctx *ctx = m->ctx;
+    if (!m->n2e_ptr) build_node_to_element(m);  // the gather walks each node's elements
+    const DevQTable &qt = device_qtable(ctx, m->etype, rule, order);
+    double *stage = static_cast<double *>(ctx->scratch_get((size_t)m->nelem * m->npe * m->ndn * sizeof(double)));
+    Timer tm(ctx);
+    tm.mark(0);
+    launch_linform_dot(ctx, m, qt, force, stage, F);
+    tm.mark(1);
+    tm.mark(2);
+    tm.done();
+}
+
+}  // namespace
+
+extern "C" int feasm_linform_dot_device(feasm_mesh *m, int rule, int order, const double *force, int nforce,
+                                        double *F_device) {
+    API_TRY
+    linform_into(m, rule, order, force, nforce, F_device);
+    API_CATCH(m ? m->ctx : nullptr)
+}
+
+extern "C" int feasm_linform_dot(feasm_mesh *m, int rule, int order, const double *force, int nforce,
+                                 double *F_host) {
+    API_TRY
+    FEASM_REQUIRE(m && F_host, FEASM_EINVAL, "null argument");
+    DevPtr F = dev_alloc(std::max<size_t>((size_t)m->nalldofs * sizeof(double), 8));
+    linform_into(m, rule, order, force, nforce, dp<double>(F));
+    FEASM_HIP(hipMemcpyAsync(F_host, F->p, (size_t)m->nalldofs * sizeof(double), hipMemcpyDeviceToHost,
+                             m->ctx->stream));
+    FEASM_HIP(hipStreamSynchronize(m->ctx->stream));
     API_CATCH(m ? m->ctx : nullptr)
 }
 

@@ -169,6 +169,7 @@ struct feasm_mesh {
     int numeric_path = 0;
     int tile_nodes = 0, tile_kib = 0, tile_threads = 0;  // fused-kernel tile parameters (0 = default)
     int64_t assemblies = 0;  // numeric assemblies run by the assembly server on this mesh
+    bool dofs_unique = true;  // no DOF number is shared by two (node, component) pairs
     std::map<int64_t, std::shared_ptr<feasm::TileSet>> tiles;  // keyed by TileShape::key()
 };
 
@@ -186,6 +187,11 @@ namespace feasm {
 
 // tables.cpp
 const DevQTable &device_qtable(feasm_ctx *ctx, int etype, int rule, int order);
+
+// vector.hip: load vector of a constant force (ndn values) into F (nalldofs,
+// device); stage holds nelem*npe*ndn doubles
+void launch_linform_dot(feasm_ctx *ctx, const feasm_mesh *m, const DevQTable &qt, const double *force,
+                        double *stage, double *F);
 
 // pattern.hip
 void build_node_to_element(feasm_mesh *m);

@@ -41,6 +41,8 @@ SIGNATURES = {
     "feasm_bilform": (I32, [P, I32, I32, I32, P, I32, I32, PP]),
     "feasm_bilform_into": (I32, [P, I32, I32, I32, P, I32, I32, P]),
     "feasm_bilform_coo": (I32, [P, I32, I32, I32, P, I32, I32, PP]),
+    "feasm_linform_dot": (I32, [P, I32, I32, P, I32, P]),
+    "feasm_linform_dot_device": (I32, [P, I32, I32, P, I32, P]),
     "feasm_pattern_build": (I32, [P]),
     "feasm_mesh_set_numeric_path": (I32, [P, I32]),
     "feasm_mesh_tile_info": (I32, [P, PI64, PI64]),

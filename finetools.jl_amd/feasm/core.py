@@ -182,6 +182,14 @@ class DeviceMesh:
         check(lib().feasm_bilform_into(self.h, form, rule, order, ptr(c), len(c), precision, mat.h), self.ctx.h)
         return mat
 
+    def linform_dot(self, rule, order, force, nalldofs):
+        """Load vector of a constant force (linform_dot, m = 3, + makevector!):
+        a host array of nalldofs entries, entry d-1 for DOF d."""
+        f = np.ascontiguousarray(np.atleast_1d(force), dtype=np.float64).ravel()
+        F = np.zeros(int(nalldofs))
+        check(lib().feasm_linform_dot(self.h, rule, order, ptr(f), len(f), ptr(F)), self.ctx.h)
+        return F
+
 
 class Batch:
     """Many meshes assembled by one persistent launch (feasm_batch)."""

@@ -234,6 +234,25 @@ class SysmatAssemblerSparse:
         return np.float64
 
 
+class SysvecAssembler:
+    """SysvecAssembler(z=0.0) (src/AssemblyModule.jl:853-917).
+
+    The generic protocol keeps the reference's dense buffer semantics
+    (startassembly zeroes it, assemble adds an element vector with the same
+    bounds errors, makevector returns a copy).  Passed to linform_dot /
+    distribloads it runs the GPU element loop instead."""
+
+    def __init__(self, z=0.0, ctx=None):
+        if not isinstance(z, float):
+            raise TypeError("only Float64 assemblers are supported")
+        self.ctx = ctx or default_context()
+        self._buffer = np.zeros(1)
+        self._row_nalldofs = 1
+
+    def eltype(self):
+        return np.float64
+
+
 def setnomatrixresult(a, flag):
     a._nomatrixresult = bool(flag)
     check(lib().feasm_coo_set_nomatrixresult(a.h, 1 if flag else 0), a.ctx.h)
@@ -244,8 +263,12 @@ def expectedntriples(a, elem_mat_nrows, elem_mat_ncols, n_elem_mats):
     return elem_mat_nrows * elem_mat_ncols * n_elem_mats
 
 
-def startassembly(a, elem_mat_nrows, elem_mat_ncols, n_elem_mats, row_nalldofs, col_nalldofs,
-                  force_init=False):
+def startassembly(a, elem_mat_nrows, elem_mat_ncols=None, n_elem_mats=None, row_nalldofs=None,
+                  col_nalldofs=None, force_init=False):
+    if isinstance(a, SysvecAssembler):  # startassembly!(self::SysvecAssembler, row_nalldofs)
+        a._row_nalldofs = int(elem_mat_nrows)
+        a._buffer = np.zeros(a._row_nalldofs)
+        return a
     check(lib().feasm_coo_start(a.h, elem_mat_nrows, elem_mat_ncols, n_elem_mats, row_nalldofs, col_nalldofs),
           a.ctx.h)
     if a._dims is None:
@@ -253,7 +276,17 @@ def startassembly(a, elem_mat_nrows, elem_mat_ncols, n_elem_mats, row_nalldofs, 
     return a
 
 
-def assemble(a, mat, dofnums_row, dofnums_col):
+def assemble(a, mat, dofnums_row, dofnums_col=None):
+    if isinstance(a, SysvecAssembler):  # assemble!(self::SysvecAssembler, vec, dofnums)
+        vec = np.ravel(np.asarray(mat, dtype=np.float64))
+        for i, gi in enumerate(np.ravel(dofnums_row)):
+            gi = int(gi)
+            if gi < 1:
+                raise ValueError("Row degree of freedom < 1")
+            if gi > a._row_nalldofs:
+                raise ValueError("Row degree of freedom > size")
+            a._buffer[gi - 1] += vec[i]
+        return a
     m = np.asfortranarray(np.asarray(mat, dtype=np.float64))
     if m.ndim == 1:
         m = m.reshape(-1, 1)
@@ -262,6 +295,11 @@ def assemble(a, mat, dofnums_row, dofnums_col):
     check(lib().feasm_coo_assemble(a.h, m.ctypes.data_as(ctypes.c_void_p), m.shape[0], m.shape[1], ptr(r), len(r),
                                    ptr(c), len(c)), a.ctx.h)
     return a
+
+
+def makevector(a):
+    """makevector!(::SysvecAssembler): a copy of the buffer."""
+    return a._buffer.copy()
 
 
 def makematrix(a):
@@ -312,3 +350,33 @@ def bilform_dot(femm, assembler, geom, u, cf, m=3, return_device=False):
     ndn = u.ndofs()
     c = np.asarray(cf.data, dtype=np.float64).reshape(ndn, ndn)
     return _run(femm, assembler, geom, u, L.DOT, c, return_device)
+
+
+# ------------------------------------------------------------------ linear forms
+class ForceIntensity:
+    """ForceIntensity with a constant force vector (its DataCache)."""
+
+    def __init__(self, force):
+        self._cache = DataCache(np.asarray(force, dtype=np.float64))
+
+
+def linform_dot(femm, assembler, geom, P, f, m=3):
+    """linform_dot(self, assembler, geom, P, f, m) (src/FEMMBaseModule.jl:1207-1241)
+    with a constant DataCache force on a 3-manifold: the GPU element loop plus
+    makevector!.  Returns the dense load vector (nalldofs entries)."""
+    if not isinstance(assembler, SysvecAssembler):
+        raise TypeError("the GPU linear form takes a SysvecAssembler")
+    if m != 3:
+        raise NotImplementedError("only body loads (m = 3) of solid elements are on this path")
+    mesh = femm._device_mesh(geom, P)
+    kind, order = _rule(femm)
+    F = mesh.linform_dot(kind, order, np.ravel(f.data), nalldofs(P))
+    assembler._row_nalldofs = nalldofs(P)
+    assembler._buffer = F.copy()
+    return F
+
+
+def distribloads(femm, assembler, geom, P, fi, m=3):
+    """distribloads (src/FEMMBaseModule.jl:1262-1297): linform_dot with the
+    force intensity's cache."""
+    return linform_dot(femm, assembler, geom, P, fi._cache, m)

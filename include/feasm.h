@@ -159,6 +159,20 @@ int feasm_pattern_build(feasm_mesh *mesh);
 /* Pattern statistics: nnz of the DOF-level CSC and number of node blocks. */
 int feasm_pattern_info(feasm_mesh *mesh, int64_t *nnz, int64_t *nblocks);
 
+/* ---- load vectors (linform_dot + SysvecAssembler) ----------------------- */
+/* linform_dot(self, SysvecAssembler, geom, P, f, m = 3) with a constant force
+ * (src/FEMMBaseModule.jl:1207-1241; distribloads :1262-1297 forwards to it)
+ * followed by makevector! (src/AssemblyModule.jl:853-917): per element and
+ * quadrature point elvec[(k, mx)] += N_k (det J w) force[mx], then
+ * F[dofnums[i]] += elvec[i] element by element on a zeroed vector.  force has
+ * one value per DOF of a node (nforce == ndn).  The DOF numbers set on the
+ * mesh must be distinct.  F_device: nalldofs doubles on the device, entry
+ * d-1 for DOF d. */
+int feasm_linform_dot_device(feasm_mesh *mesh, int rule, int order, const double *force, int nforce,
+                             double *F_device);
+/* The same into a host array of nalldofs doubles (makevector!'s copy). */
+int feasm_linform_dot(feasm_mesh *mesh, int rule, int order, const double *force, int nforce, double *F_host);
+
 /* ---- matrices ----------------------------------------------------------- */
 /* makematrix! result (src/AssemblyModule.jl:304-329): SparseMatrixCSC{Float64,Int}. */
 int feasm_matrix_dims(const feasm_matrix *mat, int64_t *m, int64_t *n, int64_t *nnz);

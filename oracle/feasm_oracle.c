@@ -457,6 +457,53 @@ int64_t orc_bilform_coo(int etype, int rule, int order, int form, int64_t nelem,
     return p;
 }
 
+/* linform_dot with a constant force on a 3-manifold (m = 3) and a
+ * SysvecAssembler (src/FEMMBaseModule.jl:1207-1241; assembler
+ * src/AssemblyModule.jl:853-917).  Per element and quadrature point:
+ * Factor = Jac * w (Jacobianmdim(m = 3) = det J for solids,
+ * src/IntegDomainModule.jl:567-575), NkxF = N_k * Factor, elvec[(k, mx)] +=
+ * NkxF * force[mx]; then F[dofnums[i]] += elvec[i] element by element, on a
+ * buffer zeroed by startassembly!.  conn 1-based (nelem x npe), xyz nnodes x
+ * 3, dofnums nnodes x ndn 1-based; F has nalldofs entries (0-based storage).
+ * Returns 0, -1 (bad rule), -2 (DOF < 1), -3 (DOF > size). */
+int orc_linform_dot(int etype, int rule, int order, int64_t nelem, const int64_t *conn, const double *xyz, int ndn,
+                    const int64_t *dofnums, int64_t nalldofs, const double *force, double *F)
+{
+    orc_idata d;
+    if (orc_idata_make(&d, etype, rule, order) < 0) return -1;
+    const int npe = d.npe;
+    double X[30], loc[3], J[9], elvec[30];
+    for (int64_t i = 0; i < nalldofs; i++) F[i] = 0.0;
+    for (int64_t e = 0; e < nelem; e++) {
+        const int64_t *c = conn + e * npe;
+        for (int a = 0; a < npe; a++)
+            for (int k = 0; k < 3; k++) X[3 * a + k] = xyz[3 * (c[a] - 1) + k];
+        for (int i = 0; i < npe * ndn; i++) elvec[i] = 0.0;
+        for (int j = 0; j < d.npts; j++) {
+            const double *Nj = d.N + j * npe;
+            orc_locjac(npe, X, Nj, d.dN + j * npe * 3, loc, J);
+            const double Jac = orc_jacobian_det(J);
+            const double Factor = Jac * d.w[j];
+            int rx = 0;
+            for (int kx = 0; kx < npe; kx++) {
+                const double NkxF = Nj[kx] * Factor;
+                for (int mx = 0; mx < ndn; mx++) {
+                    elvec[rx] = elvec[rx] + NkxF * force[mx];
+                    rx++;
+                }
+            }
+        }
+        for (int a = 0; a < npe; a++)
+            for (int k = 0; k < ndn; k++) {
+                const int64_t gi = dofnums[(c[a] - 1) * ndn + k];
+                if (gi < 1) return -2;
+                if (gi > nalldofs) return -3;
+                F[gi - 1] += elvec[a * ndn + k];
+            }
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------------- */
 /* sparse(I, J, V, m, n) -- Julia stdlib SparseArrays (called from            */
 /* makematrix!, src/AssemblyModule.jl:319-325).  Not in the reference tree;  */

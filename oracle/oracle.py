@@ -49,6 +49,7 @@ def lib():
         L.orc_bilform_coo.argtypes = [ctypes.c_int] * 4 + [i64, P, P, ctypes.c_int, P, i64, P, P, P, P]
         L.orc_bilform_coo.restype = i64
         L.orc_sparse.argtypes = [i64, P, P, P, i64, i64]
+        L.orc_linform_dot.argtypes = [ctypes.c_int] * 3 + [i64, P, P, ctypes.c_int, P, i64, P, P]
         L.orc_sparse.restype = P
         L.orc_csc_nnz.argtypes = [P]
         L.orc_csc_nnz.restype = i64
@@ -177,6 +178,29 @@ def bilform_coo(etype, form, conn, xyz, ndn, dofnums, nalldofs, coeff, rule=None
     if p < 0:
         raise ValueError("bad rule")
     return I[:p], J[:p], V[:p]
+
+
+def linform_dot(etype, conn, xyz, ndn, force, dofnums=None, nalldofs=None, rule=None):
+    """linform_dot(m = 3) with a constant force vector + SysvecAssembler's
+    makevector!: the dense global load vector (0-based storage of 1-based DOFs)."""
+    rule, order = rule if rule is not None else rule_for(etype)
+    conn, xyz = _i64(conn), _f64(xyz)
+    nn = xyz.shape[0]
+    if dofnums is None:
+        dofnums = np.arange(1, nn * ndn + 1, dtype=np.int64).reshape(nn, ndn)
+        nalldofs = nn * ndn
+    dofnums = _i64(dofnums)
+    f = _f64(np.atleast_1d(force))
+    F = np.zeros(nalldofs)
+    rc = lib().orc_linform_dot(etype, rule, order, conn.shape[0], _p(conn), _p(xyz), ndn, _p(dofnums), nalldofs,
+                               _p(f), _p(F))
+    if rc == -2:
+        raise ValueError("Row degree of freedom < 1")
+    if rc == -3:
+        raise ValueError("Row degree of freedom > size")
+    if rc < 0:
+        raise ValueError("bad rule")
+    return F
 
 
 def sparse(I, J, V, m, n):
