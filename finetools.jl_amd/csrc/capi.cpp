@@ -229,6 +229,7 @@ extern "C" int feasm_ctx_create(int device, feasm_ctx **out) {
     std::unique_ptr<feasm_ctx> c(new feasm_ctx);
     c->device = device;
     FEASM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    pool_register_stream(device, c->stream);
     *out = c.release();
     API_CATCH(nullptr)
 }
@@ -239,6 +240,7 @@ void feasm_ctx_release(feasm_ctx *ctx) {
     ctx->qtables.clear();
     ctx->scratch.reset();
     for (auto e : ctx->ring) (void)hipEventDestroy(e);
+    pool_unregister_stream(ctx->device, ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

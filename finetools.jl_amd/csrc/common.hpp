@@ -35,17 +35,25 @@ struct Error : std::runtime_error {
     } while (0)
 
 // ---------------------------------------------------------------- device buffers
+// pool.cpp: cached device blocks (bytes receives the capacity, >= n)
+void *pool_alloc(size_t n, size_t *bytes, int *device);
+void pool_release(void *p, size_t bytes, int device);
+void pool_register_stream(int device, hipStream_t s);
+void pool_unregister_stream(int device, hipStream_t s);
+size_t pool_cached_bytes(int device);
+
 // Owning device allocation; shared so that a matrix can keep the pattern
-// arrays of its mesh alive.
+// arrays of its mesh alive.  bytes is the capacity (at least what was asked).
 struct DevMem {
     void *p = nullptr;
     size_t bytes = 0;
+    int device = 0;
     DevMem() = default;
-    explicit DevMem(size_t n) : bytes(n) {
-        if (n) FEASM_HIP(hipMalloc(&p, n));
+    explicit DevMem(size_t n) {
+        if (n) p = pool_alloc(n, &bytes, &device);
     }
     ~DevMem() {
-        if (p) (void)hipFree(p);
+        if (p) pool_release(p, bytes, device);
     }
     DevMem(const DevMem &) = delete;
     DevMem &operator=(const DevMem &) = delete;
