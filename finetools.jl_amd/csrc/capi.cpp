@@ -466,6 +466,36 @@ extern "C" int feasm_linform_dot(feasm_mesh *m, int rule, int order, const doubl
     API_CATCH(m ? m->ctx : nullptr)
 }
 
+// ======================================================================= dropzeros
+extern "C" int feasm_matrix_dropzeros(feasm_matrix *mat, feasm_matrix **out) {
+    API_TRY
+    FEASM_REQUIRE(mat && out, FEASM_EINVAL, "null argument");
+    *out = drop_zeros(mat->ctx, mat);
+    API_CATCH(mat ? mat->ctx.p : nullptr)
+}
+
+// ======================================================================= lumped mass
+extern "C" int feasm_bilform_dot_lumped(feasm_mesh *m, int rule, int order, const double *coeff, int ncoeff,
+                                        feasm_matrix **out) {
+    API_TRY
+    FEASM_REQUIRE(m && coeff && out, FEASM_EINVAL, "null argument");
+    FEASM_REQUIRE(m->ndn > 0, FEASM_ESTATE, "set the DOF numbering first");
+    FEASM_REQUIRE(ncoeff == m->ndn * m->ndn, FEASM_EINVAL, "the coefficient is an ndn x ndn matrix");
+    FEASM_REQUIRE(m->dofs_unique, FEASM_EUNSUPPORTED, "lumped mass needs distinct DOF numbers per (node, component)");
+    check_rule(m, rule);
+    feasm_ctx *ctx = m->ctx;
+    if (!m->n2e_ptr) build_node_to_element(m);
+    const DevQTable &qt = device_qtable(ctx, m->etype, rule, order);
+    Timer tm(ctx);
+    tm.mark(0);
+    feasm_matrix *M = hrz_lumped(ctx, m, qt, coeff);
+    tm.mark(1);
+    tm.mark(2);
+    tm.done();
+    *out = M;
+    API_CATCH(m ? m->ctx : nullptr)
+}
+
 // ======================================================================= bilinear forms
 extern "C" int feasm_pattern_build(feasm_mesh *m) {
     API_TRY

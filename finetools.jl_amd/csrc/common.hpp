@@ -201,6 +201,13 @@ const DevQTable &device_qtable(feasm_ctx *ctx, int etype, int rule, int order);
 void launch_linform_dot(feasm_ctx *ctx, const feasm_mesh *m, const DevQTable &qt, const double *force,
                         double *stage, double *F);
 
+// lumped.hip: HRZ-lumped diagonal of the DOT form with an ndn x ndn
+// coefficient (host); a new diagonal CSC matrix
+feasm_matrix *hrz_lumped(feasm_ctx *ctx, feasm_mesh *m, const DevQTable &qt, const double *coef_host);
+
+// dropzeros.hip: a new matrix without the stored exact zeros of a
+feasm_matrix *drop_zeros(feasm_ctx *ctx, const feasm_matrix *a);
+
 // pattern.hip
 void build_node_to_element(feasm_mesh *m);
 std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn);

@@ -1,0 +1,165 @@
+// HRZ-lumped mass: bilform_dot (src/FEMMBaseModule.jl:1335-1366) into a
+// SysmatAssemblerSparseHRZLumpingSymm (src/AssemblyModule.jl:943-1141).
+//
+// k_hrz_elem: one thread per element forms the element mass matrix entries in
+// the reference's arithmetic -- per quadrature point factor = N_a N_b J w,
+// Ke += factor * c -- only as far as the lumping needs them: the column sums
+// summed (em2), the diagonal summed (dem2), and the diagonal scaled by
+// em2 / dem2, which it stages per (element, node, component).
+// k_hrz_gather: one thread per node adds its elements' staged values in
+// ascending element number (sparse() meets the element-major triples in that
+// order) and marks the DOFs that received a value.
+// k_hrz_compact: the diagonal CSC of the marked DOFs.
+#include "common.hpp"
+
+namespace feasm {
+
+namespace {
+
+template <int NPE>
+__global__ __launch_bounds__(256) void k_hrz_elem(const int32_t *__restrict__ conn, const double *__restrict__ xyz,
+                                                  int64_t nelem, int nqp, const double *__restrict__ N,
+                                                  const double *__restrict__ dN, const double *__restrict__ w,
+                                                  const double *__restrict__ coef, int ndn,
+                                                  double *__restrict__ stage) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nelem; e += (int64_t)gridDim.x * blockDim.x) {
+    double X[NPE][3];
+#pragma unroll
+    for (int a = 0; a < NPE; a++) {
+        const double *p = xyz + 3 * (int64_t)conn[e * NPE + a];
+        X[a][0] = p[0];
+        X[a][1] = p[1];
+        X[a][2] = p[2];
+    }
+    double jac[64];  // det J per point (rules up to 64 points)
+    for (int q = 0; q < nqp; q++) {
+        double J[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) J[i] = 0.0;
+#pragma unroll
+        for (int a = 0; a < NPE; a++) {
+            const double d0 = dN[(q * NPE + a) * 3], d1 = dN[(q * NPE + a) * 3 + 1], d2 = dN[(q * NPE + a) * 3 + 2];
+            J[0] += X[a][0] * d0; J[1] += X[a][0] * d1; J[2] += X[a][0] * d2;
+            J[3] += X[a][1] * d0; J[4] += X[a][1] * d1; J[5] += X[a][1] * d2;
+            J[6] += X[a][2] * d0; J[7] += X[a][2] * d1; J[8] += X[a][2] * d2;
+        }
+        jac[q] = J[0] * (J[4] * J[8] - J[7] * J[5]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                 J[2] * (J[3] * J[7] - J[4] * J[6]);
+    }
+    // em2: column (b, qq) sums of Ke[(a, p), (b, qq)] = sum_q (N_a N_b J w) c[p][qq]
+    double em2 = 0.0;
+    for (int b = 0; b < NPE; b++)
+        for (int cq = 0; cq < ndn; cq++) {
+            double cs = 0.0;
+            for (int a = 0; a < NPE; a++)
+                for (int p = 0; p < ndn; p++) {
+                    double ke = 0.0;
+                    for (int q = 0; q < nqp; q++) {
+                        const double factor = N[q * NPE + a] * N[q * NPE + b] * jac[q] * w[q];
+                        ke += factor * coef[ndn * p + cq];
+                    }
+                    cs += ke;
+                }
+            em2 += cs;
+        }
+    double diag[NPE * 3];
+    double dem2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < NPE; a++)
+        for (int p = 0; p < ndn; p++) {
+            double ke = 0.0;
+            for (int q = 0; q < nqp; q++) {
+                const double factor = N[q * NPE + a] * N[q * NPE + a] * jac[q] * w[q];
+                ke += factor * coef[ndn * p + p];
+            }
+            diag[a * 3 + p] = ke;
+            dem2 += ke;
+        }
+    const double ffactor = em2 / dem2;
+    double *out = stage + e * NPE * ndn;
+#pragma unroll
+    for (int a = 0; a < NPE; a++)
+        for (int p = 0; p < ndn; p++) out[a * ndn + p] = diag[a * 3 + p] * ffactor;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_hrz_gather(const int64_t *__restrict__ n2e_ptr,
+                                                    const uint32_t *__restrict__ n2e, int64_t nnodes, int npe,
+                                                    int ndn, const double *__restrict__ stage,
+                                                    const int32_t *__restrict__ dofs, double *__restrict__ val,
+                                                    int32_t *__restrict__ flag) {
+  for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < nnodes; n += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s0 = n2e_ptr[n], s1 = n2e_ptr[n + 1];
+    if (s0 == s1) continue;  // a node of no element: no triple, no stored entry
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int64_t s = s0; s < s1; s++) {
+        const uint32_t v = n2e[s];
+        const double *p = stage + ((int64_t)(v >> 4) * npe + (v & 15u)) * ndn;
+        for (int k = 0; k < ndn; k++) acc[k] = acc[k] + p[k];
+    }
+    for (int k = 0; k < ndn; k++) {
+        const int32_t d = dofs[n * ndn + k];
+        val[d] = acc[k];
+        flag[d] = 1;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_hrz_compact(const int32_t *__restrict__ flag, const int64_t *__restrict__ pos,
+                                                     const double *__restrict__ val, int64_t n,
+                                                     int32_t *__restrict__ rowval, double *__restrict__ nzval) {
+    for (int64_t d = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; d < n; d += (int64_t)gridDim.x * blockDim.x)
+        if (flag[d]) {
+            rowval[pos[d]] = (int32_t)d;
+            nzval[pos[d]] = val[d];
+        }
+}
+
+}  // namespace
+
+feasm_matrix *hrz_lumped(feasm_ctx *ctx, feasm_mesh *m, const DevQTable &qt, const double *coef_host) {
+    const int ndn = m->ndn;
+    const int64_t n = m->nalldofs;
+    FEASM_REQUIRE(m->conn && m->xyz && m->n2e_ptr && m->n2e && m->dofs, FEASM_ESTATE, "lumped mass: mesh arrays missing");
+    FEASM_REQUIRE(qt.nqp <= 64, FEASM_EUNSUPPORTED, "lumped mass: rules of at most 64 points");
+    DevPtr coef = dev_alloc(ndn * ndn * sizeof(double));
+    FEASM_HIP(hipMemcpyAsync(coef->p, coef_host, ndn * ndn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    DevPtr stage = dev_alloc(std::max<size_t>((size_t)m->nelem * m->npe * ndn * sizeof(double), 8));
+    DevPtr val = dev_alloc(std::max<int64_t>(n, 1) * sizeof(double));
+    DevPtr flag = dev_alloc((n + 1) * sizeof(int32_t));
+    FEASM_HIP(hipMemsetAsync(flag->p, 0, (n + 1) * sizeof(int32_t), ctx->stream));
+    const unsigned ge = grid_for(m->nelem, 256);
+    if (m->npe == 8)
+        hipLaunchKernelGGL((k_hrz_elem<8>), dim3(ge), dim3(256), 0, ctx->stream, dp<int32_t>(m->conn),
+                           dp<double>(m->xyz), m->nelem, qt.nqp, qt.N(), qt.dN(), qt.w(), dp<double>(coef), ndn,
+                           dp<double>(stage));
+    else if (m->npe == 10)
+        hipLaunchKernelGGL((k_hrz_elem<10>), dim3(ge), dim3(256), 0, ctx->stream, dp<int32_t>(m->conn),
+                           dp<double>(m->xyz), m->nelem, qt.nqp, qt.N(), qt.dN(), qt.w(), dp<double>(coef), ndn,
+                           dp<double>(stage));
+    else
+        throw Error(FEASM_EUNSUPPORTED, "lumped mass covers H8 and T10 meshes");
+    FEASM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_hrz_gather, dim3(grid_for(m->nnodes, 256)), dim3(256), 0, ctx->stream,
+                       dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e), m->nnodes, m->npe, ndn, dp<double>(stage),
+                       dp<int32_t>(m->dofs), dp<double>(val), dp<int32_t>(flag));
+    FEASM_HIP(hipGetLastError());
+    std::unique_ptr<feasm_matrix> M(new feasm_matrix);
+    M->ctx = ctx;
+    M->m = M->n = n;
+    M->colptr = dev_alloc((n + 1) * sizeof(int64_t));
+    exclusive_scan_i32_to_i64(ctx, dp<int32_t>(flag), dp<int64_t>(M->colptr), n + 1);
+    int64_t nnz = 0;
+    FEASM_HIP(hipMemcpyAsync(&nnz, dp<int64_t>(M->colptr) + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    M->nnz = nnz;
+    M->rowval = dev_alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t));
+    M->nzval = dev_alloc(std::max<int64_t>(nnz, 1) * sizeof(double));
+    hipLaunchKernelGGL(k_hrz_compact, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(flag),
+                       dp<int64_t>(M->colptr), dp<double>(val), n, dp<int32_t>(M->rowval), dp<double>(M->nzval));
+    FEASM_HIP(hipGetLastError());
+    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    return M.release();
+}
+
+}  // namespace feasm

@@ -91,6 +91,12 @@ class DeviceMatrix:
     def nnz(self):
         return self.dims[2]
 
+    def dropzeros(self):
+        """A new DeviceMatrix without the stored exact zeros (SparseArrays.dropzeros)."""
+        h = ctypes.c_void_p()
+        check(lib().feasm_matrix_dropzeros(self.h, ctypes.byref(h)), self.ctx.h)
+        return DeviceMatrix(self.ctx, h)
+
     def csc_arrays(self, base=0):
         m, n, nnz = self.dims
         colptr = np.empty(n + 1, np.int64)
@@ -181,6 +187,14 @@ class DeviceMesh:
         c = np.ascontiguousarray(np.atleast_1d(coeff), dtype=np.float64).ravel()
         check(lib().feasm_bilform_into(self.h, form, rule, order, ptr(c), len(c), precision, mat.h), self.ctx.h)
         return mat
+
+    def bilform_dot_lumped(self, rule, order, coeff):
+        """HRZ-lumped mass (bilform_dot into SysmatAssemblerSparseHRZLumpingSymm,
+        makematrix!): a diagonal DeviceMatrix."""
+        c = np.ascontiguousarray(np.atleast_1d(coeff), dtype=np.float64).ravel()
+        h = ctypes.c_void_p()
+        check(lib().feasm_bilform_dot_lumped(self.h, rule, order, ptr(c), len(c), ctypes.byref(h)), self.ctx.h)
+        return DeviceMatrix(self.ctx, h)
 
     def linform_dot(self, rule, order, force, nalldofs):
         """Load vector of a constant force (linform_dot, m = 3, + makevector!):

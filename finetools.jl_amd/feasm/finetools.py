@@ -234,6 +234,41 @@ class SysmatAssemblerSparse:
         return np.float64
 
 
+class SysmatAssemblerSparseSymm:
+    """SysmatAssemblerSparseSymm(z=0.0) (src/AssemblyModule.jl:342-583), the
+    default assembler of the convenience bilform_* methods.  Passed to a
+    bilform_* function it runs the GPU element loop and returns the full
+    symmetric matrix without the entries that sum to exactly zero, the
+    pattern the reference's S + transpose(S) keeps."""
+
+    def __init__(self, z=0.0, nomatrixresult=False, ctx=None, precision=L.FP64):
+        if not isinstance(z, float):
+            raise TypeError("only Float64 assemblers are supported")
+        if nomatrixresult:
+            raise NotImplementedError("nomatrixresult is not on the symmetric path")
+        self.ctx = ctx or default_context()
+        self.precision = precision
+
+    def eltype(self):
+        return np.float64
+
+
+class SysmatAssemblerSparseHRZLumpingSymm:
+    """SysmatAssemblerSparseHRZLumpingSymm(z=0.0) (src/AssemblyModule.jl:943-1141).
+    Passed to bilform_dot it runs the GPU element loop and returns the
+    HRZ-lumped diagonal matrix (makematrix!)."""
+
+    def __init__(self, z=0.0, nomatrixresult=False, ctx=None):
+        if not isinstance(z, float):
+            raise TypeError("only Float64 assemblers are supported")
+        if nomatrixresult:
+            raise NotImplementedError("nomatrixresult is not on the lumped path")
+        self.ctx = ctx or default_context()
+
+    def eltype(self):
+        return np.float64
+
+
 class SysvecAssembler:
     """SysvecAssembler(z=0.0) (src/AssemblyModule.jl:853-917).
 
@@ -323,11 +358,14 @@ def _rule(femm):
 
 
 def _run(femm, assembler, geom, u, form, coeff, return_device=False):
-    if not isinstance(assembler, SysmatAssemblerSparse):
-        raise TypeError("the GPU bilinear forms take a SysmatAssemblerSparse")
+    if not isinstance(assembler, (SysmatAssemblerSparse, SysmatAssemblerSparseSymm)):
+        raise TypeError("the GPU bilinear forms take a SysmatAssemblerSparse or SysmatAssemblerSparseSymm")
     mesh = femm._device_mesh(geom, u)
     kind, order = _rule(femm)
     mat = mesh.bilform(form, kind, order, coeff, precision=assembler.precision)
+    if isinstance(assembler, SysmatAssemblerSparseSymm):
+        full, mat = mat, mat.dropzeros()
+        full.close()
     return mat if return_device else mat.to_scipy()
 
 
@@ -349,6 +387,11 @@ def bilform_dot(femm, assembler, geom, u, cf, m=3, return_device=False):
         raise ValueError("That is the only acceptable option here.")
     ndn = u.ndofs()
     c = np.asarray(cf.data, dtype=np.float64).reshape(ndn, ndn)
+    if isinstance(assembler, SysmatAssemblerSparseHRZLumpingSymm):
+        mesh = femm._device_mesh(geom, u)
+        kind, order = _rule(femm)
+        mat = mesh.bilform_dot_lumped(kind, order, c)
+        return mat if return_device else mat.to_scipy()
     return _run(femm, assembler, geom, u, L.DOT, c, return_device)
 
 

@@ -173,6 +173,16 @@ int feasm_linform_dot_device(feasm_mesh *mesh, int rule, int order, const double
 /* The same into a host array of nalldofs doubles (makevector!'s copy). */
 int feasm_linform_dot(feasm_mesh *mesh, int rule, int order, const double *force, int nforce, double *F_host);
 
+/* ---- lumped mass (SysmatAssemblerSparseHRZLumpingSymm) ------------------ */
+/* bilform_dot (src/FEMMBaseModule.jl:1335-1366) with an ndn x ndn coefficient
+ * (row-major, ncoeff == ndn*ndn) into a SysmatAssemblerSparseHRZLumpingSymm
+ * and makematrix! (src/AssemblyModule.jl:943-1141): each element mass
+ * matrix's diagonal scaled by (sum of its entries) / (sum of its diagonal),
+ * summed per DOF in element order; the diagonal CSC, one entry per DOF that
+ * some element touches.  The DOF numbers set on the mesh must be distinct. */
+int feasm_bilform_dot_lumped(feasm_mesh *mesh, int rule, int order, const double *coeff, int ncoeff,
+                             feasm_matrix **out);
+
 /* ---- matrices ----------------------------------------------------------- */
 /* makematrix! result (src/AssemblyModule.jl:304-329): SparseMatrixCSC{Float64,Int}. */
 int feasm_matrix_dims(const feasm_matrix *mat, int64_t *m, int64_t *n, int64_t *nnz);
@@ -184,6 +194,12 @@ int feasm_matrix_copy_csc(feasm_matrix *mat, int base, int64_t *colptr, int64_t 
 int feasm_matrix_device_csc(feasm_matrix *mat, const int64_t **colptr, const int32_t **rowval,
                             double **nzval);
 int feasm_matrix_destroy(feasm_matrix *mat);
+/* A new matrix holding mat's entries except the stored exact zeros
+ * (SparseArrays.dropzeros).  bilform_* into a SysmatAssemblerSparseSymm
+ * (src/AssemblyModule.jl:342-583) gives the full symmetric matrix with this
+ * pattern: its makematrix! forms S + transpose(S) of the lower-triangle
+ * triples, and sparse + stores no zero result. */
+int feasm_matrix_dropzeros(feasm_matrix *mat, feasm_matrix **out);
 
 /* ---- generic assembler protocol (SysmatAssemblerSparse) ----------------- */
 /* SysmatAssemblerSparse(0.0, nomatrixresult) (src/AssemblyModule.jl:159-165).

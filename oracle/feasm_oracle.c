@@ -457,6 +457,53 @@ int64_t orc_bilform_coo(int etype, int rule, int order, int form, int64_t nelem,
     return p;
 }
 
+/* bilform_dot (src/FEMMBaseModule.jl:1335-1366) into a
+ * SysmatAssemblerSparseHRZLumpingSymm (src/AssemblyModule.jl:943-1141): for
+ * every element mass matrix, em2 = sum(sum(mat, dims = 1)) (column sums, then
+ * their sum), dem2 = sum of the diagonal, ffactor = em2 / dem2, and the
+ * triples (dj, dj, mat[j,j] * ffactor) for j in element order; makematrix!
+ * is sparse() over them.  Writes npe*ndn triples per element into I/J/V
+ * (1-based) and returns their count, or -1 (bad rule), -2 (DOF < 1), -3 (DOF
+ * > size).  Julia's sum may reassociate (@simd), so em2's last bits are
+ * "parity unpinned". */
+int64_t orc_hrz_lumped_coo(int etype, int rule, int order, int64_t nelem, const int64_t *conn, const double *xyz,
+                           int ndn, const int64_t *dofnums, int64_t nalldofs, const double *coeff, int64_t *I,
+                           int64_t *J, double *V)
+{
+    orc_idata d;
+    if (orc_idata_make(&d, etype, rule, order) < 0) return -1;
+    const int npe = d.npe, kd = ndn * npe;
+    double X[30], Ke[900];
+    int64_t p = 0;
+    for (int64_t e = 0; e < nelem; e++) {
+        const int64_t *c = conn + e * npe;
+        for (int a = 0; a < npe; a++)
+            for (int k = 0; k < 3; k++) X[3 * a + k] = xyz[3 * (c[a] - 1) + k];
+        orc_element_matrix(&d, ORC_FORM_DOT, ndn, X, coeff, Ke);
+        double em2 = 0.0;
+        for (int jj = 0; jj < kd; jj++) {
+            double cs = 0.0;
+            for (int ii = 0; ii < kd; ii++) cs += Ke[ii + kd * jj];
+            em2 += cs;
+        }
+        double dem2 = 0.0;
+        for (int ii = 0; ii < kd; ii++) dem2 += Ke[ii + kd * ii];
+        const double ffactor = em2 / dem2;
+        for (int a = 0; a < npe; a++)
+            for (int k = 0; k < ndn; k++) {
+                const int jj = a * ndn + k;
+                const int64_t dj = dofnums[(c[a] - 1) * ndn + k];
+                if (dj < 1) return -2;
+                if (dj > nalldofs) return -3;
+                V[p] = Ke[jj + kd * jj] * ffactor;
+                I[p] = dj;
+                J[p] = dj;
+                p++;
+            }
+    }
+    return p;
+}
+
 /* linform_dot with a constant force on a 3-manifold (m = 3) and a
  * SysvecAssembler (src/FEMMBaseModule.jl:1207-1241; assembler
  * src/AssemblyModule.jl:853-917).  Per element and quadrature point:
@@ -612,6 +659,111 @@ void orc_csc_free(orc_csc *S)
     free(S->rowval);
     free(S->nzval);
     free(S);
+}
+
+/* makematrix!(::SysmatAssemblerSparseSymm) after sparse() of the lower-
+ * triangle triples (src/AssemblyModule.jl:567-583): S + transpose(S), where
+ * SparseArrays' + stores only nonzero results, then every stored diagonal
+ * entry halved (S[j, j] *= 0.5 leaves an absent entry absent).  Returns a new
+ * CSC (1-based). */
+orc_csc *orc_symm_from_lower(const orc_csc *S)
+{
+    const int64_t n = S->n;
+    /* transpose of S by a counting pass */
+    int64_t *tp = calloc((size_t)n + 2, sizeof(int64_t));
+    int64_t *tr = malloc(sizeof(int64_t) * (size_t)(S->nnz ? S->nnz : 1));
+    double *tv = malloc(sizeof(double) * (size_t)(S->nnz ? S->nnz : 1));
+    for (int64_t k = 0; k < S->nnz; k++) tp[S->rowval[k]]++; /* 1-based row r counts in slot r */
+    for (int64_t c = 0; c < n; c++) tp[c + 1] += tp[c];
+    int64_t *cur = malloc(sizeof(int64_t) * (size_t)(n + 1));
+    for (int64_t c = 0; c <= n; c++) cur[c] = tp[c];
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t k = S->colptr[c] - 1; k < S->colptr[c + 1] - 1; k++) {
+            const int64_t r = S->rowval[k] - 1; /* row of S = column of the transpose */
+            tr[cur[r]] = c + 1;
+            tv[cur[r]] = S->nzval[k];
+            cur[r]++;
+        }
+    orc_csc *C = malloc(sizeof(orc_csc));
+    C->m = S->m;
+    C->n = n;
+    C->colptr = malloc(sizeof(int64_t) * (size_t)(n + 1));
+    C->rowval = malloc(sizeof(int64_t) * (size_t)(2 * S->nnz + 1));
+    C->nzval = malloc(sizeof(double) * (size_t)(2 * S->nnz + 1));
+    int64_t q = 0;
+    C->colptr[0] = 1;
+    for (int64_t c = 0; c < n; c++) {
+        int64_t a = S->colptr[c] - 1, ae = S->colptr[c + 1] - 1, b = tp[c], be = tp[c + 1];
+        while (a < ae || b < be) {
+            int64_t r;
+            double v;
+            if (b >= be || (a < ae && S->rowval[a] < tr[b])) {
+                r = S->rowval[a];
+                v = S->nzval[a] + 0.0;
+                a++;
+            } else if (a >= ae || tr[b] < S->rowval[a]) {
+                r = tr[b];
+                v = 0.0 + tv[b];
+                b++;
+            } else {
+                r = S->rowval[a];
+                v = S->nzval[a] + tv[b];
+                a++;
+                b++;
+            }
+            if (v != 0.0) {
+                C->rowval[q] = r;
+                C->nzval[q] = (r == c + 1) ? v * 0.5 : v;
+                q++;
+            }
+        }
+        C->colptr[c + 1] = q + 1;
+    }
+    C->nnz = q;
+    free(tp);
+    free(tr);
+    free(tv);
+    free(cur);
+    return C;
+}
+
+/* Element loop of bilform_* into a SysmatAssemblerSparseSymm: assemble!
+ * keeps the lower triangle in element-local order, for j, for i = j..kd
+ * (src/AssemblyModule.jl:494-523).  Same arguments and returns as
+ * orc_bilform_coo. */
+int64_t orc_bilform_coo_symm(int etype, int rule, int order, int form, int64_t nelem, const int64_t *conn,
+                             const double *xyz, int ndn, const int64_t *dofnums, int64_t nalldofs,
+                             const double *coeff, int64_t *I, int64_t *J, double *V)
+{
+    orc_idata d;
+    if (orc_idata_make(&d, etype, rule, order) < 0) return -1;
+    const int npe = d.npe, kd = ndn * npe;
+    double X[30], Ke[900];
+    int64_t dofs[30];
+    int64_t p = 0;
+    for (int64_t e = 0; e < nelem; e++) {
+        const int64_t *c = conn + e * npe;
+        for (int a = 0; a < npe; a++)
+            for (int k = 0; k < 3; k++) X[3 * a + k] = xyz[3 * (c[a] - 1) + k];
+        orc_element_matrix(&d, form, ndn, X, coeff, Ke);
+        for (int a = 0; a < npe; a++)
+            for (int k = 0; k < ndn; k++) dofs[a * ndn + k] = dofnums[(c[a] - 1) * ndn + k];
+        for (int jj = 0; jj < kd; jj++) {
+            const int64_t dj = dofs[jj];
+            if (dj < 1) return -2;
+            if (dj > nalldofs) return -3;
+            for (int ii = jj; ii < kd; ii++) {
+                const int64_t di = dofs[ii];
+                if (di < 1) return -2;
+                if (di > nalldofs) return -3;
+                V[p] = Ke[ii + kd * jj];
+                I[p] = di;
+                J[p] = dj;
+                p++;
+            }
+        }
+    }
+    return p;
 }
 
 /* ------------------------------------------------------------------------- */

@@ -50,6 +50,12 @@ def lib():
         L.orc_bilform_coo.restype = i64
         L.orc_sparse.argtypes = [i64, P, P, P, i64, i64]
         L.orc_linform_dot.argtypes = [ctypes.c_int] * 3 + [i64, P, P, ctypes.c_int, P, i64, P, P]
+        L.orc_hrz_lumped_coo.argtypes = [ctypes.c_int] * 3 + [i64, P, P, ctypes.c_int, P, i64, P, P, P, P]
+        L.orc_hrz_lumped_coo.restype = i64
+        L.orc_bilform_coo_symm.argtypes = [ctypes.c_int] * 4 + [i64, P, P, ctypes.c_int, P, i64, P, P, P, P]
+        L.orc_bilform_coo_symm.restype = i64
+        L.orc_symm_from_lower.argtypes = [P]
+        L.orc_symm_from_lower.restype = P
         L.orc_sparse.restype = P
         L.orc_csc_nnz.argtypes = [P]
         L.orc_csc_nnz.restype = i64
@@ -201,6 +207,81 @@ def linform_dot(etype, conn, xyz, ndn, force, dofnums=None, nalldofs=None, rule=
     if rc < 0:
         raise ValueError("bad rule")
     return F
+
+
+def _csc_out(h, n):
+    try:
+        nnz = lib().orc_csc_nnz(h)
+        colptr = np.zeros(n + 1, np.int64)
+        rowval = np.zeros(nnz, np.int64)
+        nzval = np.zeros(nnz)
+        lib().orc_csc_copy(h, _p(colptr), _p(rowval), _p(nzval))
+    finally:
+        lib().orc_csc_free(h)
+    return colptr, rowval, nzval
+
+
+def symm_makematrix(I, J, V, n):
+    """makematrix!(::SysmatAssemblerSparseSymm) of lower-triangle triples:
+    sparse(), then S + transpose(S) keeping nonzero sums, diagonal halved."""
+    I, J, V = _i64(I), _i64(J), _f64(V)
+    h = lib().orc_sparse(len(I), _p(I), _p(J), _p(V), n, n)
+    if not h:
+        raise ValueError("index out of range")
+    try:
+        h2 = lib().orc_symm_from_lower(h)
+    finally:
+        lib().orc_csc_free(h)
+    return _csc_out(h2, n)
+
+
+def assemble_csc_symm(etype, form, conn, xyz, ndn, coeff, dofnums=None, nalldofs=None, rule=None):
+    """bilform_* into a SysmatAssemblerSparseSymm + makematrix!."""
+    rule, order = rule if rule is not None else rule_for(etype)
+    conn, xyz = _i64(conn), _f64(xyz)
+    nn = xyz.shape[0]
+    if dofnums is None:
+        dofnums = np.arange(1, nn * ndn + 1, dtype=np.int64).reshape(nn, ndn)
+        nalldofs = nn * ndn
+    dofnums = _i64(dofnums)
+    c = _f64(np.atleast_1d(coeff))
+    ne, npe = conn.shape
+    kd = ndn * npe
+    nt = ne * (kd * (kd + 1) // 2)
+    I = np.zeros(nt, np.int64)
+    J = np.zeros(nt, np.int64)
+    V = np.zeros(nt)
+    p = lib().orc_bilform_coo_symm(etype, rule, order, form, ne, _p(conn), _p(xyz), ndn, _p(dofnums), nalldofs,
+                                   _p(c), _p(I), _p(J), _p(V))
+    if p < 0:
+        raise ValueError("bad rule or DOF")
+    return symm_makematrix(I[:p], J[:p], V[:p], nalldofs)
+
+
+def hrz_lumped_csc(etype, conn, xyz, ndn, coeff, dofnums=None, nalldofs=None, rule=None):
+    """bilform_dot + SysmatAssemblerSparseHRZLumpingSymm's makematrix!: the
+    lumped (diagonal) mass as CSC (colptr, rowval, nzval), 1-based."""
+    rule, order = rule if rule is not None else rule_for(etype)
+    conn, xyz = _i64(conn), _f64(xyz)
+    nn = xyz.shape[0]
+    if dofnums is None:
+        dofnums = np.arange(1, nn * ndn + 1, dtype=np.int64).reshape(nn, ndn)
+        nalldofs = nn * ndn
+    dofnums = _i64(dofnums)
+    c = _f64(np.atleast_1d(coeff))
+    nt = conn.shape[0] * conn.shape[1] * ndn
+    I = np.zeros(nt, np.int64)
+    J = np.zeros(nt, np.int64)
+    V = np.zeros(nt)
+    p = lib().orc_hrz_lumped_coo(etype, rule, order, conn.shape[0], _p(conn), _p(xyz), ndn, _p(dofnums), nalldofs,
+                                 _p(c), _p(I), _p(J), _p(V))
+    if p == -2:
+        raise ValueError("Column degree of freedom < 1")
+    if p == -3:
+        raise ValueError("Column degree of freedom > size")
+    if p < 0:
+        raise ValueError("bad rule")
+    return sparse(I[:p], J[:p], V[:p], nalldofs, nalldofs)
 
 
 def sparse(I, J, V, m, n):
