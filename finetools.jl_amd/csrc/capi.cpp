@@ -474,6 +474,14 @@ extern "C" int feasm_matrix_dropzeros(feasm_matrix *mat, feasm_matrix **out) {
     API_CATCH(mat ? mat->ctx.p : nullptr)
 }
 
+// ======================================================================= free-free block
+extern "C" int feasm_matrix_free_block(feasm_matrix *mat, const uint8_t *is_free, feasm_matrix **out) {
+    API_TRY
+    FEASM_REQUIRE(mat && is_free && out, FEASM_EINVAL, "null argument");
+    *out = principal_submatrix(mat->ctx, mat, is_free);
+    API_CATCH(mat ? mat->ctx.p : nullptr)
+}
+
 // ======================================================================= lumped mass
 extern "C" int feasm_bilform_dot_lumped(feasm_mesh *m, int rule, int order, const double *coeff, int ncoeff,
                                         feasm_matrix **out) {

@@ -208,6 +208,10 @@ feasm_matrix *hrz_lumped(feasm_ctx *ctx, feasm_mesh *m, const DevQTable &qt, con
 // dropzeros.hip: a new matrix without the stored exact zeros of a
 feasm_matrix *drop_zeros(feasm_ctx *ctx, const feasm_matrix *a);
 
+// blocks.hip: principal submatrix on the kept DOFs (host flags, one per
+// column), renumbered consecutively
+feasm_matrix *principal_submatrix(feasm_ctx *ctx, const feasm_matrix *a, const uint8_t *keep_host);
+
 // pattern.hip
 void build_node_to_element(feasm_mesh *m);
 std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn);

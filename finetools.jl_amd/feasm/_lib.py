@@ -44,6 +44,7 @@ SIGNATURES = {
     "feasm_linform_dot": (I32, [P, I32, I32, P, I32, P]),
     "feasm_bilform_dot_lumped": (I32, [P, I32, I32, P, I32, PP]),
     "feasm_matrix_dropzeros": (I32, [P, PP]),
+    "feasm_matrix_free_block": (I32, [P, P, PP]),
     "feasm_linform_dot_device": (I32, [P, I32, I32, P, I32, P]),
     "feasm_pattern_build": (I32, [P]),
     "feasm_mesh_set_numeric_path": (I32, [P, I32]),

@@ -97,6 +97,17 @@ class DeviceMatrix:
         check(lib().feasm_matrix_dropzeros(self.h, ctypes.byref(h)), self.ctx.h)
         return DeviceMatrix(self.ctx, h)
 
+    def free_block(self, is_free):
+        """The principal submatrix on the DOFs flagged free (one flag per column,
+        0-based), renumbered consecutively: matrix_blocked_ff of the
+        free-then-fixed numbering."""
+        f = np.ascontiguousarray(np.asarray(is_free, dtype=np.uint8).ravel())
+        if f.size != self.dims[1]:
+            raise ValueError("one flag per column")
+        h = ctypes.c_void_p()
+        check(lib().feasm_matrix_free_block(self.h, ptr(f), ctypes.byref(h)), self.ctx.h)
+        return DeviceMatrix(self.ctx, h)
+
     def csc_arrays(self, base=0):
         m, n, nnz = self.dims
         colptr = np.empty(n + 1, np.int64)

@@ -83,8 +83,8 @@ def T10block(Length, Width, Height, nL, nW, nH, orientation="a"):
 
 # ------------------------------------------------------------------ fields
 class NodalField:
-    """values (nnodes x ndn); dofnums filled by numberdofs (node-major, free
-    DOFs only -- essential boundary conditions are not part of this path)."""
+    """values (nnodes x ndn), the fixed flags set by setebc (DOF_KIND_DATA in
+    the reference) and dofnums filled by numberdofs."""
 
     def __init__(self, values):
         v = np.asarray(values, dtype=np.float64)
@@ -92,7 +92,10 @@ class NodalField:
             v = v.reshape(-1, 1)
         self.values = v.copy()
         self.dofnums = np.zeros(v.shape, np.int64)
+        self.fixed = np.zeros(v.shape, bool)
         self.nfreedofs = 0
+        self.nalldofs_ = 0
+        self._entperm_default = True
 
     def ndofs(self):
         return self.values.shape[1]
@@ -101,21 +104,47 @@ class NodalField:
         return self.values.shape[0]
 
 
+def setebc(u, fenids, is_fixed=True, comp=1, val=0.0):
+    """setebc!(self, fenids, is_fixed, comp, val) (src/FieldModule.jl:400-455):
+    1-based node ids and component; a fixed DOF takes the value, a released
+    one is reset to zero."""
+    fenids = np.atleast_1d(np.asarray(fenids, dtype=np.int64))
+    val = np.broadcast_to(np.asarray(val, dtype=np.float64), fenids.shape)
+    if not (1 <= comp <= u.values.shape[1]):
+        raise ValueError("Requested  nonexistent  degree of freedom")
+    if fenids.size and (fenids.max() > u.values.shape[0] or fenids.min() < 1):
+        raise ValueError("Requested nonexistent node")
+    for i, v in zip(fenids, val):
+        u.fixed[i - 1, comp - 1] = bool(is_fixed)
+        u.values[i - 1, comp - 1] = v if is_fixed else 0.0
+    return u
+
+
 def numberdofs(u, entperm=None):
-    """numberdofs! (src/FieldModule.jl:356-377) with every DOF free: entities in
-    the order entperm (default 1:n), components fastest."""
+    """numberdofs! (src/FieldModule.jl:343-377): the free components numbered
+    consecutively (entities in the order entperm, default 1:n; components
+    fastest), then the fixed ones, again consecutively."""
     n, dim = u.values.shape
     order = np.arange(n) if entperm is None else np.asarray(entperm, dtype=np.int64) - 1
+    u._entperm_default = entperm is None or np.array_equal(order, np.arange(n))
     nxt = 1
-    for i in order:
-        for j in range(dim):
-            u.dofnums[i, j] = nxt
-            nxt += 1
-    u.nfreedofs = nxt - 1
+    for want_fixed in (False, True):
+        for i in order:
+            for j in range(dim):
+                if u.fixed[i, j] == want_fixed:
+                    u.dofnums[i, j] = nxt
+                    nxt += 1
+        if not want_fixed:
+            u.nfreedofs = nxt - 1
+    u.nalldofs_ = nxt - 1
     return u
 
 
 def nalldofs(u):
+    return int(u.nalldofs_)
+
+
+def nfreedofs(u):
     return int(u.nfreedofs)
 
 
@@ -176,7 +205,7 @@ class FEMMBase:
     def finite_elements(self):
         return self.integdomain.fes
 
-    def _device_mesh(self, geom, u):
+    def _device_mesh(self, geom, u, nodemajor=False):
         ctx = self.ctx or default_context()
         fes = self.integdomain.fes
         if geom.values.shape[1] != 3:
@@ -193,13 +222,13 @@ class FEMMBase:
             self._xyz = geom.values.copy()
             self._key = key
         ndn = u.ndofs()
-        if self._dofs is None or not np.array_equal(self._dofs, u.dofnums):
-            if np.array_equal(u.dofnums.ravel(), np.arange(1, u.dofnums.size + 1)) and \
-                    nalldofs(u) == u.dofnums.size:
+        dofs = np.arange(1, u.dofnums.size + 1, dtype=np.int64).reshape(u.dofnums.shape) if nodemajor else u.dofnums
+        if self._dofs is None or not np.array_equal(self._dofs, dofs):
+            if np.array_equal(dofs.ravel(), np.arange(1, dofs.size + 1)):
                 self._mesh.set_dofs(ndn)
             else:
-                self._mesh.set_dofs(ndn, u.dofnums, nalldofs(u))
-            self._dofs = u.dofnums.copy()
+                self._mesh.set_dofs(ndn, dofs, nalldofs(u))
+            self._dofs = dofs.copy()
         return self._mesh
 
 
@@ -229,6 +258,22 @@ class SysmatAssemblerSparse:
             self.h = None
 
     __del__ = close
+
+    def eltype(self):
+        return np.float64
+
+
+class SysmatAssemblerFFBlock:
+    """SysmatAssemblerFFBlock(row_nfreedofs) (src/AssemblyModule.jl:1149-1231):
+    bilform_* then returns the free-free block matrix_blocked_ff of the
+    wrapped SysmatAssemblerSparse's matrix, extracted on the device."""
+
+    def __init__(self, row_nfreedofs, col_nfreedofs=None, ctx=None, precision=L.FP64):
+        if col_nfreedofs is not None and col_nfreedofs != row_nfreedofs:
+            raise NotImplementedError("square free-free blocks only")
+        self.nfree = int(row_nfreedofs)
+        self.ctx = ctx or default_context()
+        self.precision = precision
 
     def eltype(self):
         return np.float64
@@ -358,10 +403,28 @@ def _rule(femm):
 
 
 def _run(femm, assembler, geom, u, form, coeff, return_device=False):
-    if not isinstance(assembler, (SysmatAssemblerSparse, SysmatAssemblerSparseSymm)):
-        raise TypeError("the GPU bilinear forms take a SysmatAssemblerSparse or SysmatAssemblerSparseSymm")
-    mesh = femm._device_mesh(geom, u)
+    if not isinstance(assembler, (SysmatAssemblerSparse, SysmatAssemblerSparseSymm, SysmatAssemblerFFBlock)):
+        raise TypeError("the GPU bilinear forms take a SysmatAssemblerSparse, SysmatAssemblerSparseSymm or "
+                        "SysmatAssemblerFFBlock")
     kind, order = _rule(femm)
+    if isinstance(assembler, SysmatAssemblerFFBlock):
+        if assembler.nfree > nalldofs(u):
+            raise ValueError("The ff block has too many rows")
+        if u._entperm_default and assembler.nfree == nfreedofs(u):
+            # free-then-fixed in entity order keeps the free DOFs in node-major
+            # order: assemble node-major (resident path), keep the free ones
+            mesh = femm._device_mesh(geom, u, nodemajor=True)
+            full = mesh.bilform(form, kind, order, coeff, precision=assembler.precision)
+            keep = ~u.fixed.ravel()
+        else:  # the field's own numbering (COO path), its leading block
+            mesh = femm._device_mesh(geom, u)
+            full = mesh.bilform(form, kind, order, coeff, precision=assembler.precision)
+            keep = np.zeros(full.dims[1], bool)
+            keep[: assembler.nfree] = True
+        mat = full.free_block(keep)
+        full.close()
+        return mat if return_device else mat.to_scipy()
+    mesh = femm._device_mesh(geom, u)
     mat = mesh.bilform(form, kind, order, coeff, precision=assembler.precision)
     if isinstance(assembler, SysmatAssemblerSparseSymm):
         full, mat = mat, mat.dropzeros()

@@ -200,6 +200,14 @@ int feasm_matrix_destroy(feasm_matrix *mat);
  * pattern: its makematrix! forms S + transpose(S) of the lower-triangle
  * triples, and sparse + stores no zero result. */
 int feasm_matrix_dropzeros(feasm_matrix *mat, feasm_matrix **out);
+/* The free-free block: matrix_blocked_ff(A, nfreedofs)
+ * (src/MatrixUtilityModule.jl:675-690; SysmatAssemblerFFBlock's makematrix!,
+ * src/AssemblyModule.jl:1149-1231) of the same matrix assembled with
+ * numberdofs!'s free-then-fixed numbering (src/FieldModule.jl:343-377).  That
+ * numbering keeps the free DOFs in their relative order, so the block is the
+ * principal submatrix of mat on the DOFs with is_free[d] != 0 (one flag per
+ * column of mat, 0-based), renumbered consecutively.  A new matrix. */
+int feasm_matrix_free_block(feasm_matrix *mat, const uint8_t *is_free, feasm_matrix **out);
 
 /* ---- generic assembler protocol (SysmatAssemblerSparse) ----------------- */
 /* SysmatAssemblerSparse(0.0, nomatrixresult) (src/AssemblyModule.jl:159-165).

@@ -209,6 +209,42 @@ def linform_dot(etype, conn, xyz, ndn, force, dofnums=None, nalldofs=None, rule=
     return F
 
 
+def numberdofs_ebc(fixed, entperm=None):
+    """numberdofs! (src/FieldModule.jl:343-377): free components numbered
+    first, entities in the order entperm (1-based, default 1:n), components
+    fastest; then the fixed ones.  fixed: nnodes x ndn booleans.  Returns
+    (dofnums 1-based nnodes x ndn, nfreedofs)."""
+    fixed = np.asarray(fixed, dtype=bool)
+    n, dim = fixed.shape
+    order = np.arange(n) if entperm is None else np.asarray(entperm, dtype=np.int64) - 1
+    dof = np.zeros((n, dim), np.int64)
+    nxt = 1
+    nfree = 0
+    for want in (False, True):
+        for i in order:
+            for j in range(dim):
+                if fixed[i, j] == want:
+                    dof[i, j] = nxt
+                    nxt += 1
+        if not want:
+            nfree = nxt - 1
+    return dof, nfree
+
+
+def matrix_blocked_ff(colptr, rowval, nzval, nfree):
+    """matrix_blocked_ff(A, nfree) (src/MatrixUtilityModule.jl:675-690): the
+    leading nfree x nfree block A[1:nfree, 1:nfree] of a 1-based CSC."""
+    cp = [1]
+    rv, nz = [], []
+    for c in range(nfree):
+        for k in range(colptr[c] - 1, colptr[c + 1] - 1):
+            if rowval[k] <= nfree:
+                rv.append(rowval[k])
+                nz.append(nzval[k])
+        cp.append(len(rv) + 1)
+    return np.array(cp, np.int64), np.array(rv, np.int64), np.array(nz)
+
+
 def _csc_out(h, n):
     try:
         nnz = lib().orc_csc_nnz(h)
