@@ -22,9 +22,9 @@ it binds is tested from Python (tests/test_gpu_parity.py).
 module FinEtoolsHIP
 
 using FinEtools
-using FinEtools.AssemblyModule: AbstractSysmatAssembler
-import FinEtools.AssemblyModule: startassembly!, assemble!, makematrix!
-import FinEtools.FEMMBaseModule: bilform_diffusion, bilform_lin_elastic, bilform_dot
+using FinEtools.AssemblyModule: AbstractSysmatAssembler, AbstractSysvecAssembler
+import FinEtools.AssemblyModule: startassembly!, assemble!, makematrix!, makevector!
+import FinEtools.FEMMBaseModule: bilform_diffusion, bilform_lin_elastic, bilform_dot, linform_dot
 using SparseArrays: SparseMatrixCSC, spzeros
 
 const libfeasm = get(ENV, "FEASM_LIB", joinpath(@__DIR__, "..", "feasm", "libfeasm.so"))
@@ -125,10 +125,14 @@ _rule(r::GaussRule) = (RULE_GAUSS, r.dim)   # GaussRule stores order in `dim` (I
 _rule(r::TetRule) = (RULE_TET, r.npts)
 _rule(::Any) = nothing
 
-function _device_mesh(self::SysmatAssemblerSparseHIP, femm, geom, u)
+# device meshes by (element set, geometry, field numbering), shared by all
+# the assemblers of this module
+const _MESHES = Dict{UInt,Ptr{Cvoid}}()
+
+function _mesh_for(femm, geom, u)
     fes = finite_elements(femm)
     key = hash((objectid(fes), objectid(geom), objectid(u), geom.values, u.dofnums))
-    haskey(self._meshes, key) && return self._meshes[key]
+    haskey(_MESHES, key) && return _MESHES[key]
     conn = Int64[c for e in fes.conn for c in e]                 # nelem x npe, row-major
     xyz = Matrix{Float64}(geom.values)                            # nnodes x 3, column-major
     h = Ref{Ptr{Cvoid}}(C_NULL)
@@ -138,19 +142,24 @@ function _device_mesh(self::SysmatAssemblerSparseHIP, femm, geom, u)
     dn = Matrix{Int64}(u.dofnums)
     check(ccall((:feasm_dofs_set, libfeasm), Cint, (Ptr{Cvoid}, Cint, Ptr{Int64}, Int64),
                 h[], size(dn, 2), dn, nalldofs(u)), context())
-    self._meshes[key] = h[]
+    _MESHES[key] = h[]
     return h[]
 end
 
-function _gpu_bilform(self, femm, geom, u, form, coeff::Vector{Float64})
+_device_mesh(self::SysmatAssemblerSparseHIP, femm, geom, u) = _mesh_for(femm, geom, u)
+
+function _gpu_bilform_handle(femm, geom, u, form, coeff::Vector{Float64})
     kind, order = _rule(femm.integdomain.integration_rule)
-    mesh = _device_mesh(self, femm, geom, u)
+    mesh = _mesh_for(femm, geom, u)
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:feasm_bilform, libfeasm), Cint,
                 (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}, Cint, Cint, Ptr{Ptr{Cvoid}}),
                 mesh, form, kind, order, coeff, length(coeff), 0, h), context())
-    return _tocsc(h[])
+    return h[]
 end
+
+_gpu_bilform(self::SysmatAssemblerSparseHIP, femm, geom, u, form, coeff::Vector{Float64}) =
+    _tocsc(_gpu_bilform_handle(femm, geom, u, form, coeff))
 
 # The device path applies to a constant DataCache (its fill function is the
 # constant closure of DataCache(data), DataCacheModule.jl:76-89), the identity
@@ -193,6 +202,139 @@ function bilform_dot(self::FEMM, assembler::SysmatAssemblerSparseHIP, geom::Noda
     return _gpu_bilform(assembler, self, geom, u, FORM_DOT, vec(Matrix{Float64}(c)'))
 end
 
-export SysmatAssemblerSparseHIP
+# ---------------------------------------------------------------- further assemblers
+"""
+    SysmatAssemblerSparseSymmHIP()
+
+In place of `SysmatAssemblerSparseSymm` (src/AssemblyModule.jl:342-583): the
+full symmetric matrix from the GPU element loop, without the entries that sum
+to exactly zero (the pattern the reference's `S + transpose(S)` keeps).
+"""
+struct SysmatAssemblerSparseSymmHIP <: AbstractSysmatAssembler end
+
+"""
+    SysmatAssemblerSparseHRZLumpingSymmHIP()
+
+In place of `SysmatAssemblerSparseHRZLumpingSymm` (src/AssemblyModule.jl:943-1141)
+for `bilform_dot`: the HRZ-lumped diagonal mass computed on the GPU.
+"""
+struct SysmatAssemblerSparseHRZLumpingSymmHIP <: AbstractSysmatAssembler end
+
+"""
+    SysmatAssemblerFFBlockHIP(row_nfreedofs)
+
+In place of `SysmatAssemblerFFBlock` (src/AssemblyModule.jl:1149-1231): the GPU
+matrix in the field's numbering, then its leading free-free block extracted on
+the device (`matrix_blocked_ff`, src/MatrixUtilityModule.jl:675-690).
+"""
+struct SysmatAssemblerFFBlockHIP <: AbstractSysmatAssembler
+    nfree::Int
+end
+
+function _finish(::SysmatAssemblerSparseSymmHIP, h::Ptr{Cvoid}, u)
+    h2 = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:feasm_matrix_dropzeros, libfeasm), Cint, (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}), h, h2), context())
+    ccall((:feasm_matrix_destroy, libfeasm), Cint, (Ptr{Cvoid},), h)
+    return _tocsc(h2[])
+end
+
+function _finish(a::SysmatAssemblerFFBlockHIP, h::Ptr{Cvoid}, u)
+    keep = UInt8[d <= a.nfree ? 1 : 0 for d in 1:nalldofs(u)]
+    h2 = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:feasm_matrix_free_block, libfeasm), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Ptr{Ptr{Cvoid}}),
+                h, keep, h2), context())
+    ccall((:feasm_matrix_destroy, libfeasm), Cint, (Ptr{Cvoid},), h)
+    return _tocsc(h2[])
+end
+
+const _WRAPPED = Union{SysmatAssemblerSparseSymmHIP,SysmatAssemblerFFBlockHIP}
+
+_gpu_bilform(self::_WRAPPED, femm, geom, u, form, coeff::Vector{Float64}) =
+    _finish(self, _gpu_bilform_handle(femm, geom, u, form, coeff), u)
+
+for A in (:SysmatAssemblerSparseSymmHIP, :SysmatAssemblerFFBlockHIP)
+    @eval begin
+        function bilform_diffusion(self::FEMM, assembler::$A, geom::NodalField{FT}, u::NodalField{T},
+                                   cf::DC) where {FEMM<:AbstractFEMM,FT,T,DC<:DataCache}
+            _onpath(self, cf) || error("this assembler needs a constant DataCache, CSys(3) and H8/T10")
+            c = cf(zeros(1, 3), zeros(3, 3), 0, 0)
+            isempty(size(c)) && return _gpu_bilform(assembler, self, geom, u, FORM_DIFF_ISO, [Float64(c)])
+            return _gpu_bilform(assembler, self, geom, u, FORM_DIFF_MAT, vec(Matrix{Float64}(c)'))
+        end
+        function bilform_lin_elastic(self::FEMM, assembler::$A, geom::NodalField{FT}, u::NodalField{T},
+                                     mr::Type{DeforModelRed3D}, cf::DC) where {FEMM<:AbstractFEMM,FT,T,DC<:DataCache}
+            _onpath(self, cf) || error("this assembler needs a constant DataCache, CSys(3) and H8/T10")
+            D = cf(zeros(1, 3), zeros(3, 3), 0, 0)
+            return _gpu_bilform(assembler, self, geom, u, FORM_LIN_ELASTIC, vec(Matrix{Float64}(D)'))
+        end
+        function bilform_dot(self::FEMM, assembler::$A, geom::NodalField{FT}, u::NodalField{T}, cf::DC;
+                             m = 3) where {FEMM<:AbstractFEMM,FT,T,DC<:DataCache}
+            (_onpath(self, cf) && m == 3) || error("this assembler needs a constant DataCache, CSys(3), H8/T10, m = 3")
+            c = cf(zeros(1, 3), zeros(3, 3), 0, 0)
+            return _gpu_bilform(assembler, self, geom, u, FORM_DOT, vec(Matrix{Float64}(c)'))
+        end
+    end
+end
+
+function bilform_dot(self::FEMM, assembler::SysmatAssemblerSparseHRZLumpingSymmHIP, geom::NodalField{FT},
+                     u::NodalField{T}, cf::DC; m = 3) where {FEMM<:AbstractFEMM,FT,T,DC<:DataCache}
+    (_onpath(self, cf) && m == 3) || error("the lumped path needs a constant DataCache, CSys(3), H8/T10, m = 3")
+    c = cf(zeros(1, 3), zeros(3, 3), 0, 0)
+    coeff = vec(Matrix{Float64}(c)')
+    kind, order = _rule(self.integdomain.integration_rule)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:feasm_bilform_dot_lumped, libfeasm), Cint,
+                (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Cint, Ptr{Ptr{Cvoid}}),
+                _mesh_for(self, geom, u), kind, order, coeff, length(coeff), h), context())
+    return _tocsc(h[])
+end
+
+"""
+    SysvecAssemblerHIP()
+
+In place of `SysvecAssembler` (src/AssemblyModule.jl:853-917).  The protocol
+(startassembly!/assemble!/makevector!) keeps the reference's host buffer;
+`linform_dot` with a constant force on H8/T10 solids (m = 3) runs on the GPU.
+"""
+mutable struct SysvecAssemblerHIP <: AbstractSysvecAssembler
+    _buffer::Vector{Float64}
+    _row_nalldofs::Int
+end
+SysvecAssemblerHIP(z::Float64 = 0.0) = SysvecAssemblerHIP(Float64[z], 1)
+
+function startassembly!(self::SysvecAssemblerHIP, row_nalldofs::IT) where {IT<:Integer}
+    self._row_nalldofs = row_nalldofs
+    resize!(self._buffer, row_nalldofs)
+    fill!(self._buffer, 0.0)
+    return self
+end
+
+function assemble!(self::SysvecAssemblerHIP, vec::MV, dofnums::IV) where {MV,IV}
+    for i in eachindex(dofnums)
+        gi = dofnums[i]
+        gi < 1 && error("Row degree of freedom < 1")
+        gi > self._row_nalldofs && error("Row degree of freedom > size")
+        self._buffer[gi] += vec[i]
+    end
+end
+
+makevector!(self::SysvecAssemblerHIP) = deepcopy(self._buffer)
+
+function linform_dot(self::FEMM, assembler::SysvecAssemblerHIP, geom::NodalField{FT}, P::NodalField{T},
+                     f::DC, m) where {FEMM<:AbstractFEMM,FT,T,DC<:DataCache}
+    if !(_onpath(self, f) && m == 3)
+        return invoke(linform_dot, Tuple{AbstractFEMM,AbstractSysvecAssembler,NodalField,NodalField,DataCache,Any},
+                      self, assembler, geom, P, f, m)
+    end
+    force = Vector{Float64}(vec(f(zeros(1, 3), zeros(3, 3), 0, 0)))
+    kind, order = _rule(self.integdomain.integration_rule)
+    startassembly!(assembler, nalldofs(P))
+    check(ccall((:feasm_linform_dot, libfeasm), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Cint, Ptr{Float64}),
+                _mesh_for(self, geom, P), kind, order, force, length(force), assembler._buffer), context())
+    return makevector!(assembler)
+end
+
+export SysmatAssemblerSparseHIP, SysmatAssemblerSparseSymmHIP, SysmatAssemblerSparseHRZLumpingSymmHIP,
+       SysmatAssemblerFFBlockHIP, SysvecAssemblerHIP
 
 end # module
