@@ -375,6 +375,9 @@ extern "C" int feasm_mesh_set_xyz(feasm_mesh *m, const double *xyz_rowmajor) {
     API_TRY
     FEASM_REQUIRE(m && xyz_rowmajor, FEASM_EINVAL, "null argument");
     FEASM_HIP(hipMemcpy(m->xyz->p, xyz_rowmajor, m->nnodes * 3 * sizeof(double), hipMemcpyHostToDevice));
+    for (auto &kv : m->tiles)  // the fused kernel's tiles carry copies of the coordinates
+        if (kv.second) refresh_tile_coords(m, *kv.second);
+    FEASM_HIP(hipStreamSynchronize(m->ctx->stream));
     API_CATCH(m ? m->ctx : nullptr)
 }
 

@@ -168,63 +168,52 @@ __device__ __forceinline__ void block_sum(int k, const uint16_t *__restrict__ it
 }
 
 // Global loads of one tile's LDS image, issued ahead and held in registers.
+// COEF: the tile also brings its mesh's material coefficients (batched
+// launches); otherwise the launch's coefficients are loaded once
+template <bool COEF>
 struct Staged {
-    uint4 c0;    // blob chunk tid (the rest of a larger blob is copied at store time)
-    int32_t hn;  // distinct halo node tid
-    double d;    // material coefficient tid
+    uint4 c0;  // blob chunk tid (the rest of a larger blob is copied at store time)
+};
+template <>
+struct Staged<true> {
+    uint4 c0;
+    double d;  // material coefficient tid
 };
 
-__device__ __forceinline__ void stage_issue(const TileJob &job, const TileHdr &hd, const double *coef, Staged &st) {
-    const int tid = threadIdx.x, nt = blockDim.x;
+template <bool COEF>
+__device__ __forceinline__ void stage_issue(const TileJob &job, const TileHdr &hd, const double *coef,
+                                            Staged<COEF> &st) {
+    const int tid = threadIdx.x;
     const int nch = hd.nbytes >> 4;  // at least one chunk: every tile owns a column
     const uint4 *src = reinterpret_cast<const uint4 *>(job.blob + hd.blob);
-    (void)nt;
     st.c0 = src[min(tid, nch - 1)];
-    st.hn = (hd.NH > 0) ? job.hnodes[hd.x0 + min(tid, hd.NH - 1)] : 0;
-    st.d = coef[min(tid, 35)];
+    if constexpr (COEF) st.d = coef[min(tid, 35)];
 }
 
-// The staged image into LDS: blob, halo coordinates (a second, dependent
-// load), coefficients; clears the tile's sign flag.
-__device__ __forceinline__ void stage_store(const TileJob &job, const TileHdr &hd, const Staged &st,
+// The staged image into LDS (blob with the halo coordinates, material
+// coefficients); clears the tile's sign flag.
+template <bool COEF>
+__device__ __forceinline__ void stage_store(const TileJob &job, const TileHdr &hd, const Staged<COEF> &st,
                                             const TileShape::Layout &L, unsigned char *smem, double *sD,
                                             int *sNeg) {
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int nch = hd.nbytes >> 4, NH = hd.NH;
-    double x0 = 0, x1 = 0, x2 = 0;
-    if (tid < NH) {
-        const double *p = job.xyz + 3 * (int64_t)st.hn;
-        x0 = p[0];
-        x1 = p[1];
-        x2 = p[2];
-    }
+    const int nch = hd.nbytes >> 4;
     uint4 *dst = reinterpret_cast<uint4 *>(smem + L.cbase);
     if (tid < nch) dst[tid] = st.c0;
     const uint4 *src = reinterpret_cast<const uint4 *>(job.blob + hd.blob);
     for (int i = tid + nt; i < nch; i += nt) dst[i] = src[i];
-    if (tid < 36) sD[tid] = st.d;
+    if constexpr (COEF)
+        if (tid < 36) sD[tid] = st.d;
     if (tid == 0) *sNeg = 0;
-    double *sX = reinterpret_cast<double *>(smem + L.X);
-    if (tid < NH) {
-        sX[3 * tid] = x0;
-        sX[3 * tid + 1] = x1;
-        sX[3 * tid + 2] = x2;
-    }
-    for (int i = tid + nt; i < NH; i += nt) {
-        const double *p = job.xyz + 3 * (int64_t)job.hnodes[hd.x0 + i];
-        sX[3 * i] = p[0];
-        sX[3 * i + 1] = p[1];
-        sX[3 * i + 2] = p[2];
-    }
 }
 
 // One tile, phases 1 and 2, with its image already in LDS.  Between the
 // phases the next tile's loads are issued into `st` (when `more`).
-template <int NPE, int NQP, int FORM, int NDN, typename R>
+template <int NPE, int NQP, int FORM, int NDN, typename R, bool COEF>
 __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJob &job, const FusedParams &kp,
                                              const TileHdr &hd, unsigned char *smem, bool more,
                                              const TileJob &njob, const TileHdr &nhd, const double *ncoef,
-                                             Staged &st) {
+                                             Staged<COEF> &st) {
     constexpr bool GRAD = (FORM != FEASM_FORM_DOT);
     constexpr int SWP = form_swp<FORM>();
     const int tid = threadIdx.x;
@@ -448,17 +437,18 @@ __global__ __launch_bounds__(kMaxThreads) void k_fused(TileShape shape, TileJob 
     double *sD = reinterpret_cast<double *>(smem) + NQP * NPE + NQP * DS;
     int *sNeg = reinterpret_cast<int *>(sD + 36);
     load_tables<NPE, NQP>(kp, smem);
+    for (int i = threadIdx.x; i < 36; i += blockDim.x) sD[i] = kp.fp.coef[i];  // one material per launch
     int64_t t = rg.begin;
     TileHdr hd = job.hdr[t];
-    Staged st;
-    stage_issue(job, hd, kp.fp.coef, st);
+    Staged<false> st;
+    stage_issue(job, hd, nullptr, st);
     for (;;) {
         stage_store(job, hd, st, shape.layout(hd.H, hd.P, hd.NH, hd.K, hd.NT, hd.NI), smem, sD, sNeg);
         __syncthreads();
         const int64_t tn = t + rg.step;
         TileHdr hn = hd;
         if (tn < rg.end) hn = job.hdr[tn];
-        tile_compute<NPE, NQP, FORM, NDN, R>(shape, job, kp, hd, smem, tn < rg.end, job, hn, kp.fp.coef, st);
+        tile_compute<NPE, NQP, FORM, NDN, R, false>(shape, job, kp, hd, smem, tn < rg.end, job, hn, nullptr, st);
         if (tn >= rg.end) break;
         t = tn;
         hd = hn;
@@ -490,7 +480,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_fused_batch(const BatchEntry *_
     int64_t t = rg.begin;
     const BatchEntry *E = &entries[tile_entry[t]];
     TileHdr hd = E->job.hdr[t - E->tile0];
-    Staged st;
+    Staged<true> st;
     stage_issue(E->job, hd, E->kp.fp.coef, st);
     for (;;) {
         stage_store(E->job, hd, st, E->shape.layout(hd.H, hd.P, hd.NH, hd.K, hd.NT, hd.NI), smem, sD, sNeg);
@@ -502,8 +492,8 @@ __global__ __launch_bounds__(kMaxThreads) void k_fused_batch(const BatchEntry *_
             En = &entries[tile_entry[tn]];
             hn = En->job.hdr[tn - En->tile0];
         }
-        tile_compute<NPE, NQP, FORM, NDN, R>(E->shape, E->job, E->kp, hd, smem, tn < rg.end, En->job, hn,
-                                             En->kp.fp.coef, st);
+        tile_compute<NPE, NQP, FORM, NDN, R, true>(E->shape, E->job, E->kp, hd, smem, tn < rg.end, En->job, hn,
+                                                   En->kp.fp.coef, st);
         if (tn >= rg.end) break;
         t = tn;
         E = En;
@@ -649,6 +639,32 @@ void launch_batch(feasm_ctx *ctx, const BatchLaunch &b) {
         return batch_by_form<10, 4, double>(ctx, b);
     }
     throw Error(FEASM_EUNSUPPORTED, "batched launch covers H8 2x2x2 Gauss and T10 4-point rules");
+}
+
+namespace {
+
+// one workgroup per tile: the tile's halo coordinates into its blob
+__global__ __launch_bounds__(256) void k_tile_coords(TileShape shape, const TileHdr *__restrict__ hdr,
+                                                     int64_t ntiles, const int32_t *__restrict__ hnodes,
+                                                     const double *__restrict__ xyz, unsigned char *__restrict__ blob) {
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const TileHdr hd = hdr[t];
+        const TileShape::Layout L = shape.layout(hd.H, hd.P, hd.NH, hd.K, hd.NT, hd.NI);
+        double *X = reinterpret_cast<double *>(blob + hd.blob + (L.X - L.cbase));
+        for (int i = threadIdx.x; i < hd.NH * 3; i += blockDim.x)
+            X[i] = xyz[3 * (int64_t)hnodes[hd.x0 + i / 3] + i % 3];
+    }
+}
+
+}  // namespace
+
+void refresh_tile_coords(feasm_mesh *m, TileSet &t) {
+    if (t.ntiles == 0) return;
+    feasm_ctx *ctx = m->ctx;
+    hipLaunchKernelGGL(k_tile_coords, dim3((unsigned)std::min<int64_t>(t.ntiles, 1 << 16)), dim3(256), 0,
+                       ctx->stream, t.shape, static_cast<const TileHdr *>(t.hdr->p), t.ntiles, dp<int32_t>(t.hnodes),
+                       dp<double>(m->xyz), static_cast<unsigned char *>(t.blob->p));
+    FEASM_HIP(hipGetLastError());
 }
 
 bool fused_supported(int npe, int nqp) { return (npe == 8 && nqp == 8) || (npe == 10 && nqp == 4); }

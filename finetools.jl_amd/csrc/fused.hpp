@@ -30,8 +30,9 @@ struct TileShape {
     // H halo elements, P owned nodes, NH distinct nodes of the halo, K the
     // largest number of elements shared by the two nodes of one block, NT
     // block tasks and NI (block, element) items
-    // The tile's metadata [cbase .. lconn] is one contiguous blob, stored in
-    // HBM exactly as it sits in LDS (offsets relative to cbase).
+    // The tile's metadata and halo coordinates [cbase .. X] are one
+    // contiguous blob, stored in HBM exactly as it sits in LDS (offsets
+    // relative to cbase).
     __host__ __device__ Layout layout(int64_t H, int P, int64_t NH, int K, int64_t NT, int64_t NI) const {
         Layout L;
         size_t o = 0;
@@ -52,9 +53,9 @@ struct TileShape {
         o = align16(o + (size_t)NI * 2);
         L.lconn = o;
         o = align16(o + (size_t)H * npe * 2);
-        L.X = o;  // end of the blob
+        L.X = o;  // halo-node coordinates, the blob's last part
         o = align16(o + (size_t)NH * 3 * 8);
-        L.U = o;
+        L.U = o;  // end of the blob
         o = align16(o + (grad ? (size_t)H * ustride() * rbytes : 0));
         L.sgn = o;  // grad forms: one byte per (element, point), 1 where J w < 0; DOT: J w
         o = align16(o + (grad ? (size_t)H * nqp : (size_t)H * nqp * rbytes));
@@ -77,7 +78,7 @@ struct TileHdr {
     int32_t P, H, NH, NT;  // owned columns, halo elements, distinct halo nodes, block tasks
     int32_t K, NI;         // most elements shared by one block's nodes; (block, element) items
     int32_t x0;            // halo nodes: hnodes [x0, x0+NH)
-    int32_t nbytes;        // blob size (a multiple of 16)
+    int32_t nbytes;        // blob size (a multiple of 16), coordinates included
     int64_t blob;          // blob offset in bytes (16-aligned)
 };
 
@@ -89,7 +90,9 @@ struct TileHdr {
 // column c2; kNoPair when not), uint16 item[NI]
 // (h << 8 | index of the column node in e << 4 | index of the row node; a
 // task's k items in ascending element), uint16 lconn[H*npe] (halo
-// connectivity as indices into the tile's hnodes).
+// connectivity as indices into the tile's hnodes), double X[NH][3] (the
+// coordinates of the tile's hnodes; refresh_tile_coords rewrites them when
+// the mesh's coordinates change).
 constexpr uint32_t kNullTask = 0xffffffffu;
 constexpr uint16_t kNoPair = 0xffff;
 // null tasks a tile's wavefront schedule may add (the tile sizing reserves
@@ -107,6 +110,9 @@ struct TileSet {
 
 // max_threads bounds the host threads building the per-tile arrays
 std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int max_threads = 16);
+// rewrites the coordinates in every tile's blob from the mesh's coordinates
+// (on the mesh's context stream)
+void refresh_tile_coords(feasm_mesh *m, TileSet &t);
 
 // one mesh of a batched launch
 struct BatchMesh {

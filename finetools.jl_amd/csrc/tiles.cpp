@@ -241,6 +241,7 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
 
 std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int max_threads) {
     const Pattern &p = *m->pattern;
+    FEASM_HIP(hipStreamSynchronize(m->ctx->stream));  // mesh uploads may still be in flight on it
     HostMesh h;
     h.npe = m->npe;
     h.nn = m->nnodes;
@@ -392,7 +393,7 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int 
         d.K = o.K;
         d.NI = (int32_t)o.item.size();
         d.x0 = (int32_t)nX;
-        d.nbytes = (int32_t)(L.X - L.cbase);
+        d.nbytes = (int32_t)(L.U - L.cbase);
         d.blob = nB;
         nX += o.NH;
         nB += d.nbytes;
@@ -422,6 +423,10 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int 
         put(L.task, o.task.data(), o.task.size() * 4);
         put(L.item, o.item.data(), o.item.size() * 2);
         put(L.lconn, o.lconn.data(), o.lconn.size() * 2);
+        std::vector<double> X((size_t)o.NH * 3);
+        for (int i = 0; i < o.NH; i++)
+            for (int k = 0; k < 3; k++) X[(size_t)3 * i + k] = h.xyz[(size_t)3 * o.hnodes[i] + k];
+        put(L.X, X.data(), X.size() * 8);
         std::copy(o.hnodes.begin(), o.hnodes.end(), hn.begin() + d.x0);
     }
     T->blob = dev_alloc(blob.size());

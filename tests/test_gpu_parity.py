@@ -394,3 +394,29 @@ def test_server_results_same_alone_and_in_the_mix():
             ocp, orv, onz = oracle_csc(et, form, conn, xyz, ndn, coeff)
             assert np.array_equal(alone[j][0], ocp) and np.array_equal(alone[j][1], orv)
             assert rel_fro(alone[j][2], onz) <= tol
+
+
+@pytest.mark.parametrize("etype", [O.H8, O.T10])
+def test_moved_nodes_reassemble_with_the_new_coordinates(ctx, etype):
+    """set_xyz after a fused assembly: the tiles' copies of the coordinates
+    follow, and re-assembly into the same pattern matches the oracle on the
+    moved mesh (fused and staged paths)."""
+    n = (4, 3, 3) if etype == O.H8 else (3, 2, 2)
+    conn, xyz = mesh_for(etype, n, 0.1, 1)
+    _, xyz2 = mesh_for(etype, n, 0.1, 2)  # same topology, other perturbation
+    rule = rule_for(etype)
+    D = D200 if etype == O.H8 else D70
+    for path in ("fused", "staged"):
+        say("moved nodes", etype, "path", path)
+        m = core.DeviceMesh(ctx, etype, conn, xyz)
+        m.set_dofs(3)
+        m.set_numeric_path(path)
+        K = m.bilform(L.LIN_ELASTIC, rule[0], rule[1], D)
+        m.set_xyz(xyz2)
+        m.bilform_into(K, L.LIN_ELASTIC, rule[0], rule[1], D)
+        cp, rv, nz = K.csc_arrays(1)
+        ocp, orv, onz = oracle_csc(etype, L.LIN_ELASTIC, conn, xyz2, 3, D)
+        assert np.array_equal(cp, ocp) and np.array_equal(rv, orv)
+        assert rel_fro(nz, onz) <= FP64_TOL
+        K.close()
+        m.close()
