@@ -80,7 +80,7 @@ int fail(feasm_ctx *ctx, int code, const char *what) {
 TileShape shape_for(feasm_batch *b, const feasm_batch::Item &it) {
     const DevQTable &qt = device_qtable(b->ctx, it.mesh->etype, it.rule, it.order);
     TileShape s = tile_shape_for(b->ctx, it.mesh->npe, qt.nqp, it.fp.form, it.precision);
-    if (b->max_nodes > 0) s.max_nodes = std::min(64, b->max_nodes);
+    if (b->max_nodes > 0) s.max_nodes = std::min(255, b->max_nodes);
     if (b->budget > 0) s.budget = std::min(s.budget, b->budget);
     if (b->threads == 256 || b->threads == 512 || b->threads == 1024) s.threads = b->threads;
     return s;

@@ -528,9 +528,9 @@ extern "C" int feasm_mesh_set_numeric_path(feasm_mesh *m, int path) {
 extern "C" int feasm_mesh_set_tile_params(feasm_mesh *m, int max_nodes, int budget_kib, int threads) {
     API_TRY
     FEASM_REQUIRE(m, FEASM_EINVAL, "null mesh");
-    FEASM_REQUIRE(max_nodes >= 0 && max_nodes <= 64 && budget_kib >= 0 &&
+    FEASM_REQUIRE(max_nodes >= 0 && max_nodes <= 255 && budget_kib >= 0 &&
                       (threads == 0 || threads == 256 || threads == 512 || threads == 1024),
-                  FEASM_EINVAL, "max_nodes 0..64, budget_kib >= 0, threads 0/256/512/1024");
+                  FEASM_EINVAL, "max_nodes 0..255, budget_kib >= 0, threads 0/256/512/1024");
     m->tile_nodes = max_nodes;
     m->tile_kib = budget_kib;
     m->tile_threads = threads;

@@ -88,40 +88,85 @@ __device__ __forceinline__ uint64_t neg_mask(const uint8_t *s) {
 
 // Adds M^e for the node pair (row ar, column ac) of halo element h to M: the
 // sum over the points of s_q u_ar u_ac^T (grad forms; 9 values, or its trace
-// for SWP 1) or of N_ar N_ac J w (DOT).
+// for SWP 1) or of N_ar N_ac J w (DOT).  DIAG: a diagonal block (ar == ac),
+// whose M^e is symmetric: one vector per point is read and, for SWP 9, only
+// the upper triangle (M[0..2], M[4], M[5], M[8]) is formed.
 template <int FORM>
 constexpr int form_swp() {
     return (FORM == FEASM_FORM_LIN_ELASTIC || FORM == kFormElasticCubic || FORM == FEASM_FORM_DIFFUSION_MAT) ? 9 : 1;
 }
 
-template <int NPE, int NQP, int FORM, typename R, bool SIGNED, typename A>
+// The 16-byte chunk type of the scaled-gradient records (see
+// TileShape::uchunks): two doubles or four floats.
+template <typename R>
+struct Chunk;
+template <>
+struct Chunk<double> {
+    using T = double2;
+};
+template <>
+struct Chunk<float> {
+    using T = float4;
+};
+
+// s_q r c^T (SWP 9: the 9 products; DIAG: the upper triangle) or its trace
+template <int SWP, bool SIGNED, bool DIAG, typename R, typename A>
+__device__ __forceinline__ void point_product(bool negative, R c0, R c1, R c2, R r0, R r1, R r2, A *M) {
+    if (SIGNED && negative) {
+        r0 = -r0;
+        r1 = -r1;
+        r2 = -r2;
+    }
+    if (SWP == 9) {
+        M[0] = fma(r0, c0, M[0]); M[1] = fma(r0, c1, M[1]); M[2] = fma(r0, c2, M[2]);
+        if (!DIAG) M[3] = fma(r1, c0, M[3]);
+        M[4] = fma(r1, c1, M[4]); M[5] = fma(r1, c2, M[5]);
+        if (!DIAG) { M[6] = fma(r2, c0, M[6]); M[7] = fma(r2, c1, M[7]); }
+        M[8] = fma(r2, c2, M[8]);
+    } else {
+        M[0] = fma(r0, c0, M[0]);
+        M[0] = fma(r1, c1, M[0]);
+        M[0] = fma(r2, c2, M[0]);
+    }
+}
+
+template <int NPE, int NQP, int FORM, typename R, bool SIGNED, bool DIAG, typename A>
 __device__ __forceinline__ void elem_block(uint32_t it, const R *__restrict__ sU, const uint8_t *__restrict__ sSg,
                                            const double *__restrict__ sN, const R *__restrict__ sW, A *M) {
-    constexpr int US = NQP * NPE * 3 + 1;  // TileShape::ustride()
     constexpr int SWP = form_swp<FORM>();
     const int h = (int)(it >> 8), ac = (int)((it >> 4) & 15u), ar = (int)(it & 15u);
     if constexpr (FORM != FEASM_FORM_DOT) {
-        const R *U = sU + h * US;
+        using C = typename Chunk<R>::T;
+        constexpr int PX = (sizeof(R) == 8) ? 1 : 2;  // points per xy chunk
+        constexpr int PZ = 2 * PX;                    // points per z chunk
+        constexpr int US = (NQP * NPE * 3 * (int)sizeof(R) / 16 + 1) * 16 / (int)sizeof(R);  // TileShape::ustride()
+        static_assert(NQP % PZ == 0, "points per element must fill the z chunks");
+        const C *U = reinterpret_cast<const C *>(sU + h * US);
+        const C *UZ = U + (NQP / PX) * NPE;
         uint64_t neg = 0;
         if (SIGNED) neg = neg_mask<NQP>(sSg + h * NQP);
 #pragma unroll
-        for (int q = 0; q < NQP; q++) {
-            const R *uc = U + (q * NPE + ac) * 3, *ur = U + (q * NPE + ar) * 3;
-            const R c0 = uc[0], c1 = uc[1], c2 = uc[2];
-            R r0 = ur[0], r1 = ur[1], r2 = ur[2];
-            if (SIGNED && ((neg >> (8 * q)) & 255u)) {
-                r0 = -r0;
-                r1 = -r1;
-                r2 = -r2;
-            }
-            if (SWP == 9) {
-                M[0] = fma(r0, c0, M[0]); M[1] = fma(r0, c1, M[1]); M[2] = fma(r0, c2, M[2]);
-                M[3] = fma(r1, c0, M[3]); M[4] = fma(r1, c1, M[4]); M[5] = fma(r1, c2, M[5]);
-                M[6] = fma(r2, c0, M[6]); M[7] = fma(r2, c1, M[7]); M[8] = fma(r2, c2, M[8]);
-            } else {
-                M[0] = fma(r0, c0, M[0]);
-                M[0] = fma(r1, c1, M[0]);
-                M[0] = fma(r2, c2, M[0]);
+        for (int g = 0; g < NQP / PZ; g++) {
+            const C cz = UZ[g * NPE + ac];
+            const C rz = DIAG ? cz : UZ[g * NPE + ar];
+#pragma unroll
+            for (int x = 0; x < PZ / PX; x++) {
+                const C cxy = U[(g * (PZ / PX) + x) * NPE + ac];
+                const C rxy = DIAG ? cxy : U[(g * (PZ / PX) + x) * NPE + ar];
+                if constexpr (sizeof(R) == 8) {  // one point: (u0, u1), u2 in cz.x / cz.y
+                    const int q = g * 2 + x;
+                    const R c2 = x ? cz.y : cz.x, r2 = x ? rz.y : rz.x;
+                    point_product<SWP, SIGNED, DIAG>(((neg >> (8 * q)) & 255u) != 0, (R)cxy.x, (R)cxy.y, c2,
+                                                     (R)rxy.x, (R)rxy.y, r2, M);
+                } else {  // two points: (u0, u1, u0', u1'), u2 of four points in cz
+                    const int q = g * 4 + x * 2;
+                    const R c2a = x ? cz.z : cz.x, c2b = x ? cz.w : cz.y;
+                    const R r2a = x ? rz.z : rz.x, r2b = x ? rz.w : rz.y;
+                    point_product<SWP, SIGNED, DIAG>(((neg >> (8 * q)) & 255u) != 0, (R)cxy.x, (R)cxy.y, c2a,
+                                                     (R)rxy.x, (R)rxy.y, r2a, M);
+                    point_product<SWP, SIGNED, DIAG>(((neg >> (8 * (q + 1))) & 255u) != 0, (R)cxy.z, (R)cxy.w,
+                                                     c2b, (R)rxy.z, (R)rxy.w, r2b, M);
+                }
             }
         }
     } else {
@@ -138,12 +183,12 @@ __device__ __forceinline__ void elem_block(uint32_t it, const R *__restrict__ sU
 // time.  In FP64 the products go straight into the FP64 sums (element after
 // element, point after point); in mixed precision each element's matrix is
 // formed in FP32 and added in FP64.
-template <int NPE, int NQP, int FORM, typename R, bool SIGNED, int SWP, int ILP>
+template <int NPE, int NQP, int FORM, typename R, bool SIGNED, bool DIAG, int SWP, int ILP>
 __device__ __forceinline__ void block_sum(int k, const uint16_t *__restrict__ items, const R *__restrict__ sU,
                                           const uint8_t *__restrict__ sSg, const double *__restrict__ sN,
                                           const R *__restrict__ sW, double *acc) {
     if constexpr (sizeof(R) == 8) {
-        for (int j = 0; j < k; j++) elem_block<NPE, NQP, FORM, R, SIGNED>(items[j], sU, sSg, sN, sW, acc);
+        for (int j = 0; j < k; j++) elem_block<NPE, NQP, FORM, R, SIGNED, DIAG>(items[j], sU, sSg, sN, sW, acc);
     } else {
         int j = 0;
         for (; ILP == 2 && j + 1 < k; j += 2) {
@@ -151,8 +196,8 @@ __device__ __forceinline__ void block_sum(int k, const uint16_t *__restrict__ it
             R M0[SWP], M1[SWP];
 #pragma unroll
             for (int i = 0; i < SWP; i++) M0[i] = M1[i] = (R)0;
-            elem_block<NPE, NQP, FORM, R, SIGNED>(i0, sU, sSg, sN, sW, M0);
-            elem_block<NPE, NQP, FORM, R, SIGNED>(i1, sU, sSg, sN, sW, M1);
+            elem_block<NPE, NQP, FORM, R, SIGNED, DIAG>(i0, sU, sSg, sN, sW, M0);
+            elem_block<NPE, NQP, FORM, R, SIGNED, DIAG>(i1, sU, sSg, sN, sW, M1);
 #pragma unroll
             for (int i = 0; i < SWP; i++) acc[i] = (acc[i] + (double)M0[i]) + (double)M1[i];
         }
@@ -160,7 +205,7 @@ __device__ __forceinline__ void block_sum(int k, const uint16_t *__restrict__ it
             R M0[SWP];
 #pragma unroll
             for (int i = 0; i < SWP; i++) M0[i] = (R)0;
-            elem_block<NPE, NQP, FORM, R, SIGNED>(items[j], sU, sSg, sN, sW, M0);
+            elem_block<NPE, NQP, FORM, R, SIGNED, DIAG>(items[j], sU, sSg, sN, sW, M0);
 #pragma unroll
             for (int i = 0; i < SWP; i++) acc[i] = acc[i] + (double)M0[i];
         }
@@ -219,7 +264,7 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
     const int tid = threadIdx.x;
     const int kThreads = blockDim.x;
     const int P = hd.P, H = hd.H, NH = hd.NH, NT = hd.NT, K = hd.K, NI = hd.NI;
-    constexpr int US = NQP * NPE * 3 + 1;  // TileShape::ustride()
+    constexpr int US = (NQP * NPE * 3 * (int)sizeof(R) / 16 + 1) * 16 / (int)sizeof(R);  // TileShape::ustride()
     // two elements in flight where the registers allow it under 1024 threads
     constexpr int ILP = 2;
     constexpr int DS = NPE * 3 + 1;  // TileShape::dstride()
@@ -278,7 +323,10 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
             R ivr[9];
 #pragma unroll
             for (int i = 0; i < 9; i++) ivr[i] = (R)iv[i];
-            R *u = sU + (size_t)h * US + q * NPE * 3;
+            // the element's record of 16-byte chunks (TileShape::uchunks):
+            // (u0, u1) of point q in xy chunk q / PX, u2 in z chunk q / PZ
+            constexpr int PX = (sizeof(R) == 8) ? 1 : 2, PZ = 2 * PX, CR = 16 / (int)sizeof(R);
+            R *urec = sU + (size_t)h * US;
             // node order rotated by the point, so that the lanes of one
             // element store into different LDS banks
 #pragma unroll
@@ -286,9 +334,15 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
                 const int a = (i + q) % NPE;
                 const double *d = sDN + q * DS + a * 3;
                 const R d0 = (R)d[0], d1 = (R)d[1], d2 = (R)d[2];
-                u[3 * a + 0] = r * (d0 * ivr[0] + d1 * ivr[3] + d2 * ivr[6]);
-                u[3 * a + 1] = r * (d0 * ivr[1] + d1 * ivr[4] + d2 * ivr[7]);
-                u[3 * a + 2] = r * (d0 * ivr[2] + d1 * ivr[5] + d2 * ivr[8]);
+                const R u0 = r * (d0 * ivr[0] + d1 * ivr[3] + d2 * ivr[6]);
+                const R u1 = r * (d0 * ivr[1] + d1 * ivr[4] + d2 * ivr[7]);
+                const R u2 = r * (d0 * ivr[2] + d1 * ivr[5] + d2 * ivr[8]);
+                const int cxy = (q / PX) * NPE + a, cz = (NQP / PX) * NPE + (q / PZ) * NPE + a;
+                if constexpr (sizeof(R) == 8)
+                    reinterpret_cast<double2 *>(urec)[cxy] = make_double2(u0, u1);
+                else
+                    reinterpret_cast<float2 *>(urec)[cxy * 2 + (q % PX)] = make_float2(u0, u1);
+                urec[cz * CR + (q % PZ)] = u2;
             }
         } else {
             sW[h * NQP + q] = (R)wj;
@@ -309,14 +363,27 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
         const uint32_t th = sTask[t];
         if (th == kNullTask) continue;
         const int c = (int)(th & 255u), r = (int)((th >> 8) & 255u), k = (int)((th >> 16) & 255u);
-        const bool sw = (th >> 24) & 1u, diag = (th >> 25) & 1u;
+        const bool sw = (th >> 24) & 1u, diag = (th >> 25) & 1u, sym = (th >> 26) & 1u;
         double acc[SWP];
 #pragma unroll
         for (int i = 0; i < SWP; i++) acc[i] = 0.0;
-        if (tile_neg)
-            block_sum<NPE, NQP, FORM, R, true, SWP, ILP>(k, sItem + sToff[t], sU, sSg, sN, sW, acc);
-        else
-            block_sum<NPE, NQP, FORM, R, false, SWP, ILP>(k, sItem + sToff[t], sU, sSg, sN, sW, acc);
+        const uint16_t *items = sItem + sToff[t];
+        if (GRAD && sym) {  // symmetric diagonal blocks sit in wavefronts of their own
+            if (tile_neg)
+                block_sum<NPE, NQP, FORM, R, true, true, SWP, ILP>(k, items, sU, sSg, sN, sW, acc);
+            else
+                block_sum<NPE, NQP, FORM, R, false, true, SWP, ILP>(k, items, sU, sSg, sN, sW, acc);
+            if constexpr (SWP == 9) {  // the lower triangle of the symmetric sum
+                acc[3] = acc[1];
+                acc[6] = acc[2];
+                acc[7] = acc[5];
+            }
+        } else {
+            if (tile_neg)
+                block_sum<NPE, NQP, FORM, R, true, false, SWP, ILP>(k, items, sU, sSg, sN, sW, acc);
+            else
+                block_sum<NPE, NQP, FORM, R, false, false, SWP, ILP>(k, items, sU, sSg, sN, sW, acc);
+        }
         double out[NDN * NDN];
         if constexpr (SWP == 9) {
             // contract in the orientation of the lower node number

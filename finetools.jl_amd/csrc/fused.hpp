@@ -16,10 +16,15 @@ struct TileShape {
     int threads = 256;  // workgroup size of the fused kernel (256, 512 or 1024)
     size_t budget = 0;  // bytes of LDS available to one workgroup
 
-    // doubles between consecutive halo elements' scaled-gradient records:
-    // nqp*npe*3 plus one, so that records of different elements start in
-    // different LDS banks
-    __host__ __device__ int ustride() const { return nqp * npe * 3 + 1; }
+    // A halo element's scaled gradients u (3 values per point and node) sit
+    // in LDS as 16-byte chunks, so that the block sums read them with single
+    // 16-byte loads: G = 16 / (2 rbytes) points per "xy" chunk (u0, u1 of G
+    // points), 2G points per "z" chunk (u2 of 2G points); nqp*npe*3*rbytes/16
+    // chunks per record.  Consecutive records are one chunk further apart, so
+    // that they start in different LDS banks.  (nqp a multiple of 2G.)
+    __host__ __device__ int uchunks() const { return nqp * npe * 3 * rbytes / 16 + 1; }
+    // the record stride in elements of the arithmetic type
+    __host__ __device__ int ustride() const { return uchunks() * 16 / rbytes; }
 
     struct Layout {
         size_t tab, X, lconn, cbase, cdeg, task, toff, tpair, item, U, sgn, total;
@@ -84,7 +89,8 @@ struct TileHdr {
 
 // Blob contents (see TileShape::layout): int64 cbase[P] (first node block of
 // each owned column), int32 cdeg[P] (its node blocks), uint32 task[NT] (c |
-// r << 8 | k << 16 | (row > column) << 24 | (row == column) << 25, or
+// r << 8 | k << 16 | (row > column) << 24 | (row == column) << 25 | (a
+// diagonal block none of whose elements repeats the node) << 26, or
 // kNullTask), uint16 toff[NT] (first item of the task), uint16 tpair[NT] (c2 |
 // r2 << 8: the task also writes the transposed block, row r2 of owned
 // column c2; kNoPair when not), uint16 item[NI]

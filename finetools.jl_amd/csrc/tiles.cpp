@@ -174,8 +174,20 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
         }
         tasks.erase(std::remove_if(tasks.begin(), tasks.end(), [](const Task &T) { return T.drop; }), tasks.end());
     }
-    std::stable_sort(tasks.begin(), tasks.end(),
-                     [](const Task &x, const Task &y) { return x.it.size() > y.it.size(); });
+    // A diagonal block whose items all pair a node with itself (no element
+    // repeats the node) is a sum of symmetric products: the kernel forms its
+    // upper triangle from one vector per point.
+    auto symmetric = [](const Task &T) {
+        if (!T.diag) return false;
+        for (uint16_t v : T.it)
+            if (((v >> 4) & 15u) != (v & 15u)) return false;
+        return true;
+    };
+    // longest first; among equal counts the symmetric diagonal blocks first,
+    // so that they fill wavefronts of their own
+    std::stable_sort(tasks.begin(), tasks.end(), [&](const Task &x, const Task &y) {
+        return x.it.size() != y.it.size() ? x.it.size() > y.it.size() : (symmetric(x) && !symmetric(y));
+    });
     const int K = tasks.empty() ? 0 : (int)tasks.front().it.size();
     FEASM_REQUIRE(K <= 255, FEASM_EUNSUPPORTED, "a node block is shared by more than 255 elements");
     o.K = K;
@@ -223,7 +235,8 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
                 FEASM_REQUIRE(o.item.size() + T.it.size() <= 65535, FEASM_EUNSUPPORTED,
                               "tile holds too many (block, element) items");
                 o.task.push_back((uint32_t)T.c | ((uint32_t)T.r << 8) | ((uint32_t)T.it.size() << 16) |
-                                 ((uint32_t)T.sw << 24) | ((uint32_t)T.diag << 25));
+                                 ((uint32_t)T.sw << 24) | ((uint32_t)T.diag << 25) |
+                                 ((uint32_t)symmetric(T) << 26));
                 o.toff.push_back((uint16_t)o.item.size());
                 o.tpair.push_back(T.c2 < 0 ? kNoPair : (uint16_t)(T.c2 | (T.r2 << 8)));
                 o.item.insert(o.item.end(), T.it.begin(), T.it.end());

@@ -150,7 +150,7 @@ int feasm_bilform_coo(feasm_mesh *mesh, int form, int rule, int order, const dou
  * pattern; values agree to rounding. */
 int feasm_mesh_set_numeric_path(feasm_mesh *mesh, int path);
 /* Tile parameters of the fused kernel for this mesh (0 = default):
- * owned nodes per tile (<= 64), LDS per workgroup in KiB, threads (256/512/1024). */
+ * owned nodes per tile (<= 255), LDS per workgroup in KiB, threads (256/512/1024). */
 int feasm_mesh_set_tile_params(feasm_mesh *mesh, int max_nodes, int budget_kib, int threads);
 /* Tiles built for the fused kernel so far (summed over forms/precisions). */
 int feasm_mesh_tile_info(feasm_mesh *mesh, int64_t *ntiles, int64_t *nhalo);
@@ -281,7 +281,7 @@ int feasm_batch_add(feasm_batch *b, feasm_mesh *mesh, int form, int rule, int or
                     int ncoeff, int precision);
 /* Build patterns, tiles and launch tables; use_graph != 0 captures the
  * launches in a hipGraph replayed by feasm_batch_run.  max_nodes (owned
- * nodes per tile, <= 64), budget_kib (LDS per workgroup) and threads (256, 512
+ * nodes per tile, <= 255), budget_kib (LDS per workgroup) and threads (256, 512
  * or 1024 per workgroup) tune the tiles; 0 picks the defaults. */
 int feasm_batch_prepare(feasm_batch *b, int use_graph, int max_nodes, int budget_kib, int threads);
 /* Enqueue one batched assembly on the context stream (asynchronous). */
