@@ -187,8 +187,14 @@ template <int NPE, int NQP, int FORM, typename R, bool SIGNED, bool DIAG, int SW
 __device__ __forceinline__ void block_sum(int k, const uint16_t *__restrict__ items, const R *__restrict__ sU,
                                           const uint8_t *__restrict__ sSg, const double *__restrict__ sN,
                                           const R *__restrict__ sW, double *acc) {
+    // the next item's index is read while the current one is summed
     if constexpr (sizeof(R) == 8) {
-        for (int j = 0; j < k; j++) elem_block<NPE, NQP, FORM, R, SIGNED, DIAG>(items[j], sU, sSg, sN, sW, acc);
+        uint32_t it = items[0];  // k >= 1: every block of the pattern has an element
+        for (int j = 0; j < k; j++) {
+            const uint32_t nx = items[min(j + 1, k - 1)];
+            elem_block<NPE, NQP, FORM, R, SIGNED, DIAG>(it, sU, sSg, sN, sW, acc);
+            it = nx;
+        }
     } else {
         int j = 0;
         for (; ILP == 2 && j + 1 < k; j += 2) {
@@ -267,20 +273,18 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
     constexpr int US = (NQP * NPE * 3 * (int)sizeof(R) / 16 + 1) * 16 / (int)sizeof(R);  // TileShape::ustride()
     // two elements in flight where the registers allow it under 1024 threads
     constexpr int ILP = 2;
-    constexpr int DS = NPE * 3 + 1;  // TileShape::dstride()
+    constexpr int DS = NPE * 3;  // TileShape::dstride()
     const TileShape::Layout L = shape.layout(H, P, NH, K, NT, NI);
     double *sN = reinterpret_cast<double *>(smem + L.tab);
     double *sDN = sN + NQP * NPE;
     double *sD = sDN + NQP * DS;  // material coefficients (read as broadcasts)
     int *sNeg = reinterpret_cast<int *>(sD + 36);  // set when J w < 0 somewhere in the tile
-    uint32_t *sTask = reinterpret_cast<uint32_t *>(smem + L.task);
+    const uint64_t *sTask = reinterpret_cast<const uint64_t *>(smem + L.task);
     uint16_t *sItem = reinterpret_cast<uint16_t *>(smem + L.item);
     double *sX = reinterpret_cast<double *>(smem + L.X);
     uint16_t *sLc = reinterpret_cast<uint16_t *>(smem + L.lconn);
     int64_t *sCb = reinterpret_cast<int64_t *>(smem + L.cbase);
     int32_t *sCd = reinterpret_cast<int32_t *>(smem + L.cdeg);
-    uint16_t *sToff = reinterpret_cast<uint16_t *>(smem + L.toff);
-    uint16_t *sPair = reinterpret_cast<uint16_t *>(smem + L.tpair);
     R *sU = reinterpret_cast<R *>(smem + L.U);
     uint8_t *sSg = reinterpret_cast<uint8_t *>(smem + L.sgn);
     R *sW = reinterpret_cast<R *>(smem + L.sgn);
@@ -289,14 +293,25 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
     for (int task = tid; task < H * NQP; task += kThreads) {
         const int h = task / NQP, q = task % NQP;
         const uint16_t *cn = sLc + h * NPE;
+        // coordinates: (x, y) pairs then z values; gradient table: (d0, d1)
+        // pairs then d2 values; nodes visited in an order rotated by the
+        // point, so that the reads are single 16- and 8-byte loads and the
+        // lanes of one element fall into different banks
+        const double2 *sXY = reinterpret_cast<const double2 *>(sX);
+        const double *sZ = sX + 2 * NH;
+        const double2 *dxy = reinterpret_cast<const double2 *>(sDN) + q * NPE;
+        const double *dz = sDN + 2 * NQP * NPE + q * NPE;
         double J[9];
 #pragma unroll
         for (int i = 0; i < 9; i++) J[i] = 0.0;
 #pragma unroll
-        for (int a = 0; a < NPE; a++) {
-            const double *x = sX + (int)cn[a] * 3;
-            const double x0 = x[0], x1 = x[1], x2 = x[2];
-            const double d0 = sDN[q * DS + a * 3], d1 = sDN[q * DS + a * 3 + 1], d2 = sDN[q * DS + a * 3 + 2];
+        for (int i = 0; i < NPE; i++) {
+            const int a = (i + q) % NPE;
+            const int n = (int)cn[a];
+            const double2 xy = sXY[n];
+            const double x0 = xy.x, x1 = xy.y, x2 = sZ[n];
+            const double2 dd = dxy[a];
+            const double d0 = dd.x, d1 = dd.y, d2 = dz[a];
             J[0] += x0 * d0; J[1] += x0 * d1; J[2] += x0 * d2;
             J[3] += x1 * d0; J[4] += x1 * d1; J[5] += x1 * d2;
             J[6] += x2 * d0; J[7] += x2 * d1; J[8] += x2 * d2;
@@ -332,8 +347,8 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
 #pragma unroll
             for (int i = 0; i < NPE; i++) {
                 const int a = (i + q) % NPE;
-                const double *d = sDN + q * DS + a * 3;
-                const R d0 = (R)d[0], d1 = (R)d[1], d2 = (R)d[2];
+                const double2 dd = dxy[a];
+                const R d0 = (R)dd.x, d1 = (R)dd.y, d2 = (R)dz[a];
                 const R u0 = r * (d0 * ivr[0] + d1 * ivr[3] + d2 * ivr[6]);
                 const R u1 = r * (d0 * ivr[1] + d1 * ivr[4] + d2 * ivr[7]);
                 const R u2 = r * (d0 * ivr[2] + d1 * ivr[5] + d2 * ivr[8]);
@@ -360,14 +375,15 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
     // elements, ordered sum, material, one write
     double *__restrict__ nzval = job.nzval;
     for (int t = tid; t < NT; t += kThreads) {
-        const uint32_t th = sTask[t];
+        const uint64_t rec = sTask[t];
+        const uint32_t th = (uint32_t)rec;
         if (th == kNullTask) continue;
         const int c = (int)(th & 255u), r = (int)((th >> 8) & 255u), k = (int)((th >> 16) & 255u);
         const bool sw = (th >> 24) & 1u, diag = (th >> 25) & 1u, sym = (th >> 26) & 1u;
         double acc[SWP];
 #pragma unroll
         for (int i = 0; i < SWP; i++) acc[i] = 0.0;
-        const uint16_t *items = sItem + sToff[t];
+        const uint16_t *items = sItem + (uint32_t)((rec >> 32) & 0xffffu);
         if (GRAD && sym) {  // symmetric diagonal blocks sit in wavefronts of their own
             if (tile_neg)
                 block_sum<NPE, NQP, FORM, R, true, true, SWP, ILP>(k, items, sU, sSg, sN, sW, acc);
@@ -452,7 +468,7 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
         for (int l = 0; l < NDN; l++)
 #pragma unroll
             for (int a = 0; a < NDN; a++) nzval[base + (int64_t)l * NDN * deg + a] = out[NDN * a + l];
-        const uint32_t pr = sPair[t];
+        const uint32_t pr = (uint32_t)(rec >> 48);
         if (pr != kNoPair) {  // the transposed block (row c, column m), in its own column
             const int c2 = (int)(pr & 255u), r2 = (int)(pr >> 8);
             const int deg2 = sCd[c2];
@@ -473,12 +489,14 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
 // The tables of the element family, once per workgroup.
 template <int NPE, int NQP>
 __device__ __forceinline__ void load_tables(const FusedParams &kp, unsigned char *smem) {
-    constexpr int DS = NPE * 3 + 1;
     double *sN = reinterpret_cast<double *>(smem);  // the layout's tab region starts at 0
-    double *sDN = sN + NQP * NPE;
+    double *sDN = sN + NQP * NPE;  // (d0, d1) pairs [q][a], then d2 [q][a]
     for (int i = threadIdx.x; i < NQP * NPE; i += blockDim.x) sN[i] = kp.N[i];
-    for (int i = threadIdx.x; i < NQP * NPE * 3; i += blockDim.x)
-        sDN[(i / (NPE * 3)) * DS + i % (NPE * 3)] = kp.dN[i];
+    for (int i = threadIdx.x; i < NQP * NPE; i += blockDim.x) {
+        sDN[2 * i] = kp.dN[3 * i];
+        sDN[2 * i + 1] = kp.dN[3 * i + 1];
+        sDN[2 * NQP * NPE + i] = kp.dN[3 * i + 2];
+    }
 }
 
 // XCD-aware persistent walk: consecutive workgroups land on different XCDs,
@@ -500,7 +518,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_fused(TileShape shape, TileJob 
     extern __shared__ __align__(16) unsigned char smem[];
     const TileRange rg = xcd_range(ntiles);
     if (rg.begin >= rg.end) return;
-    constexpr int DS = NPE * 3 + 1;
+    constexpr int DS = NPE * 3;  // TileShape::dstride()
     double *sD = reinterpret_cast<double *>(smem) + NQP * NPE + NQP * DS;
     int *sNeg = reinterpret_cast<int *>(sD + 36);
     load_tables<NPE, NQP>(kp, smem);
@@ -540,7 +558,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_fused_batch(const BatchEntry *_
     extern __shared__ __align__(16) unsigned char smem[];
     const TileRange rg = xcd_range(ntiles);
     if (rg.begin >= rg.end) return;
-    constexpr int DS = NPE * 3 + 1;
+    constexpr int DS = NPE * 3;  // TileShape::dstride()
     double *sD = reinterpret_cast<double *>(smem) + NQP * NPE + NQP * DS;
     int *sNeg = reinterpret_cast<int *>(sD + 36);
     load_tables<NPE, NQP>(entries[tile_entry[rg.begin]].kp, smem);
@@ -718,8 +736,12 @@ __global__ __launch_bounds__(256) void k_tile_coords(TileShape shape, const Tile
         const TileHdr hd = hdr[t];
         const TileShape::Layout L = shape.layout(hd.H, hd.P, hd.NH, hd.K, hd.NT, hd.NI);
         double *X = reinterpret_cast<double *>(blob + hd.blob + (L.X - L.cbase));
-        for (int i = threadIdx.x; i < hd.NH * 3; i += blockDim.x)
-            X[i] = xyz[3 * (int64_t)hnodes[hd.x0 + i / 3] + i % 3];
+        for (int i = threadIdx.x; i < hd.NH; i += blockDim.x) {  // (x, y) pairs, then the z values
+            const int64_t n = hnodes[hd.x0 + i];
+            X[2 * i] = xyz[3 * n];
+            X[2 * i + 1] = xyz[3 * n + 1];
+            X[2 * hd.NH + i] = xyz[3 * n + 2];
+        }
     }
 }
 

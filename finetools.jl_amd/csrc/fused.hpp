@@ -27,11 +27,11 @@ struct TileShape {
     __host__ __device__ int ustride() const { return uchunks() * 16 / rbytes; }
 
     struct Layout {
-        size_t tab, X, lconn, cbase, cdeg, task, toff, tpair, item, U, sgn, total;
+        size_t tab, X, lconn, cbase, cdeg, task, item, U, sgn, total;
     };
-    // doubles between the gradient-table rows of consecutive points: npe*3
-    // plus one, for the same reason
-    __host__ __device__ int dstride() const { return npe * 3 + 1; }
+    // doubles of the gradient table per point: (d0, d1) pairs of the npe
+    // nodes in one plane [nqp][npe], d2 in a second plane [nqp][npe]
+    __host__ __device__ int dstride() const { return npe * 3; }
     // H halo elements, P owned nodes, NH distinct nodes of the halo, K the
     // largest number of elements shared by the two nodes of one block, NT
     // block tasks and NI (block, element) items
@@ -41,7 +41,7 @@ struct TileShape {
     __host__ __device__ Layout layout(int64_t H, int P, int64_t NH, int K, int64_t NT, int64_t NI) const {
         Layout L;
         size_t o = 0;
-        L.tab = o;  // N [nqp][npe], dN [nqp][dstride], the 36 material coefficients (FP64), a flag
+        L.tab = o;  // N [nqp][npe], dN (two planes, see dstride), the 36 material coefficients (FP64), a flag
         o = align16(o + ((size_t)nqp * (npe + dstride()) + 36) * 8 + 4);
         L.cbase = o;
         o = align16(o + (size_t)P * 8);
@@ -49,11 +49,7 @@ struct TileShape {
         o = align16(o + (size_t)P * 4);
         (void)K;
         L.task = o;
-        o = align16(o + (size_t)NT * 4);
-        L.toff = o;
-        o = align16(o + (size_t)NT * 2);
-        L.tpair = o;
-        o = align16(o + (size_t)NT * 2);
+        o = align16(o + (size_t)NT * 8);
         L.item = o;
         o = align16(o + (size_t)NI * 2);
         L.lconn = o;
@@ -88,16 +84,16 @@ struct TileHdr {
 };
 
 // Blob contents (see TileShape::layout): int64 cbase[P] (first node block of
-// each owned column), int32 cdeg[P] (its node blocks), uint32 task[NT] (c |
-// r << 8 | k << 16 | (row > column) << 24 | (row == column) << 25 | (a
-// diagonal block none of whose elements repeats the node) << 26, or
-// kNullTask), uint16 toff[NT] (first item of the task), uint16 tpair[NT] (c2 |
-// r2 << 8: the task also writes the transposed block, row r2 of owned
-// column c2; kNoPair when not), uint16 item[NI]
-// (h << 8 | index of the column node in e << 4 | index of the row node; a
-// task's k items in ascending element), uint16 lconn[H*npe] (halo
-// connectivity as indices into the tile's hnodes), double X[NH][3] (the
-// coordinates of the tile's hnodes; refresh_tile_coords rewrites them when
+// each owned column), int32 cdeg[P] (its node blocks), uint64 task[NT] (one
+// record per block task: the task word c | r << 8 | k << 16 | (row > column)
+// << 24 | (row == column) << 25 | (a diagonal block none of whose elements
+// repeats the node) << 26, or kNullTask; bits 32-47 the task's first item;
+// bits 48-63 c2 | r2 << 8 when the task also writes the transposed block, row
+// r2 of owned column c2, kNoPair when not), uint16 item[NI] (h << 8 | index
+// of the column node in e << 4 | index of the row node; a task's k items in
+// ascending element), uint16 lconn[H*npe] (halo connectivity as indices into
+// the tile's hnodes), double X[3*NH] (the coordinates of the tile's hnodes:
+// NH (x, y) pairs, then NH z values; refresh_tile_coords rewrites them when
 // the mesh's coordinates change).
 constexpr uint32_t kNullTask = 0xffffffffu;
 constexpr uint16_t kNoPair = 0xffff;

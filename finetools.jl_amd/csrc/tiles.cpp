@@ -174,6 +174,8 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
         }
         tasks.erase(std::remove_if(tasks.begin(), tasks.end(), [](const Task &T) { return T.drop; }), tasks.end());
     }
+    for (const Task &T : tasks)  // the kernel reads a task's first item unconditionally
+        FEASM_REQUIRE(!T.it.empty(), FEASM_ESTATE, "a pattern block has no element");
     // A diagonal block whose items all pair a node with itself (no element
     // repeats the node) is a sum of symmetric products: the kernel forms its
     // upper triangle from one vector per point.
@@ -431,14 +433,18 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int 
         };
         put(L.cbase, o.cbase.data(), o.cbase.size() * 8);
         put(L.cdeg, o.cdeg.data(), o.cdeg.size() * 4);
-        put(L.toff, o.toff.data(), o.toff.size() * 2);
-        put(L.tpair, o.tpair.data(), o.tpair.size() * 2);
-        put(L.task, o.task.data(), o.task.size() * 4);
+        std::vector<uint64_t> rec(o.task.size());
+        for (size_t t = 0; t < rec.size(); t++)
+            rec[t] = (uint64_t)o.task[t] | ((uint64_t)o.toff[t] << 32) | ((uint64_t)o.tpair[t] << 48);
+        put(L.task, rec.data(), rec.size() * 8);
         put(L.item, o.item.data(), o.item.size() * 2);
         put(L.lconn, o.lconn.data(), o.lconn.size() * 2);
-        std::vector<double> X((size_t)o.NH * 3);
-        for (int i = 0; i < o.NH; i++)
-            for (int k = 0; k < 3; k++) X[(size_t)3 * i + k] = h.xyz[(size_t)3 * o.hnodes[i] + k];
+        std::vector<double> X((size_t)o.NH * 3);  // (x, y) pairs, then the z values
+        for (int i = 0; i < o.NH; i++) {
+            X[(size_t)2 * i] = h.xyz[(size_t)3 * o.hnodes[i]];
+            X[(size_t)2 * i + 1] = h.xyz[(size_t)3 * o.hnodes[i] + 1];
+            X[(size_t)2 * o.NH + i] = h.xyz[(size_t)3 * o.hnodes[i] + 2];
+        }
         put(L.X, X.data(), X.size() * 8);
         std::copy(o.hnodes.begin(), o.hnodes.end(), hn.begin() + d.x0);
     }
