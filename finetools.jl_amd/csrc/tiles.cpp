@@ -400,6 +400,7 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int 
         const TileOut &o = outs[t];
         const TileShape::Layout L =
             shape.layout(o.H, o.P, o.NH, o.K, (int64_t)o.task.size(), (int64_t)o.item.size());
+        FEASM_REQUIRE(L.total <= shape.budget, FEASM_ESTATE, "a tile exceeds the LDS budget it was sized for");
         TileHdr &d = hdr[t];
         d.P = o.P;
         d.H = o.H;
