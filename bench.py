@@ -115,6 +115,41 @@ def bench_vec(feasm, core, n=100, steps=10):
     return out
 
 
+def bench_c2_mixed(feasm, core, D, n=100, steps=10):
+    """C2 in mixed precision (FP32 element arithmetic with FP64 Jacobians,
+    FP64 sums): device time per re-assembly and the relative Frobenius error
+    of the values against the FP64 assembly of the same mesh."""
+    ctx = feasm.Context(0)
+    conn, xyz = core.gen_h8block(1.0, 1.0, 1.0, n, n, n)
+    xyz = core.gen_perturb(xyz, [0, 0, 0], [1, 1, 1], 0.15 / n, 2)
+    ne = conn.shape[0]
+    mesh = core.DeviceMesh(ctx, feasm.H8, conn, xyz)
+    del conn, xyz
+    mesh.set_dofs(3)
+    args = (feasm.LIN_ELASTIC, feasm.RULE_GAUSS, 2, D)
+    K64 = mesh.bilform(*args)
+    Km = mesh.bilform(*args, precision=feasm.MIXED)
+    for _ in range(2):
+        mesh.bilform_into(Km, *args, precision=feasm.MIXED)
+    ctx.set_timing(steps)
+    for _ in range(steps):
+        mesh.bilform_into(Km, *args, precision=feasm.MIXED)
+    ctx.synchronize()
+    ms = float(np.mean(ctx.timings()[:, 0]))
+    ctx.set_timing(0)
+    nz64 = K64.csc_arrays(0)[2]
+    nzm = Km.csc_arrays(0)[2]
+    err = float(np.linalg.norm(nzm - nz64) / np.linalg.norm(nz64))
+    del nz64, nzm
+    out = {"workload": "H8 %d^3 linear elasticity, mixed precision, resident pattern" % n, "ms": ms,
+           "elements_per_s": ne / (ms * 1e-3), "rel_fro_vs_fp64": err}
+    K64.close()
+    Km.close()
+    mesh.close()
+    ctx.close()
+    return out
+
+
 def bench_t10(feasm, core, n=60, steps=5):
     """C3: T10 n^3 x 6 tets, E=70e3 nu=0.33, rho=2.7e-9: FP64 and mixed stiffness
     plus the consistent mass; error of the mixed results against FP64."""
@@ -322,7 +357,7 @@ def main():
     ap.add_argument("--cells", type=int, default=100, help="cells per direction of the H8 block")
     ap.add_argument("--cpu-cells", type=int, default=50, help="cells per direction of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extras", default="", help="comma list of extra workloads: vec,t10,batch,served")
+    ap.add_argument("--extras", default="", help="comma list of extra workloads: mixed,vec,t10,batch,served")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB streaming copy (3 launches) to scale traffic counters")
     a = ap.parse_args()
@@ -421,6 +456,10 @@ def main():
     mesh.close()
     ctx.close()
     extras = [x for x in a.extras.split(",") if x]
+    if "mixed" in extras:
+        log("C2 mixed ...")
+        out["c2_mixed"] = bench_c2_mixed(feasm, core, D, n)
+        out["c2_mixed"]["kernel_ratio_vs_fp64"] = el_ms / out["c2_mixed"]["ms"]
     if "vec" in extras:
         log("load vector ...")
         out["vector"] = bench_vec(feasm, core)

@@ -141,7 +141,7 @@ void feasm::numeric_into_ctx(feasm_ctx *ctx, feasm_mesh *m, const FormParams &fp
     std::shared_ptr<TileSet> tiles;
     if (fused) {
         TileShape shape = tile_shape_for(ctx, m->npe, qt.nqp, fp.form, precision);
-        if (m->tile_nodes > 0) shape.max_nodes = std::min(64, m->tile_nodes);
+        if (m->tile_nodes > 0) shape.max_nodes = std::min(255, m->tile_nodes);
         if (m->tile_kib > 0) shape.budget = std::min(shape.budget, (size_t)m->tile_kib * 1024);
         if (m->tile_threads == 256 || m->tile_threads == 512 || m->tile_threads == 1024)
             shape.threads = m->tile_threads;
