@@ -59,6 +59,30 @@ def test_load_vector_matches_oracle(ctx, case):
     m.close()
 
 
+RULES = [("H8 GaussRule(3,3)", O.H8, (4, 3, 3), 3, (L.RULE_GAUSS, 3)),
+         ("H8 GaussRule(3,1)", O.H8, (4, 3, 3), 2, (L.RULE_GAUSS, 1)),
+         ("T10 TetRule(5)", O.T10, (2, 2, 2), 3, (L.RULE_TET, 5)),
+         ("T10 TetRule(1)", O.T10, (2, 2, 2), 1, (L.RULE_TET, 1))]
+
+
+@pytest.mark.parametrize("case", RULES, ids=[c[0] for c in RULES])
+def test_other_rules_take_the_element_vector_path(ctx, case):
+    """Rules other than H8 2x2x2 and T10 4-point stage whole element vectors;
+    same bar against the oracle run with the same rule."""
+    name, et, n, ndn, rule = case
+    say("case", name)
+    conn, xyz, _ = mesh_for(et, n, 0.1, 13)
+    m = core.DeviceMesh(ctx, et, conn, xyz)
+    m.set_dofs(ndn)
+    f = [1.0, -0.5, 2.0][:ndn]
+    got = m.linform_dot(rule[0], rule[1], f, xyz.shape[0] * ndn)
+    want = O.linform_dot(et, conn, xyz, ndn, f, rule=rule)
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    say(name, "rel err", err)
+    assert err <= 1e-14
+    m.close()
+
+
 def test_general_numbering_and_unused_dofs(ctx):
     conn, xyz, rule = mesh_for(O.H8, (3, 4, 2), 0.1, 5)
     nn = xyz.shape[0]
