@@ -81,6 +81,183 @@ def T10block(Length, Width, Height, nL, nW, nH, orientation="a"):
     return FENodeSet(xyz), FESetT10(conn)
 
 
+class FESetT4(_FESet):
+    """Linear tetrahedra: read by the importers; the device path assembles H8
+    and T10 meshes only (a T4 set raises at assembly)."""
+    NPE = 4
+    ETYPE = -1
+
+
+# ------------------------------------------------------------------ mesh import
+def _fixupdecimal(s):
+    """A NASTRAN number with the exponent letter left out ("-1.66618812195+1")
+    made parseable: an E goes before every sign that follows a digit
+    (src/MeshImportModule.jl:37-49)."""
+    out = []
+    for i, ch in enumerate(s):
+        if ch in "+-" and i > 0 and s[i - 1].isdigit():
+            out.append("E")
+        out.append(ch)
+    return "".join(out)
+
+
+def import_NASTRAN(filename, allocationchunk=None, expectfixedformat=False):
+    """GRID and CTETRA records of a NASTRAN bulk-data file
+    (src/MeshImportModule.jl:69-196): free-format GRID, or large-field fixed
+    format GRID* with its continuation line; CTETRA with 4 or 10 nodes,
+    continued on the next line when the record is cut (fixed format: a
+    72-column record).  Nodes must be numbered serially.  One finite element
+    set per property id, in order of first appearance.  As in the reference
+    the file's last line is not examined (it is ENDDATA).  allocationchunk is
+    accepted for signature compatibility (storage grows as needed)."""
+    del allocationchunk
+    with open(filename) as f:
+        lines = [ln.rstrip("\r\n") for ln in f]
+    nodes, elems = [], []
+    cur = 0
+    while cur < len(lines) - 1:
+        temp = lines[cur]
+        cur += 1
+        if len(temp) >= 4 and temp[:4].upper() == "GRID":
+            fixed, large = expectfixedformat, False
+            if len(temp) >= 5 and temp[:5].upper() == "GRID*":
+                large, fixed = True, True
+            if fixed and not large:
+                raise ValueError("Can handle either free format or large-field fixed format")
+            if fixed:
+                nid = float(_fixupdecimal(temp[8:24]))
+                x = float(_fixupdecimal(temp[40:56]))
+                y = float(_fixupdecimal(temp[56:72]))
+                temp = lines[cur]
+                cur += 1
+                z = float(_fixupdecimal(temp[8:24]))
+                nodes.append((nid, x, y, z))
+            else:
+                a = temp.replace(",", " ").split()
+                nodes.append(tuple(float(a[k]) for k in range(1, 5)))
+        if len(temp) >= 6 and temp[:6].upper() == "CTETRA":
+            continuation = ""
+            fixed = len(temp) == 72 or expectfixedformat
+            if fixed:
+                continuation = temp[min(len(temp), 72) - 1:]
+                temp = temp[:min(len(temp), 72)]
+            a = temp.replace(",", " ").split()
+            eid, pid = int(a[1]), int(a[2])
+            if len(a) == 7:
+                nper = 4
+            else:
+                nper = 10
+                if len(a) < 13:  # the record continues on the next line
+                    temp = lines[cur]
+                    cur += 1
+                    if fixed:
+                        temp = temp[len(continuation):min(len(temp), 72)]
+                    a = a + temp.strip().replace(",", " ").split()
+            elems.append((eid, pid, [int(a[k]) for k in range(3, 3 + nper)]))
+    node = np.array(nodes, dtype=np.float64).reshape(-1, 4)
+    if not np.array_equal(node[:, 0], np.arange(1, node.shape[0] + 1)):
+        raise ValueError("Nodes are not in serial order")
+    npers = sorted({len(e[2]) for e in elems})
+    if len(npers) != 1:
+        raise ValueError("Cannot handle mixture of element types")
+    conn = np.array([e[2] for e in elems], dtype=np.int64)
+    pids = [e[1] for e in elems]
+    upids = list(dict.fromkeys(pids))
+    cls = FESetT4 if npers[0] == 4 else FESetT10
+    fesets = [cls(conn[[i for i, q in enumerate(pids) if q == pid]]) for pid in upids]
+    return {"fens": FENodeSet(node[:, 1:4]), "fesets": fesets, "property_ids": upids}
+
+
+def import_ABAQUS(filename, allocationchunk=None):
+    """*NODE, *ELEMENT, *NSET and *ELSET sections of an Abaqus input file
+    (src/MeshImportModule.jl:236-470): nodes renumbered serially in file
+    order and the connectivity with them; consecutive *ELEMENT sections are
+    read (C3D8 -> H8, C3D10 -> T10, C3D4/DC3D4 -> T4; other types are
+    reported in "warnings", like every type the device path does not
+    assemble); an element line ending in a comma continues on the next line;
+    GENERATE ranges expand.  nsets hold the new node numbers."""
+    del allocationchunk
+    with open(filename) as f:
+        lines = [ln.strip().upper() for ln in f]
+    warnings = []
+    # nodes: the first *NODE section
+    node = []
+    i = 0
+    while i < len(lines):
+        if lines[i][:5] == "*NODE":
+            i += 1
+            while i < len(lines) and lines[i][:1] != "*":
+                node.append([float(v) for v in lines[i].split(",") if v.strip() != ""][:4])
+                i += 1
+            break
+        i += 1
+    node = np.array(node, dtype=np.float64).reshape(-1, 4)
+    ids = node[:, 0].astype(np.int64)
+    newnum = np.zeros(int(ids.max()) + 1, dtype=np.int64)
+    newnum[ids] = np.arange(1, len(ids) + 1)
+    # element sections: consecutive *ELEMENT blocks from the first one
+    sections = []
+    i = 0
+    while i < len(lines) and lines[i][:8] != "*ELEMENT":
+        i += 1
+    while i < len(lines) and lines[i][:8] == "*ELEMENT":
+        header, rows = lines[i], []
+        i += 1
+        while i < len(lines) and lines[i][:1] != "*":
+            a = lines[i].split(",")
+            i += 1
+            if a[-1] == "" and i < len(lines):
+                a = a[:-1] + lines[i].split(",")
+                i += 1
+            rows.append([int(v) for v in a if v.strip() != ""])
+        sections.append((header, rows))
+    fesets = []
+    for header, rows in sections:
+        etype = None
+        for part in header.split(","):
+            kv = part.split("=")
+            if kv[0].strip() == "TYPE" and len(kv) > 1:
+                etype = kv[1].strip()
+        if etype is not None and etype[:4] == "C3D8":
+            cls, n = FESetH8, 8
+        elif etype is not None and etype[:5] == "C3D10":
+            cls, n = FESetT10, 10
+        elif etype is not None and (etype[:4] == "C3D4" or etype[:5] == "DC3D4"):
+            cls, n = FESetT4, 4
+        else:
+            warnings.append("Don't know how to handle " + header)
+            continue
+        conn = np.array([r[1:1 + n] for r in rows], dtype=np.int64).reshape(-1, n)
+        fesets.append(cls(newnum[conn]))
+    # node and element sets
+    def sets(tag, key):
+        out = {}
+        j = 0
+        while j < len(lines):
+            if lines[j][:len(tag)] == tag:
+                name, gen = "", False
+                for part in lines[j].split(","):
+                    kv = [t.strip() for t in part.split("=")]
+                    if kv[0] == key and len(kv) > 1:
+                        name = kv[1]
+                    if kv[0] == "GENERATE":
+                        gen = True
+                vals = []
+                j += 1
+                while j < len(lines) and lines[j][:1] != "*":
+                    a = [int(v) for v in lines[j].split(",") if v.strip() != ""]
+                    vals.extend(range(a[0], a[1] + 1, a[2]) if gen else a)
+                    j += 1
+                out[name] = np.array(vals, dtype=np.int64)
+                continue
+            j += 1
+        return out
+    nsets = {k: newnum[v] for k, v in sets("*NSET", "NSET").items()}
+    elsets = sets("*ELSET", "ELSET")
+    return {"fens": FENodeSet(node[:, 1:4]), "fesets": fesets, "nsets": nsets, "elsets": elsets,
+            "warnings": warnings}
+
+
 # ------------------------------------------------------------------ fields
 class NodalField:
     """values (nnodes x ndn), the fixed flags set by setebc (DOF_KIND_DATA in
