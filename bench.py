@@ -14,6 +14,7 @@ import argparse
 import json
 import os
 import statistics
+import subprocess
 import sys
 import time
 
@@ -47,24 +48,79 @@ def algorithmic_bytes_numeric(npe, ne, nn, nnz):
     return 4 * npe * ne + 24 * nn + 8 * nnz
 
 
-def cpu_baseline(n_cells, seed, D, min_seconds=10.0, max_runs=5):
-    """Serial CPU restatement of the reference's element loop + sparse() timed
-    on a bounded sample of the same workload (elements/s)."""
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak
+
+
+def algorithmic_flops_numeric(npe, nqp, ne, nnz, ndn=3):
+    """FP64 flops of one numeric elasticity assembly in the node-pair block
+    formulation, counted once per element (the fused kernel recomputes the
+    per-point work of its halo elements on top of this):
+    per element and point: J = X^T dN (9 npe FMA), det J by cofactors (14),
+    J w (1), the adjugate inverse scaled by 1/det (36 + 1 division), sqrt|J w|
+    (1), u_a = sqrt|J w| * (dN_a inv J) (6 per component, 3 npe components);
+    per element: the node-pair products sum_q u_m u_c^T over the npe(npe-1)/2
+    off-diagonal pairs (9 FMA per point) and the npe diagonal ones (6 FMA, the
+    symmetric upper triangle); per assembled node block (nnz / ndn^2 of them)
+    the cubic-symmetry material contraction (32)."""
+    per_point = 2 * 9 * npe + 14 + 1 + 37 + 1 + 6 * 3 * npe
+    pairs = npe * (npe - 1) // 2 * nqp * 9 * 2 + npe * nqp * 6 * 2
+    return ne * (nqp * per_point + pairs) + (nnz // (ndn * ndn)) * 32
+
+
+def host_info():
+    """nproc and, where lscpu answers, the CPU model and socket count."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            if key.strip() == "Model name":
+                info["model"] = val.strip()
+            elif key.strip() == "Socket(s)":
+                info["sockets"] = val.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(n_cells, seed, D, runs=5, threads=None):
+    """CPU restatement of the reference's element loop + sparse() timed on a
+    bounded sample of the same workload: one warm-up, then the median of
+    `runs` runs, split into the element loop (COO triples) and sparse().
+    The headline figure is serial (the reference's loop is serial); beside
+    it the same element loop on `threads` OpenMP threads (sparse() serial)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     conn, xyz = O.block_mesh(O.H8, (n_cells,) * 3, L=1.0, amp_frac=0.15, seed=seed)
-    ne = conn.shape[0]
-    times = []
-    t_start = time.perf_counter()
-    while len(times) < max_runs and (len(times) < 3 or time.perf_counter() - t_start < min_seconds):
-        t0 = time.perf_counter()
-        O.assemble_csc(O.H8, O.LIN_ELASTIC, conn, xyz, 3, D.ravel())
-        times.append(time.perf_counter() - t0)
-    t = statistics.median(times)
-    return {"value": ne / t, "unit": "elements/s", "cores": 1, "kind": "port",
-            "sample": "H8 %d^3 = %d elements linear elasticity FP64, element loop + sparse(), median of %d runs "
-                      "(%.2f s each)" % (n_cells, ne, len(times), t)}
+    ne, nn = conn.shape[0], xyz.shape[0]
+    dofnums = np.arange(1, 3 * nn + 1, dtype=np.int64).reshape(nn, 3)
+    coeff = D.ravel()
+
+    def timed(nthreads):
+        loop, spar = [], []
+        for r in range(runs + 1):
+            t0 = time.perf_counter()
+            I, J, V = O.bilform_coo(O.H8, O.LIN_ELASTIC, conn, xyz, 3, dofnums, 3 * nn, coeff, nthreads=nthreads)
+            t1 = time.perf_counter()
+            O.sparse(I, J, V, 3 * nn, 3 * nn)
+            t2 = time.perf_counter()
+            del I, J, V
+            if r:  # the first run warms up
+                loop.append(t1 - t0)
+                spar.append(t2 - t1)
+        tl, ts = statistics.median(loop), statistics.median(spar)
+        return tl, ts, statistics.median([a + b for a, b in zip(loop, spar)])
+
+    tl, ts, tt = timed(1)
+    nt = threads or max(1, min(16, os.cpu_count() or 1))
+    tl2, ts2, tt2 = timed(nt)
+    return {"value": ne / tt, "unit": "elements/s", "cores": 1, "kind": "port",
+            "sample": "H8 %d^3 = %d elements linear elasticity FP64, element loop + sparse(), one warm-up, median of "
+                      "%d runs (%.2f s each)" % (n_cells, ne, runs, tt),
+            "element_loop_s": tl, "sparse_s": ts, "host": host_info(),
+            "threaded": {"value": ne / tt2, "unit": "elements/s", "cores": nt, "element_loop_s": tl2,
+                         "sparse_s": ts2, "note": "OpenMP element loop on %d threads, sparse() serial" % nt}}
 
 
 PROFILE = os.path.join(ROOT, "profiles", "k_fused_h8_elasticity_100cube.md")
@@ -448,8 +504,14 @@ def main():
                      "kernel": "k_fused (fused tile kernel, whole numeric phase)" if ntiles > 0
                      else "k_element + k_gather (staged numeric phase)",
                      "algorithmic_bytes": bytes_alg},
+        "fp64_flops": None,
         "served": None,
     }
+    flops = algorithmic_flops_numeric(8, 8, ne, nnz)
+    out["fp64_flops"] = {"per_element": flops / ne, "achieved_tflops": flops / (kern_ms * 1e-3) / 1e12,
+                         "peak_tflops": FP64_PEAK_TFLOPS,
+                         "frac": flops / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                         "count": "node-pair block formulation, once per element (bench.algorithmic_flops_numeric)"}
     if copy_gbs is not None:
         out["copy_gbs_measured"] = copy_gbs
     del M

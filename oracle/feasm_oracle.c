@@ -457,6 +457,45 @@ int64_t orc_bilform_coo(int etype, int rule, int order, int form, int64_t nelem,
     return p;
 }
 
+/* orc_bilform_coo with the element loop spread over nthreads OpenMP threads
+ * (CPU-baseline timing only).  Every element owns its kd*kd triple slots
+ * [e*kd*kd, (e+1)*kd*kd), so the threads write disjoint ranges and the
+ * triples are the serial function's, in the same order; the reference's loop
+ * is serial, and this variant is the multi-core CPU figure beside it.
+ * Returns the number of triples or the serial function's error codes. */
+int64_t orc_bilform_coo_threads(int etype, int rule, int order, int form, int64_t nelem, const int64_t *conn,
+                                const double *xyz, int ndn, const int64_t *dofnums, int64_t nalldofs,
+                                const double *coeff, int64_t *I, int64_t *J, double *V, int nthreads)
+{
+    orc_idata d;
+    if (orc_idata_make(&d, etype, rule, order) < 0) return -1;
+    const int npe = d.npe, kd = ndn * npe;
+    int err = 0;
+#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(min : err)
+    for (int64_t e = 0; e < nelem; e++) {
+        double X[30], Ke[900];
+        int64_t dofs[30];
+        const int64_t *c = conn + e * npe;
+        for (int a = 0; a < npe; a++)
+            for (int k = 0; k < 3; k++) X[3 * a + k] = xyz[3 * (c[a] - 1) + k];
+        orc_element_matrix(&d, form, ndn, X, coeff, Ke);
+        for (int a = 0; a < npe; a++)
+            for (int k = 0; k < ndn; k++) dofs[a * ndn + k] = dofnums[(c[a] - 1) * ndn + k];
+        int64_t p = e * (int64_t)kd * kd;
+        for (int jj = 0; jj < kd; jj++)
+            for (int ii = 0; ii < kd; ii++) {
+                const int64_t di = dofs[ii], dj = dofs[jj];
+                if (di < 1 || dj < 1) err = err < -2 ? err : -2;
+                else if (di > nalldofs || dj > nalldofs) err = err < -3 ? err : -3;
+                V[p] = Ke[ii + kd * jj];
+                I[p] = di;
+                J[p] = dj;
+                p++;
+            }
+    }
+    return err < 0 ? err : nelem * (int64_t)kd * kd;
+}
+
 /* bilform_dot (src/FEMMBaseModule.jl:1335-1366) into a
  * SysmatAssemblerSparseHRZLumpingSymm (src/AssemblyModule.jl:943-1141): for
  * every element mass matrix, em2 = sum(sum(mat, dims = 1)) (column sums, then

@@ -48,6 +48,9 @@ def lib():
         L.orc_elmat.argtypes = [ctypes.c_int] * 5 + [P, P, P]
         L.orc_bilform_coo.argtypes = [ctypes.c_int] * 4 + [i64, P, P, ctypes.c_int, P, i64, P, P, P, P]
         L.orc_bilform_coo.restype = i64
+        L.orc_bilform_coo_threads.argtypes = ([ctypes.c_int] * 4 + [i64, P, P, ctypes.c_int, P, i64, P, P, P, P]
+                                              + [ctypes.c_int])
+        L.orc_bilform_coo_threads.restype = i64
         L.orc_sparse.argtypes = [i64, P, P, P, i64, i64]
         L.orc_linform_dot.argtypes = [ctypes.c_int] * 3 + [i64, P, P, ctypes.c_int, P, i64, P, P]
         L.orc_hrz_lumped_coo.argtypes = [ctypes.c_int] * 3 + [i64, P, P, ctypes.c_int, P, i64, P, P, P, P]
@@ -165,8 +168,9 @@ def elmat(etype, form, ndn, X, coeff, rule=None):
 
 
 # ---------------------------------------------------------------- assembly
-def bilform_coo(etype, form, conn, xyz, ndn, dofnums, nalldofs, coeff, rule=None):
-    """COO triples (I, J, V; 1-based) exactly as assemble! serializes them."""
+def bilform_coo(etype, form, conn, xyz, ndn, dofnums, nalldofs, coeff, rule=None, nthreads=1):
+    """COO triples (I, J, V; 1-based) exactly as assemble! serializes them
+    (nthreads > 1: the same triples from an OpenMP element loop)."""
     rule, order = rule if rule is not None else rule_for(etype)
     conn, xyz, dofnums = _i64(conn), _f64(xyz), _i64(dofnums)
     c = _f64(np.atleast_1d(coeff))
@@ -175,8 +179,12 @@ def bilform_coo(etype, form, conn, xyz, ndn, dofnums, nalldofs, coeff, rule=None
     I = np.zeros(nt, np.int64)
     J = np.zeros(nt, np.int64)
     V = np.zeros(nt)
-    p = lib().orc_bilform_coo(etype, rule, order, form, ne, _p(conn), _p(xyz), ndn, _p(dofnums),
-                              nalldofs, _p(c), _p(I), _p(J), _p(V))
+    if nthreads > 1:
+        p = lib().orc_bilform_coo_threads(etype, rule, order, form, ne, _p(conn), _p(xyz), ndn, _p(dofnums),
+                                          nalldofs, _p(c), _p(I), _p(J), _p(V), int(nthreads))
+    else:
+        p = lib().orc_bilform_coo(etype, rule, order, form, ne, _p(conn), _p(xyz), ndn, _p(dofnums),
+                                  nalldofs, _p(c), _p(I), _p(J), _p(V))
     if p == -2:
         raise ValueError("Row degree of freedom < 1")
     if p == -3:
