@@ -503,7 +503,10 @@ def main():
                      "traffic": profiled_traffic() if (ntiles > 0 and n == 100) else None,
                      "kernel": "k_fused (fused tile kernel, whole numeric phase)" if ntiles > 0
                      else "k_element + k_gather (staged numeric phase)",
-                     "algorithmic_bytes": bytes_alg},
+                     "algorithmic_bytes": bytes_alg,
+                     # diagnostic: what a COO path would move (every element's 24 x 24
+                     # triples written and read back at 16 bytes each)
+                     "coo_path_bytes": 2 * 16 * (8 * 3) ** 2 * ne},
         "fp64_flops": None,
         "served": None,
     }
