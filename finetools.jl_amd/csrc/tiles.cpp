@@ -20,6 +20,8 @@
 // in the second half of the tile, which keeps tiles box-shaped.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <memory>
 #include <numeric>
 #include <thread>
 
@@ -118,37 +120,62 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
             const int32_t v = h.conn[(int64_t)halo[i] * npe + a];
             o.lconn[(size_t)i * npe + a] = (uint16_t)(std::lower_bound(hn.begin(), hn.end(), v) - hn.begin());
         }
-    // block tasks: items of (column c, row position r) in ascending element
+    // block tasks: items of (column c, row position r) in ascending element,
+    // held in one array (task i's items at [off, off + k)); two passes over
+    // the owned nodes' elements: count, then fill
     struct Task {
         int c, r;
-        bool sw, diag;
-        std::vector<uint16_t> it;
+        bool sw, diag, sym = false;
+        int off = 0, k = 0;   // its items in `its`
         int c2 = -1, r2 = 0;  // the transposed block, when its column is owned too
         bool drop = false;    // computed by its transposed block's task
     };
     std::vector<Task> tasks;
+    std::vector<uint16_t> its;
     std::vector<size_t> tfirst(P);
     o.cbase.resize(P);
     o.cdeg.resize(P);
-    for (int c = 0; c < P; c++) {
-        const int32_t node = owned[c];
-        const int64_t b0 = h.adj_ptr[node], deg = h.adj_ptr[node + 1] - b0;
-        FEASM_REQUIRE(deg <= 255, FEASM_EUNSUPPORTED, "a node has more than 255 neighbours");
-        o.cbase[c] = b0;
-        o.cdeg[c] = (int32_t)deg;
-        const int32_t *rows = h.adj.data() + b0;
-        const size_t tb = tasks.size();
-        tfirst[c] = tb;
-        for (int r = 0; r < deg; r++) tasks.push_back(Task{c, r, rows[r] > node, rows[r] == node, {}});
-        for (int64_t s = h.n2e_ptr[node]; s < h.n2e_ptr[node + 1]; s++) {
-            const int32_t e = (int32_t)(h.n2e[s] >> 4);
-            const int ac = (int)(h.n2e[s] & 15u);
-            const int hl = (int)(std::lower_bound(halo.begin(), halo.end(), e) - halo.begin());
-            for (int a = 0; a < npe; a++) {
-                const int32_t v = h.conn[(int64_t)e * npe + a];
-                const int r = (int)(std::lower_bound(rows, rows + deg, v) - rows);
-                FEASM_REQUIRE(r < deg && rows[r] == v, FEASM_ESTATE, "pattern misses an element node pair");
-                tasks[tb + r].it.push_back((uint16_t)((hl << 8) | (ac << 4) | a));
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) {  // offsets from the counts; the counts restart as fill positions
+            int off = 0;
+            for (Task &T : tasks) {
+                T.off = off;
+                off += T.k;
+                T.k = 0;
+            }
+            its.resize(off);
+        }
+        for (int c = 0; c < P; c++) {
+            const int32_t node = owned[c];
+            const int64_t b0 = h.adj_ptr[node], deg = h.adj_ptr[node + 1] - b0;
+            const int32_t *rows = h.adj.data() + b0;
+            if (pass == 0) {
+                FEASM_REQUIRE(deg <= 255, FEASM_EUNSUPPORTED, "a node has more than 255 neighbours");
+                o.cbase[c] = b0;
+                o.cdeg[c] = (int32_t)deg;
+                tfirst[c] = tasks.size();
+                for (int r = 0; r < deg; r++) {
+                    Task T;
+                    T.c = c;
+                    T.r = r;
+                    T.sw = rows[r] > node;
+                    T.diag = rows[r] == node;
+                    tasks.push_back(T);
+                }
+            }
+            const size_t tb = tfirst[c];
+            for (int64_t s = h.n2e_ptr[node]; s < h.n2e_ptr[node + 1]; s++) {
+                const int32_t e = (int32_t)(h.n2e[s] >> 4);
+                const int ac = (int)(h.n2e[s] & 15u);
+                const int hl = (int)(std::lower_bound(halo.begin(), halo.end(), e) - halo.begin());
+                for (int a = 0; a < npe; a++) {
+                    const int32_t v = h.conn[(int64_t)e * npe + a];
+                    const int r = (int)(std::lower_bound(rows, rows + deg, v) - rows);
+                    FEASM_REQUIRE(r < deg && rows[r] == v, FEASM_ESTATE, "pattern misses an element node pair");
+                    Task &T = tasks[tb + r];
+                    if (pass == 1) its[T.off + T.k] = (uint16_t)((hl << 8) | (ac << 4) | a);
+                    T.k++;
+                }
             }
         }
     }
@@ -175,22 +202,21 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
         tasks.erase(std::remove_if(tasks.begin(), tasks.end(), [](const Task &T) { return T.drop; }), tasks.end());
     }
     for (const Task &T : tasks)  // the kernel reads a task's first item unconditionally
-        FEASM_REQUIRE(!T.it.empty(), FEASM_ESTATE, "a pattern block has no element");
+        FEASM_REQUIRE(T.k > 0, FEASM_ESTATE, "a pattern block has no element");
     // A diagonal block whose items all pair a node with itself (no element
     // repeats the node) is a sum of symmetric products: the kernel forms its
     // upper triangle from one vector per point.
-    auto symmetric = [](const Task &T) {
-        if (!T.diag) return false;
-        for (uint16_t v : T.it)
-            if (((v >> 4) & 15u) != (v & 15u)) return false;
-        return true;
-    };
+    for (Task &T : tasks) {
+        T.sym = T.diag;
+        for (int j = 0; j < T.k; j++)
+            if (((its[T.off + j] >> 4) & 15u) != (its[T.off + j] & 15u)) T.sym = false;
+    }
     // longest first; among equal counts the symmetric diagonal blocks first,
     // so that they fill wavefronts of their own
-    std::stable_sort(tasks.begin(), tasks.end(), [&](const Task &x, const Task &y) {
-        return x.it.size() != y.it.size() ? x.it.size() > y.it.size() : (symmetric(x) && !symmetric(y));
+    std::stable_sort(tasks.begin(), tasks.end(), [](const Task &x, const Task &y) {
+        return x.k != y.k ? x.k > y.k : (x.sym && !y.sym);
     });
-    const int K = tasks.empty() ? 0 : (int)tasks.front().it.size();
+    const int K = tasks.empty() ? 0 : tasks.front().k;
     FEASM_REQUIRE(K <= 255, FEASM_EUNSUPPORTED, "a node block is shared by more than 255 elements");
     o.K = K;
     // Wavefront schedule: 64 consecutive tasks run as one wavefront, and the
@@ -203,7 +229,7 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
     {
         std::vector<double> load(NS, 0.0);
         for (size_t c0 = 0; c0 < tasks.size(); c0 += W) {
-            const double cost = (double)tasks[c0].it.size() + 2.0;
+            const double cost = (double)tasks[c0].k + 2.0;
             const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
             queue[q].push_back((int)c0);
             load[q] += cost;
@@ -234,14 +260,14 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
                     continue;
                 }
                 const Task &T = tasks[t];
-                FEASM_REQUIRE(o.item.size() + T.it.size() <= 65535, FEASM_EUNSUPPORTED,
+                FEASM_REQUIRE(o.item.size() + (size_t)T.k <= 65535, FEASM_EUNSUPPORTED,
                               "tile holds too many (block, element) items");
-                o.task.push_back((uint32_t)T.c | ((uint32_t)T.r << 8) | ((uint32_t)T.it.size() << 16) |
+                o.task.push_back((uint32_t)T.c | ((uint32_t)T.r << 8) | ((uint32_t)T.k << 16) |
                                  ((uint32_t)T.sw << 24) | ((uint32_t)T.diag << 25) |
-                                 ((uint32_t)symmetric(T) << 26));
+                                 ((uint32_t)T.sym << 26));
                 o.toff.push_back((uint16_t)o.item.size());
                 o.tpair.push_back(T.c2 < 0 ? kNoPair : (uint16_t)(T.c2 | (T.r2 << 8)));
-                o.item.insert(o.item.end(), T.it.begin(), T.it.end());
+                o.item.insert(o.item.end(), its.begin() + T.off, its.begin() + T.off + T.k);
             }
         }
     // trailing null tasks are not needed
@@ -422,35 +448,44 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int 
     }
     FEASM_REQUIRE(nX < INT32_MAX, FEASM_EUNSUPPORTED, "mesh too large for 32-bit tile offsets");
     T->blob_bytes = nB;
-    std::vector<unsigned char> blob(std::max<int64_t>(nB, 16), 0);
+    const size_t blob_size = (size_t)std::max<int64_t>(nB, 16);
+    std::unique_ptr<unsigned char[]> blob(new unsigned char[blob_size]);  // every byte is written below
+    std::memset(blob.get() + nB, 0, blob_size - (size_t)nB);
     std::vector<int32_t> hn(std::max<int64_t>(nX, 1));
-    for (int64_t t = 0; t < ntiles; t++) {
+    auto fill = [&](int64_t t) {
         const TileOut &o = outs[t];
         const TileHdr &d = hdr[t];
         const TileShape::Layout L = shape.layout(d.H, d.P, d.NH, d.K, d.NT, d.NI);
-        unsigned char *b = blob.data() + d.blob;
+        unsigned char *b = blob.get() + d.blob;
+        std::memset(b, 0, (size_t)d.nbytes);  // alignment gaps
         auto put = [&](size_t off, const void *src, size_t bytes) {
             if (bytes) std::memcpy(b + (off - L.cbase), src, bytes);
         };
         put(L.cbase, o.cbase.data(), o.cbase.size() * 8);
         put(L.cdeg, o.cdeg.data(), o.cdeg.size() * 4);
-        std::vector<uint64_t> rec(o.task.size());
-        for (size_t t = 0; t < rec.size(); t++)
-            rec[t] = (uint64_t)o.task[t] | ((uint64_t)o.toff[t] << 32) | ((uint64_t)o.tpair[t] << 48);
-        put(L.task, rec.data(), rec.size() * 8);
+        uint64_t *rec = reinterpret_cast<uint64_t *>(b + (L.task - L.cbase));  // 16-byte aligned
+        for (size_t k = 0; k < o.task.size(); k++)
+            rec[k] = (uint64_t)o.task[k] | ((uint64_t)o.toff[k] << 32) | ((uint64_t)o.tpair[k] << 48);
         put(L.item, o.item.data(), o.item.size() * 2);
         put(L.lconn, o.lconn.data(), o.lconn.size() * 2);
-        std::vector<double> X((size_t)o.NH * 3);  // (x, y) pairs, then the z values
+        double *X = reinterpret_cast<double *>(b + (L.X - L.cbase));  // (x, y) pairs, then the z values
         for (int i = 0; i < o.NH; i++) {
             X[(size_t)2 * i] = h.xyz[(size_t)3 * o.hnodes[i]];
             X[(size_t)2 * i + 1] = h.xyz[(size_t)3 * o.hnodes[i] + 1];
             X[(size_t)2 * o.NH + i] = h.xyz[(size_t)3 * o.hnodes[i] + 2];
         }
-        put(L.X, X.data(), X.size() * 8);
         std::copy(o.hnodes.begin(), o.hnodes.end(), hn.begin() + d.x0);
+    };
+    {
+        std::vector<std::thread> fillers;
+        for (int w = 0; w < nthreads; w++)
+            fillers.emplace_back([&, w]() {
+                for (int64_t t = w; t < ntiles; t += nthreads) fill(t);
+            });
+        for (auto &th : fillers) th.join();
     }
-    T->blob = dev_alloc(blob.size());
-    FEASM_HIP(hipMemcpy(T->blob->p, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    T->blob = dev_alloc(blob_size);
+    FEASM_HIP(hipMemcpy(T->blob->p, blob.get(), blob_size, hipMemcpyHostToDevice));
     T->hnodes = dev_alloc(hn.size() * 4);
     FEASM_HIP(hipMemcpy(T->hnodes->p, hn.data(), hn.size() * 4, hipMemcpyHostToDevice));
     T->hdr = dev_alloc(std::max<size_t>(hdr.size() * sizeof(TileHdr), 16));
