@@ -83,7 +83,21 @@ std::vector<std::pair<uint64_t, int32_t>> morton_order(const HostMesh &h) {
         }
         key[n] = {code, (int32_t)n};
     }
-    std::sort(key.begin(), key.end());
+    // stable LSD radix sort over the codes' significant bits (11 bits a
+    // pass): the order of std::sort on (code, node), nodes ascending within
+    // equal codes
+    uint64_t maxc = 0;
+    for (const auto &k : key) maxc = std::max(maxc, k.first);
+    const int bits = maxc ? 64 - __builtin_clzll(maxc) : 0;
+    std::vector<std::pair<uint64_t, int32_t>> tmp(nn);
+    std::vector<int64_t> cnt(2049);
+    for (int shift = 0; shift < bits; shift += 11) {
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (const auto &k : key) cnt[((k.first >> shift) & 2047u) + 1]++;
+        for (int d = 0; d < 2048; d++) cnt[d + 1] += cnt[d];
+        for (const auto &k : key) tmp[cnt[(k.first >> shift) & 2047u]++] = k;
+        key.swap(tmp);
+    }
     return key;
 }
 
@@ -304,89 +318,121 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int 
     const int64_t nn = h.nn;
     const int npe = h.npe;
 
-    // ---- partition the Morton order into tiles
-    std::vector<int32_t> estamp(h.ne, -1), nstamp(nn, -1);
-    int32_t epoch = 0;
-    int64_t H = 0, NH = 0, K = 0, NT = 0, NI = 0;
-    auto add_node = [&](int32_t n) {
-        const int64_t d = h.n2e_ptr[n + 1] - h.n2e_ptr[n];
-        K = std::max(K, d);
-        NT += h.adj_ptr[n + 1] - h.adj_ptr[n];
-        NI += d * npe;
-        for (int64_t s = h.n2e_ptr[n]; s < h.n2e_ptr[n + 1]; s++) {
-            const int32_t e = (int32_t)(h.n2e[s] >> 4);
-            if (estamp[e] == epoch) continue;
-            estamp[e] = epoch;
-            H++;
-            for (int a = 0; a < npe; a++) {
-                const int32_t v = h.conn[(int64_t)e * npe + a];
-                if (nstamp[v] != epoch) {
-                    nstamp[v] = epoch;
-                    NH++;
-                }
-            }
-        }
-    };
-    // what the tile would hold with node n added (tile state left unchanged;
-    // pstamp marks the nodes this probe has counted)
-    std::vector<int64_t> pstamp(nn, -1);
-    int64_t pepoch = 0;
-    auto probe = [&](int32_t n, int64_t &h2, int64_t &nh2, int64_t &k2) {
-        h2 = H;
-        nh2 = NH;
-        k2 = std::max(K, h.n2e_ptr[n + 1] - h.n2e_ptr[n]);
-        pepoch++;
-        for (int64_t s = h.n2e_ptr[n]; s < h.n2e_ptr[n + 1]; s++) {
-            const int32_t e = (int32_t)(h.n2e[s] >> 4);
-            if (estamp[e] == epoch) continue;
-            h2++;
-            for (int a = 0; a < npe; a++) {
-                const int32_t v = h.conn[(int64_t)e * npe + a];
-                if (nstamp[v] != epoch && pstamp[v] != pepoch) {
-                    pstamp[v] = pepoch;
-                    nh2++;
-                }
-            }
-        }
+    // ---- partition the Morton order into tiles.  The order is cut into a
+    // fixed number of equal node ranges partitioned independently (in
+    // parallel; no tile straddles a cut, and the tiles do not depend on the
+    // thread count).
+    auto level = [&](int64_t j) {  // coarseness of the Morton boundary before sorted position j
+        const uint64_t x = key[j - 1].first ^ key[j].first;
+        return x ? 64 - __builtin_clzll(x) : 0;
     };
     auto fits = [&](int64_t P, int64_t h2, int64_t nh2, int64_t k2, int64_t nt2, int64_t ni2) {
         return P <= shape.max_nodes && P <= 255 && h2 <= shape.max_halo && h2 <= 255 && nh2 <= 65535 &&
                k2 <= 255 && shape.lds_bytes(h2, (int)P, nh2, (int)k2, nt2, ni2) <= shape.budget;
     };
-    auto level = [&](int64_t j) {  // coarseness of the Morton boundary before sorted position j
-        const uint64_t x = key[j - 1].first ^ key[j].first;
-        return x ? 64 - __builtin_clzll(x) : 0;
-    };
-    std::vector<int64_t> starts;
-    int64_t s = 0;
-    while (s < nn) {
-        epoch++;
-        H = NH = K = NT = NI = 0;
-        int64_t i = s;
-        for (; i < nn; i++) {
-            const int32_t n = key[i].second;
-            int64_t h2, nh2, k2;
-            probe(n, h2, nh2, k2);
-            const int64_t nt2 = NT + (h.adj_ptr[n + 1] - h.adj_ptr[n]) + kPadTasks;
-            const int64_t ni2 = NI + (h.n2e_ptr[n + 1] - h.n2e_ptr[n]) * npe;
-            if (!fits(i - s + 1, h2, nh2, k2, nt2, ni2)) break;
-            add_node(key[i].second);
-        }
-        FEASM_REQUIRE(i > s, FEASM_EUNSUPPORTED, "a node has too many incident elements for the fused kernel");
-        int64_t cut = i;
-        if (i < nn && i - s >= 4) {  // move the cut back to the coarsest boundary in the second half
-            int best = -1;
-            for (int64_t j = i; j >= s + (i - s + 1) / 2; j--) {
-                const int l = level(j);
-                if (l > best) {
-                    best = l;
-                    cut = j;
+    // tile starts within [s0, s1)
+    auto partition_range = [&](int64_t s0, int64_t s1, std::vector<int64_t> &starts) {
+        std::vector<int32_t> estamp(h.ne, -1), nstamp(nn, -1);
+        std::vector<int64_t> pstamp(nn, -1);
+        int32_t epoch = 0;
+        int64_t pepoch = 0;
+        int64_t H = 0, NH = 0, K = 0, NT = 0, NI = 0;
+        auto add_node = [&](int32_t n) {
+            const int64_t d = h.n2e_ptr[n + 1] - h.n2e_ptr[n];
+            K = std::max(K, d);
+            NT += h.adj_ptr[n + 1] - h.adj_ptr[n];
+            NI += d * npe;
+            for (int64_t q = h.n2e_ptr[n]; q < h.n2e_ptr[n + 1]; q++) {
+                const int32_t e = (int32_t)(h.n2e[q] >> 4);
+                if (estamp[e] == epoch) continue;
+                estamp[e] = epoch;
+                H++;
+                for (int a = 0; a < npe; a++) {
+                    const int32_t v = h.conn[(int64_t)e * npe + a];
+                    if (nstamp[v] != epoch) {
+                        nstamp[v] = epoch;
+                        NH++;
+                    }
                 }
             }
+        };
+        // what the tile would hold with node n added (tile state left
+        // unchanged; pstamp marks the nodes this probe has counted)
+        auto probe = [&](int32_t n, int64_t &h2, int64_t &nh2, int64_t &k2) {
+            h2 = H;
+            nh2 = NH;
+            k2 = std::max(K, h.n2e_ptr[n + 1] - h.n2e_ptr[n]);
+            pepoch++;
+            for (int64_t q = h.n2e_ptr[n]; q < h.n2e_ptr[n + 1]; q++) {
+                const int32_t e = (int32_t)(h.n2e[q] >> 4);
+                if (estamp[e] == epoch) continue;
+                h2++;
+                for (int a = 0; a < npe; a++) {
+                    const int32_t v = h.conn[(int64_t)e * npe + a];
+                    if (nstamp[v] != epoch && pstamp[v] != pepoch) {
+                        pstamp[v] = pepoch;
+                        nh2++;
+                    }
+                }
+            }
+        };
+        int64_t s = s0;
+        while (s < s1) {
+            epoch++;
+            H = NH = K = NT = NI = 0;
+            int64_t i = s;
+            for (; i < s1; i++) {
+                const int32_t n = key[i].second;
+                int64_t h2, nh2, k2;
+                probe(n, h2, nh2, k2);
+                const int64_t nt2 = NT + (h.adj_ptr[n + 1] - h.adj_ptr[n]) + kPadTasks;
+                const int64_t ni2 = NI + (h.n2e_ptr[n + 1] - h.n2e_ptr[n]) * npe;
+                if (!fits(i - s + 1, h2, nh2, k2, nt2, ni2)) break;
+                add_node(n);
+            }
+            FEASM_REQUIRE(i > s, FEASM_EUNSUPPORTED, "a node has too many incident elements for the fused kernel");
+            int64_t cut = i;
+            if (i < s1 && i - s >= 4) {  // move the cut back to the coarsest boundary in the second half
+                int best = -1;
+                for (int64_t j = i; j >= s + (i - s + 1) / 2; j--) {
+                    const int l = level(j);
+                    if (l > best) {
+                        best = l;
+                        cut = j;
+                    }
+                }
+            }
+            starts.push_back(s);
+            s = cut;
         }
-        starts.push_back(s);
-        s = cut;
+    };
+    const int nchunks = (int)std::max<int64_t>(1, std::min<int64_t>(16, nn / 65536));
+    std::vector<std::vector<int64_t>> chunk_starts(nchunks);
+    {
+        const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(std::max(1, max_threads), nchunks),
+                                                                   std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        std::vector<std::string> errs(nt);
+        std::vector<int> codes(nt, 0);
+        for (int w = 0; w < nt; w++)
+            pool.emplace_back([&, w]() {
+                try {
+                    for (int c = w; c < nchunks; c += nt)
+                        partition_range(nn * c / nchunks, nn * (c + 1) / nchunks, chunk_starts[c]);
+                } catch (const Error &e) {
+                    codes[w] = e.code;
+                    errs[w] = e.what();
+                } catch (const std::exception &e) {
+                    codes[w] = FEASM_ENOMEM;
+                    errs[w] = e.what();
+                }
+            });
+        for (auto &th : pool) th.join();
+        for (int w = 0; w < nt; w++)
+            if (codes[w]) throw Error(codes[w], errs[w]);
     }
+    std::vector<int64_t> starts;
+    for (const auto &cs : chunk_starts) starts.insert(starts.end(), cs.begin(), cs.end());
     starts.push_back(nn);
     const int64_t ntiles = (int64_t)starts.size() - 1;
 
