@@ -110,7 +110,15 @@ struct TileOut {
     std::vector<uint32_t> task;
 };
 
-void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
+// Per-thread lookup tables of make_tile: an element's halo index, a node's
+// index among the halo nodes, a row node's position in the current column
+// (written for the current tile or column before they are read).
+struct TileScratch {
+    std::vector<int32_t> ehalo, npos, rpos;
+    TileScratch(int64_t ne, int64_t nn) : ehalo(ne), npos(nn), rpos(nn) {}
+};
+
+void make_tile(const HostMesh &h, const int32_t *owned, int P, TileScratch &sc, TileOut &o) {
     const int npe = h.npe;
     std::vector<int32_t> halo;
     for (int c = 0; c < P; c++)
@@ -128,12 +136,11 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
     FEASM_REQUIRE(o.H <= 255 && o.NH <= 65535 && npe <= 15 && P <= 255, FEASM_EUNSUPPORTED,
                   "tile exceeds the fused kernel's index widths");
     o.hnodes = hn;
+    for (int i = 0; i < o.H; i++) sc.ehalo[halo[i]] = i;
+    for (int i = 0; i < o.NH; i++) sc.npos[hn[i]] = i;
     o.lconn.resize((size_t)o.H * npe);
     for (int i = 0; i < o.H; i++)
-        for (int a = 0; a < npe; a++) {
-            const int32_t v = h.conn[(int64_t)halo[i] * npe + a];
-            o.lconn[(size_t)i * npe + a] = (uint16_t)(std::lower_bound(hn.begin(), hn.end(), v) - hn.begin());
-        }
+        for (int a = 0; a < npe; a++) o.lconn[(size_t)i * npe + a] = (uint16_t)sc.npos[h.conn[(int64_t)halo[i] * npe + a]];
     // block tasks: items of (column c, row position r) in ascending element,
     // held in one array (task i's items at [off, off + k)); two passes over
     // the owned nodes' elements: count, then fill
@@ -178,14 +185,16 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileOut &o) {
                 }
             }
             const size_t tb = tfirst[c];
+            for (int r = 0; r < deg; r++) sc.rpos[rows[r]] = r;
             for (int64_t s = h.n2e_ptr[node]; s < h.n2e_ptr[node + 1]; s++) {
                 const int32_t e = (int32_t)(h.n2e[s] >> 4);
                 const int ac = (int)(h.n2e[s] & 15u);
-                const int hl = (int)(std::lower_bound(halo.begin(), halo.end(), e) - halo.begin());
+                const int hl = sc.ehalo[e];
                 for (int a = 0; a < npe; a++) {
                     const int32_t v = h.conn[(int64_t)e * npe + a];
-                    const int r = (int)(std::lower_bound(rows, rows + deg, v) - rows);
-                    FEASM_REQUIRE(r < deg && rows[r] == v, FEASM_ESTATE, "pattern misses an element node pair");
+                    const int r = sc.rpos[v];
+                    FEASM_REQUIRE(r >= 0 && r < deg && rows[r] == v, FEASM_ESTATE,
+                                  "pattern misses an element node pair");
                     Task &T = tasks[tb + r];
                     if (pass == 1) its[T.off + T.k] = (uint16_t)((hl << 8) | (ac << 4) | a);
                     T.k++;
@@ -449,8 +458,9 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int 
     for (int w = 0; w < nthreads; w++)
         pool.emplace_back([&, w]() {
             try {
+                TileScratch sc(h.ne, nn);
                 for (int64_t t = w; t < ntiles; t += nthreads)
-                    make_tile(h, order.data() + starts[t], (int)(starts[t + 1] - starts[t]), outs[t]);
+                    make_tile(h, order.data() + starts[t], (int)(starts[t + 1] - starts[t]), sc, outs[t]);
             } catch (const Error &e) {
                 codes[w] = e.code;
                 errs[w] = e.what();
