@@ -224,12 +224,27 @@ __device__ __forceinline__ void block_sum(int k, const uint16_t *__restrict__ it
 template <bool COEF>
 struct Staged {
     uint4 c0;  // blob chunk tid (the rest of a larger blob is copied at store time)
+    // waits for the loads here: called before the wavefront's global stores,
+    // so that the wait does not include them (the vector memory counter
+    // counts loads and stores in issue order)
+    __device__ __forceinline__ void ready() const {
+        asm volatile("" ::"v"(c0.x), "v"(c0.y), "v"(c0.z), "v"(c0.w));
+    }
 };
 template <>
 struct Staged<true> {
     uint4 c0;
     double d;  // material coefficient tid
+    __device__ __forceinline__ void ready() const {
+        asm volatile("" ::"v"(c0.x), "v"(c0.y), "v"(c0.z), "v"(c0.w), "v"(d));
+    }
 };
+
+// A tile header.  With `hdr` a __restrict__ kernel argument (nothing the
+// kernel stores aliases it) and t the same in every lane, the compiler reads
+// it with scalar loads, so that it sits in scalar registers and no vector
+// memory wait is needed for it.
+__device__ __forceinline__ TileHdr load_hdr(const TileHdr *__restrict__ hdr, int64_t t) { return hdr[t]; }
 
 template <bool COEF>
 __device__ __forceinline__ void stage_issue(const TileJob &job, const TileHdr &hd, const double *coef,
@@ -258,13 +273,16 @@ __device__ __forceinline__ void stage_store(const TileJob &job, const TileHdr &h
     if (tid == 0) *sNeg = 0;
 }
 
-// One tile, phases 1 and 2, with its image already in LDS.  Between the
-// phases the next tile's loads are issued into `st` (when `more`).
-template <int NPE, int NQP, int FORM, int NDN, typename R, bool COEF>
+// One tile, phases 1 and 2, with its image already in LDS.  `after_phase1`
+// runs between the phases; `before_stores` in phase 2, before the first
+// global store of every thread.  LDSCOEF: the scalar material coefficients
+// are read from LDS (batched launches, where each tile brings its own)
+// rather than from the launch's parameters.
+template <int NPE, int NQP, int FORM, int NDN, typename R, bool LDSCOEF, typename AfterPhase1,
+          typename BeforeStores>
 __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJob &job, const FusedParams &kp,
-                                             const TileHdr &hd, unsigned char *smem, bool more,
-                                             const TileJob &njob, const TileHdr &nhd, const double *ncoef,
-                                             Staged<COEF> &st) {
+                                             const TileHdr &hd, unsigned char *smem, const AfterPhase1 &after_phase1,
+                                             const BeforeStores &before_stores) {
     constexpr bool GRAD = (FORM != FEASM_FORM_DOT);
     constexpr int SWP = form_swp<FORM>();
     const int tid = threadIdx.x;
@@ -279,6 +297,8 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
     double *sDN = sN + NQP * NPE;
     double *sD = sDN + NQP * DS;  // material coefficients (read as broadcasts)
     int *sNeg = reinterpret_cast<int *>(sD + 36);  // set when J w < 0 somewhere in the tile
+    const double *sWq = sD + 37;                   // quadrature weights
+    const double *coef = LDSCOEF ? sD : kp.fp.coef;
     const uint64_t *sTask = reinterpret_cast<const uint64_t *>(smem + L.task);
     uint16_t *sItem = reinterpret_cast<uint16_t *>(smem + L.item);
     double *sX = reinterpret_cast<double *>(smem + L.X);
@@ -320,7 +340,7 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
         const double det = RJ(1, 1) * (RJ(2, 2) * RJ(3, 3) - RJ(3, 2) * RJ(2, 3)) -
                            RJ(1, 2) * (RJ(2, 1) * RJ(3, 3) - RJ(2, 3) * RJ(3, 1)) +
                            RJ(1, 3) * (RJ(2, 1) * RJ(3, 2) - RJ(2, 2) * RJ(3, 1));
-        double wj = (FORM == FEASM_FORM_DIFFUSION_ISO) ? (kp.fp.coef[0] * det) * kp.w[q] : det * kp.w[q];
+        double wj = (FORM == FEASM_FORM_DIFFUSION_ISO) ? (coef[0] * det) * sWq[q] : det * sWq[q];
         if (GRAD) {
             const double invdet = 1.0 / det;
             const double iv[9] = {(RJ(2, 2) * RJ(3, 3) - RJ(3, 2) * RJ(2, 3)) * invdet,
@@ -368,16 +388,24 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
     // the path that applies the signs
     __syncthreads();
     const bool tile_neg = *sNeg != 0;
-    if (more) stage_issue(njob, nhd, ncoef, st);
+    after_phase1();
 
 
     // ---- phase 2: one thread per node block: element matrices of the shared
     // elements, ordered sum, material, one write
     double *__restrict__ nzval = job.nzval;
-    for (int t = tid; t < NT; t += kThreads) {
+    // one block task: its values and where they go
+    struct BlockOut {
+        double out[NDN * NDN];
+        int64_t base, base2;
+        int deg, deg2;
+        bool valid, pair;
+    };
+    const auto compute = [&](int t, BlockOut &b) {
+        b.valid = false;
         const uint64_t rec = sTask[t];
         const uint32_t th = (uint32_t)rec;
-        if (th == kNullTask) continue;
+        if (th == kNullTask) return;
         const int c = (int)(th & 255u), r = (int)((th >> 8) & 255u), k = (int)((th >> 16) & 255u);
         const bool sw = (th >> 24) & 1u, diag = (th >> 25) & 1u, sym = (th >> 26) & 1u;
         double acc[SWP];
@@ -400,7 +428,7 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
             else
                 block_sum<NPE, NQP, FORM, R, false, false, SWP, ILP>(k, items, sU, sSg, sN, sW, acc);
         }
-        double out[NDN * NDN];
+        double *out = b.out;
         if constexpr (SWP == 9) {
             // contract in the orientation of the lower node number
             double Mc[9];
@@ -421,7 +449,7 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
             } else {
                 double Kc[9];
                 if constexpr (FORM == kFormElasticCubic) {
-                    const double d00 = kp.fp.coef[0], d01 = kp.fp.coef[1], d33 = kp.fp.coef[21];
+                    const double d00 = coef[0], d01 = coef[1], d33 = coef[21];
                     const double tr = (Mc[0] + Mc[4]) + Mc[8];
 #pragma unroll
                     for (int i = 0; i < 3; i++)
@@ -462,26 +490,48 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
         } else {
             out[0] = acc[0];
         }
-        const int deg = sCd[c];
-        const int64_t base = (int64_t)NDN * NDN * sCb[c] + (int64_t)r * NDN;
+        b.deg = sCd[c];
+        b.base = (int64_t)NDN * NDN * sCb[c] + (int64_t)r * NDN;
+        const uint32_t pr = (uint32_t)(rec >> 48);
+        b.pair = pr != kNoPair;
+        if (b.pair) {  // the transposed block (row c, column m), in its own column
+            const int c2 = (int)(pr & 255u), r2 = (int)(pr >> 8);
+            b.deg2 = sCd[c2];
+            b.base2 = (int64_t)NDN * NDN * sCb[c2] + (int64_t)r2 * NDN;
+        }
+        b.valid = true;
+    };
+    const auto store = [&](const BlockOut &b) {
+        if (!b.valid) return;
 #pragma unroll
         for (int l = 0; l < NDN; l++)
 #pragma unroll
-            for (int a = 0; a < NDN; a++) nzval[base + (int64_t)l * NDN * deg + a] = out[NDN * a + l];
-        const uint32_t pr = (uint32_t)(rec >> 48);
-        if (pr != kNoPair) {  // the transposed block (row c, column m), in its own column
-            const int c2 = (int)(pr & 255u), r2 = (int)(pr >> 8);
-            const int deg2 = sCd[c2];
-            const int64_t base2 = (int64_t)NDN * NDN * sCb[c2] + (int64_t)r2 * NDN;
+            for (int a = 0; a < NDN; a++) nzval[b.base + (int64_t)l * NDN * b.deg + a] = b.out[NDN * a + l];
+        if (b.pair) {
             // K_cm = K_mc^T; the DOT forms scale a scalar by the same matrix
             // in both blocks
 #pragma unroll
             for (int l = 0; l < NDN; l++)
 #pragma unroll
                 for (int a = 0; a < NDN; a++)
-                    nzval[base2 + (int64_t)l * NDN * deg2 + a] =
-                        (FORM == FEASM_FORM_DOT) ? out[NDN * a + l] : out[NDN * l + a];
+                    nzval[b.base2 + (int64_t)l * NDN * b.deg2 + a] =
+                        (FORM == FEASM_FORM_DOT) ? b.out[NDN * a + l] : b.out[NDN * l + a];
         }
+    };
+    // The first task of every thread is computed before `before_stores`
+    // runs and stored after it: the wait there (for loads issued before
+    // phase 1) then comes as late as it can without including any store.
+    {
+        BlockOut b;
+        b.valid = false;
+        if (tid < NT) compute(tid, b);
+        before_stores();
+        store(b);
+    }
+    for (int t = tid + kThreads; t < NT; t += kThreads) {
+        BlockOut b;
+        compute(t, b);
+        store(b);
     }
     __syncthreads();  // LDS is reused by the workgroup's next tile
 }
@@ -491,6 +541,8 @@ template <int NPE, int NQP>
 __device__ __forceinline__ void load_tables(const FusedParams &kp, unsigned char *smem) {
     double *sN = reinterpret_cast<double *>(smem);  // the layout's tab region starts at 0
     double *sDN = sN + NQP * NPE;  // (d0, d1) pairs [q][a], then d2 [q][a]
+    double *sWq = sDN + NQP * NPE * 3 + 37;  // after the material coefficients and the flag
+    for (int i = threadIdx.x; i < NQP; i += blockDim.x) sWq[i] = kp.w[i];
     for (int i = threadIdx.x; i < NQP * NPE; i += blockDim.x) sN[i] = kp.N[i];
     for (int i = threadIdx.x; i < NQP * NPE; i += blockDim.x) {
         sDN[2 * i] = kp.dN[3 * i];
@@ -514,7 +566,9 @@ __device__ __forceinline__ TileRange xcd_range(int64_t ntiles) {
 }
 
 template <int NPE, int NQP, int FORM, int NDN, typename R>
-__global__ __launch_bounds__(kMaxThreads) void k_fused(TileShape shape, TileJob job, int64_t ntiles, FusedParams kp) {
+__global__ __launch_bounds__(kMaxThreads) void k_fused(TileShape shape, TileJob job,
+                                                    const TileHdr *__restrict__ hdr, int64_t ntiles,
+                                                    FusedParams kp) {
     extern __shared__ __align__(16) unsigned char smem[];
     const TileRange rg = xcd_range(ntiles);
     if (rg.begin >= rg.end) return;
@@ -523,20 +577,51 @@ __global__ __launch_bounds__(kMaxThreads) void k_fused(TileShape shape, TileJob 
     int *sNeg = reinterpret_cast<int *>(sD + 36);
     load_tables<NPE, NQP>(kp, smem);
     for (int i = threadIdx.x; i < 36; i += blockDim.x) sD[i] = kp.fp.coef[i];  // one material per launch
-    int64_t t = rg.begin;
-    TileHdr hd = job.hdr[t];
+    // Tile t runs while the blob of tile t1 (the next one) is loaded and
+    // the header of tile t2 is read.  For the gradient forms the blob load is
+    // issued before phase 1 and waited for before phase 2's global stores:
+    // the vector memory counter counts loads and stores together, in issue
+    // order, so a load waited for after the stores would wait for every
+    // store too.  The DOT forms have too short a phase 1 to hide the load:
+    // they issue it after phase 1 and wait for it (and the stores) at the
+    // next tile (measured faster for them).
+    constexpr bool EARLY = (FORM != FEASM_FORM_DOT);
+    int64_t t1 = rg.begin + rg.step;
+    TileHdr hd = load_hdr(hdr, rg.begin);
+    TileHdr hn = load_hdr(hdr, t1 < rg.end ? t1 : rg.begin);
+    // (the loads are unconditional, from valid tiles, so that no register
+    // copy of a pending load forces an early wait)
     Staged<false> st;
     stage_issue(job, hd, nullptr, st);
+    st.ready();
     for (;;) {
+        const bool more = t1 < rg.end;
+        const int64_t t2 = t1 + rg.step;
+        const int64_t i2 = t2 < rg.end ? t2 : rg.begin;
+        // the gradient forms read the header of tile t2 while the blob goes
+        // to LDS: the barrier's wait covers it, and no scalar load is left
+        // pending in the phases (that would turn every wait for an LDS read
+        // into a wait for all of them); the store-bound DOT forms measured
+        // faster with the read after the barrier
+        TileHdr h2;
+        if constexpr (EARLY) h2 = load_hdr(hdr, i2);
         stage_store(job, hd, st, shape.layout(hd.H, hd.P, hd.NH, hd.K, hd.NT, hd.NI), smem, sD, sNeg);
         __syncthreads();
-        const int64_t tn = t + rg.step;
-        TileHdr hn = hd;
-        if (tn < rg.end) hn = job.hdr[tn];
-        tile_compute<NPE, NQP, FORM, NDN, R, false>(shape, job, kp, hd, smem, tn < rg.end, job, hn, nullptr, st);
-        if (tn >= rg.end) break;
-        t = tn;
+        if constexpr (!EARLY) h2 = load_hdr(hdr, i2);
+        const TileHdr &hb = more ? hn : hd;
+        if constexpr (EARLY) stage_issue(job, hb, nullptr, st);
+        tile_compute<NPE, NQP, FORM, NDN, R, false>(
+            shape, job, kp, hd, smem,
+            [&]() {
+                if constexpr (!EARLY) stage_issue(job, hb, nullptr, st);
+            },
+            [&]() {
+                if constexpr (EARLY) st.ready();
+            });
+        if (!more) break;
+        t1 = t2;
         hd = hn;
+        hn = h2;
     }
 }
 
@@ -554,7 +639,9 @@ struct BatchEntry {
 // type, rule, arithmetic and form.
 template <int NPE, int NQP, int FORM, int NDN, typename R>
 __global__ __launch_bounds__(kMaxThreads) void k_fused_batch(const BatchEntry *__restrict__ entries,
-                                                          const int32_t *__restrict__ tile_entry, int64_t ntiles) {
+                                                          const int32_t *__restrict__ tile_entry,
+                                                          const TileHdr *__restrict__ hdr, int64_t ntiles,
+                                                          TileShape shape) {
     extern __shared__ __align__(16) unsigned char smem[];
     const TileRange rg = xcd_range(ntiles);
     if (rg.begin >= rg.end) return;
@@ -562,27 +649,51 @@ __global__ __launch_bounds__(kMaxThreads) void k_fused_batch(const BatchEntry *_
     double *sD = reinterpret_cast<double *>(smem) + NQP * NPE + NQP * DS;
     int *sNeg = reinterpret_cast<int *>(sD + 36);
     load_tables<NPE, NQP>(entries[tile_entry[rg.begin]].kp, smem);
-    int64_t t = rg.begin;
-    const BatchEntry *E = &entries[tile_entry[t]];
-    TileHdr hd = E->job.hdr[t - E->tile0];
+    // the tile walk and its loads as in k_fused; every tile also brings its
+    // mesh's coefficients
+    constexpr bool EARLY = (FORM != FEASM_FORM_DOT);
+    int64_t t1 = rg.begin + rg.step;
+    TileHdr hd = load_hdr(hdr, rg.begin);
+    const BatchEntry *E = entries + tile_entry[rg.begin];
+    const int64_t i1 = t1 < rg.end ? t1 : rg.begin;
+    TileHdr hn = load_hdr(hdr, i1);
+    const BatchEntry *En = entries + tile_entry[i1];
     Staged<true> st;
     stage_issue(E->job, hd, E->kp.fp.coef, st);
+    st.ready();
     for (;;) {
-        stage_store(E->job, hd, st, E->shape.layout(hd.H, hd.P, hd.NH, hd.K, hd.NT, hd.NI), smem, sD, sNeg);
-        __syncthreads();
-        const int64_t tn = t + rg.step;
-        const BatchEntry *En = E;
-        TileHdr hn = hd;
-        if (tn < rg.end) {
-            En = &entries[tile_entry[tn]];
-            hn = En->job.hdr[tn - En->tile0];
+        const bool more = t1 < rg.end;
+        const int64_t t2 = t1 + rg.step;
+        const int64_t i2 = t2 < rg.end ? t2 : rg.begin;
+        TileHdr h2;
+        const BatchEntry *E2 = nullptr;
+        if constexpr (EARLY) {
+            h2 = load_hdr(hdr, i2);
+            E2 = entries + tile_entry[i2];
         }
-        tile_compute<NPE, NQP, FORM, NDN, R, true>(E->shape, E->job, E->kp, hd, smem, tn < rg.end, En->job, hn,
-                                                   En->kp.fp.coef, st);
-        if (tn >= rg.end) break;
-        t = tn;
-        E = En;
+        stage_store(E->job, hd, st, shape.layout(hd.H, hd.P, hd.NH, hd.K, hd.NT, hd.NI), smem, sD, sNeg);
+        __syncthreads();
+        if constexpr (!EARLY) {
+            h2 = load_hdr(hdr, i2);
+            E2 = entries + tile_entry[i2];
+        }
+        const TileHdr &hb = more ? hn : hd;
+        const BatchEntry *Eb = more ? En : E;
+        if constexpr (EARLY) stage_issue(Eb->job, hb, Eb->kp.fp.coef, st);
+        tile_compute<NPE, NQP, FORM, NDN, R, true>(
+            shape, E->job, E->kp, hd, smem,
+            [&]() {
+                if constexpr (!EARLY) stage_issue(Eb->job, hb, Eb->kp.fp.coef, st);
+            },
+            [&]() {
+                if constexpr (EARLY) st.ready();
+            });
+        if (!more) break;
+        t1 = t2;
         hd = hn;
+        E = En;
+        hn = h2;
+        En = E2;
     }
 }
 
@@ -617,7 +728,8 @@ void go(feasm_ctx *ctx, const feasm_mesh *m, const Pattern &p, const TileSet &t,
     FEASM_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, budget));
     const unsigned grid = persistent_grid(ctx, t.shape.budget, t.ntiles);
     TileJob job = make_job(m, t, nzval);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(t.shape.threads), budget, ctx->stream, t.shape, job, t.ntiles, kp);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(t.shape.threads), budget, ctx->stream, t.shape, job, job.hdr, t.ntiles,
+                       kp);
     FEASM_HIP(hipGetLastError());
 }
 
@@ -652,7 +764,8 @@ void go_batch(feasm_ctx *ctx, const BatchLaunch &b) {
     auto kern = k_fused_batch<NPE, NQP, FORM, NDN, R>;
     FEASM_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.budget));
     hipLaunchKernelGGL(kern, dim3(b.grid), dim3(b.threads), b.budget, ctx->stream,
-                       static_cast<const BatchEntry *>(b.entries->p), dp<int32_t>(b.tile_entry), b.ntiles);
+                       static_cast<const BatchEntry *>(b.entries->p), dp<int32_t>(b.tile_entry),
+                       static_cast<const TileHdr *>(b.hdr->p), b.ntiles, b.shape);
     FEASM_HIP(hipGetLastError());
 }
 
@@ -679,12 +792,20 @@ void prepare_batch_launch(feasm_ctx *ctx, BatchLaunch &b, const std::vector<Batc
         b.ntiles += it.tiles->ntiles;
         b.budget = std::max(b.budget, it.tiles->shape.budget);
         b.threads = it.tiles->shape.threads;
+        if (i == 0) b.shape = it.tiles->shape;
+        const TileShape &s0 = b.shape, &si = it.tiles->shape;
+        FEASM_REQUIRE(si.npe == s0.npe && si.nqp == s0.nqp && si.rbytes == s0.rbytes && si.grad == s0.grad,
+                      FEASM_ESTATE, "meshes of one batched launch differ in their tile layout");
         host.push_back(E);
     }
     b.entries = dev_alloc(std::max<size_t>(host.size() * sizeof(BatchEntry), 16));
     b.tile_entry = dev_alloc(std::max<size_t>(tile_entry.size() * 4, 4));
+    b.hdr = dev_alloc(std::max<size_t>((size_t)b.ntiles * sizeof(TileHdr), 16));
     FEASM_HIP(hipMemcpy(b.entries->p, host.data(), host.size() * sizeof(BatchEntry), hipMemcpyHostToDevice));
     FEASM_HIP(hipMemcpy(b.tile_entry->p, tile_entry.data(), tile_entry.size() * 4, hipMemcpyHostToDevice));
+    for (size_t i = 0; i < items.size(); i++)  // the meshes' headers, one after the other
+        FEASM_HIP(hipMemcpy(static_cast<TileHdr *>(b.hdr->p) + host[i].tile0, items[i].tiles->hdr->p,
+                            (size_t)items[i].tiles->ntiles * sizeof(TileHdr), hipMemcpyDeviceToDevice));
     b.grid = persistent_grid(ctx, b.budget, b.ntiles);
 }
 

@@ -41,8 +41,10 @@ struct TileShape {
     __host__ __device__ Layout layout(int64_t H, int P, int64_t NH, int K, int64_t NT, int64_t NI) const {
         Layout L;
         size_t o = 0;
-        L.tab = o;  // N [nqp][npe], dN (two planes, see dstride), the 36 material coefficients (FP64), a flag
-        o = align16(o + ((size_t)nqp * (npe + dstride()) + 36) * 8 + 4);
+        // N [nqp][npe], dN (two planes, see dstride), the 36 material
+        // coefficients (FP64), a flag (in an 8-byte slot), the weights [nqp]
+        L.tab = o;
+        o = align16(o + ((size_t)nqp * (npe + dstride() + 1) + 37) * 8);
         L.cbase = o;
         o = align16(o + (size_t)P * 8);
         L.cdeg = o;
@@ -132,7 +134,8 @@ struct BatchLaunch {
     int64_t ntiles = 0;
     size_t budget = 0;
     unsigned grid = 0;
-    DevPtr entries, tile_entry;
+    DevPtr entries, tile_entry, hdr;  // hdr: the tiles' headers, mesh after mesh
+    TileShape shape;  // the meshes' tile layout (npe, nqp, rbytes and grad agree)
 };
 void prepare_batch_launch(feasm_ctx *ctx, BatchLaunch &b, const std::vector<BatchMesh> &items);
 // the fused kernel's form id for these parameters (forms with a specialised
