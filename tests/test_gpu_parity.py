@@ -420,3 +420,37 @@ def test_moved_nodes_reassemble_with_the_new_coordinates(ctx, etype):
         assert rel_fro(nz, onz) <= FP64_TOL
         K.close()
         m.close()
+
+
+@pytest.mark.parametrize("etype,n,form,ndn,coeff", [
+    (O.H8, (7, 6, 5), L.LIN_ELASTIC, 3, D200),
+    (O.H8, (5, 6, 4), L.LIN_ELASTIC, 3, DANISO),
+    (O.T10, (3, 3, 3), L.DOT, 3, 2.7e-9 * np.eye(3)),
+    (O.T10, (3, 2, 3), L.DIFFUSION_ISO, 1, [1.7]),
+])
+def test_tile_parameters_do_not_change_the_values(ctx, etype, n, form, ndn, coeff):
+    """Every block adds the same element matrices in ascending element order
+    whatever the tiling, so the values are bitwise the same for every tile
+    size, LDS budget and workgroup size (small workgroups run several block
+    tasks per thread), and match the oracle."""
+    conn, xyz = mesh_for(etype, n, 0.15 if etype == O.H8 else 0.1, 11)
+    rule, order = rule_for(etype)
+    ocp, orv, onz = oracle_csc(etype, form, conn, xyz, ndn, coeff)
+    for prec, tol in ((L.FP64, FP64_TOL), (L.MIXED, MIXED_TOL)):
+        ref = None
+        for params in ((0, 0, 0), (8, 48, 256), (24, 96, 512), (255, 160, 1024)):
+            say("tile params", etype, form, prec, params)
+            m = core.DeviceMesh(ctx, etype, conn, xyz)
+            m.set_dofs(ndn)
+            m.set_numeric_path("fused")
+            m.set_tile_params(*params)
+            K = m.bilform(form, rule, order, np.ravel(coeff), precision=prec)
+            cp, rv, nz = K.csc_arrays(1)
+            assert np.array_equal(cp, ocp) and np.array_equal(rv, orv)
+            assert rel_fro(nz, onz) <= tol
+            if ref is None:
+                ref = nz
+            else:
+                assert np.array_equal(nz, ref), params
+            K.close()
+            m.close()
