@@ -218,27 +218,31 @@ __device__ __forceinline__ void block_sum(int k, const uint16_t *__restrict__ it
     }
 }
 
-// Global loads of one tile's LDS image, issued ahead and held in registers.
-// COEF: the tile also brings its mesh's material coefficients (batched
-// launches); otherwise the launch's coefficients are loaded once
-template <bool COEF>
+// Global loads of one tile's LDS image, issued ahead and held in registers:
+// NC 16-byte chunks per thread (chunks tid, tid + threads, ...; the rest of a
+// larger blob is copied at store time).  COEF: the tile also brings its
+// mesh's material coefficients (batched launches); otherwise the launch's
+// coefficients are loaded once.
+template <bool COEF, int NC>
 struct Staged {
-    uint4 c0;  // blob chunk tid (the rest of a larger blob is copied at store time)
+    uint4 c[NC];
+    double d;  // material coefficient tid (COEF)
     // waits for the loads here: called before the wavefront's global stores,
     // so that the wait does not include them (the vector memory counter
     // counts loads and stores in issue order)
     __device__ __forceinline__ void ready() const {
-        asm volatile("" ::"v"(c0.x), "v"(c0.y), "v"(c0.z), "v"(c0.w));
+#pragma unroll
+        for (int i = 0; i < NC; i++) asm volatile("" ::"v"(c[i].x), "v"(c[i].y), "v"(c[i].z), "v"(c[i].w));
+        if constexpr (COEF) asm volatile("" ::"v"(d));
     }
 };
-template <>
-struct Staged<true> {
-    uint4 c0;
-    double d;  // material coefficient tid
-    __device__ __forceinline__ void ready() const {
-        asm volatile("" ::"v"(c0.x), "v"(c0.y), "v"(c0.z), "v"(c0.w), "v"(d));
-    }
-};
+// chunks prefetched per thread (measured: the FP64 and DOT kernels gain from
+// a second chunk on the larger T10 blobs and nothing from a third; the
+// mixed-precision gradient kernels are faster with one)
+template <typename R, int FORM>
+constexpr int staged_chunks() {
+    return (sizeof(R) == 4 && FORM != FEASM_FORM_DOT) ? 1 : 2;
+}
 
 // A tile header.  With `hdr` a __restrict__ kernel argument (nothing the
 // kernel stores aliases it) and t the same in every lane, the compiler reads
@@ -246,28 +250,31 @@ struct Staged<true> {
 // memory wait is needed for it.
 __device__ __forceinline__ TileHdr load_hdr(const TileHdr *__restrict__ hdr, int64_t t) { return hdr[t]; }
 
-template <bool COEF>
+template <bool COEF, int NC>
 __device__ __forceinline__ void stage_issue(const TileJob &job, const TileHdr &hd, const double *coef,
-                                            Staged<COEF> &st) {
-    const int tid = threadIdx.x;
+                                            Staged<COEF, NC> &st) {
+    const int tid = threadIdx.x, nt = blockDim.x;
     const int nch = hd.nbytes >> 4;  // at least one chunk: every tile owns a column
     const uint4 *src = reinterpret_cast<const uint4 *>(job.blob + hd.blob);
-    st.c0 = src[min(tid, nch - 1)];
+#pragma unroll
+    for (int i = 0; i < NC; i++) st.c[i] = src[min(tid + i * nt, nch - 1)];
     if constexpr (COEF) st.d = coef[min(tid, 35)];
 }
 
 // The staged image into LDS (blob with the halo coordinates, material
 // coefficients); clears the tile's sign flag.
-template <bool COEF>
-__device__ __forceinline__ void stage_store(const TileJob &job, const TileHdr &hd, const Staged<COEF> &st,
+template <bool COEF, int NC>
+__device__ __forceinline__ void stage_store(const TileJob &job, const TileHdr &hd, const Staged<COEF, NC> &st,
                                             const TileShape::Layout &L, unsigned char *smem, double *sD,
                                             int *sNeg) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nch = hd.nbytes >> 4;
     uint4 *dst = reinterpret_cast<uint4 *>(smem + L.cbase);
-    if (tid < nch) dst[tid] = st.c0;
+#pragma unroll
+    for (int i = 0; i < NC; i++)
+        if (tid + i * nt < nch) dst[tid + i * nt] = st.c[i];
     const uint4 *src = reinterpret_cast<const uint4 *>(job.blob + hd.blob);
-    for (int i = tid + nt; i < nch; i += nt) dst[i] = src[i];
+    for (int i = tid + NC * nt; i < nch; i += nt) dst[i] = src[i];
     if constexpr (COEF)
         if (tid < 36) sD[tid] = st.d;
     if (tid == 0) *sNeg = 0;
@@ -591,7 +598,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_fused(TileShape shape, TileJob 
     TileHdr hn = load_hdr(hdr, t1 < rg.end ? t1 : rg.begin);
     // (the loads are unconditional, from valid tiles, so that no register
     // copy of a pending load forces an early wait)
-    Staged<false> st;
+    Staged<false, staged_chunks<R, FORM>()> st;
     stage_issue(job, hd, nullptr, st);
     st.ready();
     for (;;) {
@@ -658,7 +665,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_fused_batch(const BatchEntry *_
     const int64_t i1 = t1 < rg.end ? t1 : rg.begin;
     TileHdr hn = load_hdr(hdr, i1);
     const BatchEntry *En = entries + tile_entry[i1];
-    Staged<true> st;
+    Staged<true, staged_chunks<R, FORM>()> st;
     stage_issue(E->job, hd, E->kp.fp.coef, st);
     st.ready();
     for (;;) {
