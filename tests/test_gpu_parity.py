@@ -454,3 +454,37 @@ def test_tile_parameters_do_not_change_the_values(ctx, etype, n, form, ndn, coef
                 assert np.array_equal(nz, ref), params
             K.close()
             m.close()
+
+
+@pytest.mark.parametrize("etype", [O.H8, O.T10])
+def test_inverted_elements_match_the_oracle(ctx, etype):
+    """Elements with J w < 0 contribute with that sign, as the reference's
+    `Jac * w[j]` does (src/FEMMBaseModule.jl:1494-1528, 1800-1806: no check
+    on the sign).  Every fifth element is mirrored (H8: bottom and top faces
+    swapped; T10: vertices 1 and 2 and their edge nodes swapped), which sends
+    the fused kernel through its signed path."""
+    n = (5, 4, 3) if etype == O.H8 else (3, 3, 2)
+    conn, xyz = mesh_for(etype, n, 0.1, 3)
+    perm = [4, 5, 6, 7, 0, 1, 2, 3] if etype == O.H8 else [0, 2, 1, 3, 6, 5, 4, 7, 9, 8]
+    conn = conn.copy()
+    conn[::5] = conn[::5][:, perm]
+    rule, order = rule_for(etype)
+    D = D200 if etype == O.H8 else D70
+    for form, ndn, coeff in ((L.LIN_ELASTIC, 3, D), (L.DIFFUSION_ISO, 1, [1.3]), (L.DOT, 3, 2.7e-9 * np.eye(3))):
+        ocp, orv, onz = oracle_csc(etype, form, conn, xyz, ndn, coeff)
+        for path in ("fused", "staged", "coo"):
+            for prec, tol in ((L.FP64, FP64_TOL), (L.MIXED, MIXED_TOL)):
+                if path == "coo" and prec == L.MIXED:
+                    continue
+                say("inverted", etype, form, path, prec)
+                m = core.DeviceMesh(ctx, etype, conn, xyz)
+                m.set_dofs(ndn)
+                if path != "coo":
+                    m.set_numeric_path(path)
+                K = m.bilform(form, rule, order, np.ravel(coeff), precision=prec,
+                              path="coo" if path == "coo" else "auto")
+                cp, rv, nz = K.csc_arrays(1)
+                assert np.array_equal(cp, ocp) and np.array_equal(rv, orv)
+                assert rel_fro(nz, onz) <= tol, (form, path, prec, rel_fro(nz, onz))
+                K.close()
+                m.close()
