@@ -360,6 +360,10 @@ extern "C" int feasm_mesh_create_device(feasm_ctx *ctx, int elem_type, int64_t n
                              ctx->stream));
     FEASM_HIP(hipMemcpyAsync(m->xyz->p, d_xyz, nnodes * 3 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    // the host path checks every entry; here a kernel does (an entry out of
+    // range would send the pattern kernels outside their arrays)
+    FEASM_REQUIRE(conn_in_range(ctx, dp<int32_t>(m->conn), nelem * npe, nnodes), FEASM_EINVAL,
+                  "connectivity refers to a missing node");
     *out = m.release();
     API_CATCH(ctx)
 }

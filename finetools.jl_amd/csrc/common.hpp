@@ -215,6 +215,8 @@ feasm_matrix *principal_submatrix(feasm_ctx *ctx, const feasm_matrix *a, const u
 // pattern.hip
 void build_node_to_element(feasm_mesh *m);
 std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn);
+// every entry of the device connectivity d_conn[0..n) lies in [0, nnodes)
+bool conn_in_range(feasm_ctx *ctx, const int32_t *d_conn, int64_t n, int64_t nnodes);
 void build_contributions(feasm_mesh *m, Pattern &p);
 
 // element.hip

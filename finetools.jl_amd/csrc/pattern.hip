@@ -24,6 +24,15 @@ namespace {
 
 constexpr int kCap = 256;  // max distinct neighbours of one node
 
+// flags connectivity entries outside [0, nnodes)
+__global__ void k_conn_range(const int32_t *__restrict__ conn, int64_t n, int64_t nnodes,
+                             unsigned int *__restrict__ bad) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = conn[i];
+        if (v < 0 || v >= nnodes) atomicOr(bad, 1u);
+    }
+}
+
 __global__ void k_n2e_count(const int32_t *__restrict__ conn, int64_t n, int32_t *__restrict__ cnt) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         atomicAdd(&cnt[conn[i]], 1);
@@ -175,6 +184,19 @@ void exclusive_scan_i64(feasm_ctx *ctx, const int64_t *in, int64_t *out, int64_t
     void *tmp = ctx->scratch_get(bytes);
     FEASM_HIP(
         rocprim::exclusive_scan(tmp, bytes, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), ctx->stream));
+}
+
+bool conn_in_range(feasm_ctx *ctx, const int32_t *d_conn, int64_t n, int64_t nnodes) {
+    if (n == 0) return true;
+    DevPtr flag = dev_alloc(sizeof(unsigned int));
+    FEASM_HIP(hipMemsetAsync(flag->p, 0, sizeof(unsigned int), ctx->stream));
+    hipLaunchKernelGGL(k_conn_range, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, d_conn, n, nnodes,
+                       dp<unsigned int>(flag));
+    FEASM_HIP(hipGetLastError());
+    unsigned int bad = 0;
+    FEASM_HIP(hipMemcpyAsync(&bad, flag->p, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
+    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    return bad == 0;
 }
 
 void build_node_to_element(feasm_mesh *m) {

@@ -107,7 +107,8 @@ int feasm_copy_calibration(feasm_ctx *ctx, int64_t nbytes, int repeats, double *
 int feasm_mesh_create(feasm_ctx *ctx, int elem_type, int64_t nelem, int npe, const int64_t *conn,
                       int64_t nnodes, const double *xyz_colmajor, feasm_mesh **out);
 /* Same, from device memory already resident in HBM: conn0 int32 0-based
- * row-major (nelem x npe), xyz row-major (nnodes x 3).  The mesh copies them. */
+ * row-major (nelem x npe), xyz row-major (nnodes x 3).  The mesh copies them;
+ * an entry of conn0 outside [0, nnodes) is FEASM_EINVAL, as on the host path. */
 int feasm_mesh_create_device(feasm_ctx *ctx, int elem_type, int64_t nelem, int npe, const int32_t *d_conn0,
                              int64_t nnodes, const double *d_xyz, feasm_mesh **out);
 int feasm_mesh_destroy(feasm_mesh *mesh);

@@ -488,3 +488,69 @@ def test_inverted_elements_match_the_oracle(ctx, etype):
                 assert rel_fro(nz, onz) <= tol, (form, path, prec, rel_fro(nz, onz))
                 K.close()
                 m.close()
+
+
+def _hip():
+    """The HIP runtime libfeasm links (the system ROCm), for device buffers."""
+    import ctypes.util
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so", ctypes.util.find_library("amdhip64")):
+        if not name:
+            continue
+        try:
+            return ctypes.CDLL(name)
+        except OSError:
+            continue
+    pytest.skip("HIP runtime library not found")
+
+
+def test_device_resident_mesh_matches_host_mesh_and_checks_connectivity(ctx):
+    """feasm_mesh_create_device: a mesh made from device arrays assembles
+    bitwise like the same mesh made from host arrays, and a connectivity entry
+    outside [0, nnodes) is rejected with the host path's error."""
+    hip = _hip()
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    conn, xyz = mesh_for(O.H8, (4, 3, 3), 0.1, 8)
+    conn0 = np.ascontiguousarray(conn - 1, dtype=np.int32)
+    xyzr = np.ascontiguousarray(xyz, dtype=np.float64)  # row-major nnodes x 3
+    bufs = []
+
+    def upload(a):
+        p = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(p), a.nbytes) == 0
+        bufs.append(p)
+        assert hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # hipMemcpyHostToDevice
+        return p
+
+    try:
+        d_conn, d_xyz = upload(conn0), upload(xyzr)
+        lib = L.lib()
+        h = ctypes.c_void_p()
+        rc = lib.feasm_mesh_create_device(ctx.h, L.H8, conn.shape[0], 8, d_conn, xyz.shape[0], d_xyz,
+                                          ctypes.byref(h))
+        assert rc == 0, lib.feasm_last_error(ctx.h)
+        dev = core.DeviceMesh.__new__(core.DeviceMesh)  # wraps the handle made above
+        dev.ctx, dev.h, dev.etype, dev.ndn = ctx, h, O.H8, 0
+        dev.nelem, dev.npe, dev.nnodes = conn.shape[0], 8, xyz.shape[0]
+        host = core.DeviceMesh(ctx, O.H8, conn, xyz)
+        vals = []
+        for m in (dev, host):
+            m.set_dofs(3)
+            K = m.bilform(L.LIN_ELASTIC, L.RULE_GAUSS, 2, D200)
+            vals.append(K.csc_arrays(1))
+            K.close()
+        for a, b in zip(*vals):
+            assert np.array_equal(a, b)
+        host.close()
+        dev.close()
+        bad = conn0.copy()
+        bad[3, 5] = xyz.shape[0]  # one past the last node
+        d_bad = upload(bad)
+        h2 = ctypes.c_void_p()
+        rc = lib.feasm_mesh_create_device(ctx.h, L.H8, conn.shape[0], 8, d_bad, xyz.shape[0], d_xyz,
+                                          ctypes.byref(h2))
+        assert rc == L.EINVAL and not h2.value
+    finally:
+        for p in bufs:
+            hip.hipFree(p)
