@@ -168,7 +168,8 @@ int feasm_pattern_info(feasm_mesh *mesh, int64_t *nnz, int64_t *nblocks);
  * F[dofnums[i]] += elvec[i] element by element on a zeroed vector.  force has
  * one value per DOF of a node (nforce == ndn).  The DOF numbers set on the
  * mesh must be distinct.  F_device: nalldofs doubles on the device, entry
- * d-1 for DOF d. */
+ * d-1 for DOF d, written by kernels enqueued on the context's stream
+ * (feasm_ctx_stream): synchronise with it before reading F_device elsewhere. */
 int feasm_linform_dot_device(feasm_mesh *mesh, int rule, int order, const double *force, int nforce,
                              double *F_device);
 /* The same into a host array of nalldofs doubles (makevector!'s copy). */

@@ -20,3 +20,24 @@ def ctx():
     c = feasm.Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture(scope="session")
+def hiprt():
+    """The HIP runtime libfeasm links (the system ROCm), loaded with ctypes
+    for device buffers of the tests of the device-pointer entry points."""
+    import ctypes
+    import ctypes.util
+
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so", ctypes.util.find_library("amdhip64")):
+        if not name:
+            continue
+        try:
+            h = ctypes.CDLL(name)
+        except OSError:
+            continue
+        h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        h.hipFree.argtypes = [ctypes.c_void_p]
+        return h
+    pytest.skip("HIP runtime library not found")

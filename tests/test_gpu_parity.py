@@ -490,27 +490,11 @@ def test_inverted_elements_match_the_oracle(ctx, etype):
                 m.close()
 
 
-def _hip():
-    """The HIP runtime libfeasm links (the system ROCm), for device buffers."""
-    import ctypes.util
-    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so", ctypes.util.find_library("amdhip64")):
-        if not name:
-            continue
-        try:
-            return ctypes.CDLL(name)
-        except OSError:
-            continue
-    pytest.skip("HIP runtime library not found")
-
-
-def test_device_resident_mesh_matches_host_mesh_and_checks_connectivity(ctx):
+def test_device_resident_mesh_matches_host_mesh_and_checks_connectivity(ctx, hiprt):
     """feasm_mesh_create_device: a mesh made from device arrays assembles
     bitwise like the same mesh made from host arrays, and a connectivity entry
     outside [0, nnodes) is rejected with the host path's error."""
-    hip = _hip()
-    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
-    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-    hip.hipFree.argtypes = [ctypes.c_void_p]
+    hip = hiprt
     conn, xyz = mesh_for(O.H8, (4, 3, 3), 0.1, 8)
     conn0 = np.ascontiguousarray(conn - 1, dtype=np.int32)
     xyzr = np.ascontiguousarray(xyz, dtype=np.float64)  # row-major nnodes x 3
