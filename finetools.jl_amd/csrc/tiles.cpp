@@ -112,24 +112,40 @@ struct TileOut {
 
 // Per-thread lookup tables of make_tile: an element's halo index, a node's
 // index among the halo nodes, a row node's position in the current column
-// (written for the current tile or column before they are read).
+// (written for the current tile or column before they are read), a node's
+// index among the tile's owned nodes (-1 when not owned; reset after each
+// tile), and the tile in which an element or a node was last collected.
 struct TileScratch {
-    std::vector<int32_t> ehalo, npos, rpos;
-    TileScratch(int64_t ne, int64_t nn) : ehalo(ne), npos(nn), rpos(nn) {}
+    std::vector<int32_t> ehalo, npos, rpos, opos, estamp, nstamp;
+    int32_t epoch = 0;
+    TileScratch(int64_t ne, int64_t nn)
+        : ehalo(ne), npos(nn), rpos(nn), opos(nn, -1), estamp(ne, -1), nstamp(nn, -1) {}
 };
 
 void make_tile(const HostMesh &h, const int32_t *owned, int P, TileScratch &sc, TileOut &o) {
     const int npe = h.npe;
+    // the halo elements and their nodes, each once, ascending
+    const int32_t epoch = ++sc.epoch;
     std::vector<int32_t> halo;
     for (int c = 0; c < P; c++)
-        for (int64_t s = h.n2e_ptr[owned[c]]; s < h.n2e_ptr[owned[c] + 1]; s++) halo.push_back((int32_t)(h.n2e[s] >> 4));
+        for (int64_t s = h.n2e_ptr[owned[c]]; s < h.n2e_ptr[owned[c] + 1]; s++) {
+            const int32_t e = (int32_t)(h.n2e[s] >> 4);
+            if (sc.estamp[e] != epoch) {
+                sc.estamp[e] = epoch;
+                halo.push_back(e);
+            }
+        }
     std::sort(halo.begin(), halo.end());
-    halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
     std::vector<int32_t> hn;
     for (int32_t e : halo)
-        for (int a = 0; a < npe; a++) hn.push_back(h.conn[(int64_t)e * npe + a]);
+        for (int a = 0; a < npe; a++) {
+            const int32_t v = h.conn[(int64_t)e * npe + a];
+            if (sc.nstamp[v] != epoch) {
+                sc.nstamp[v] = epoch;
+                hn.push_back(v);
+            }
+        }
     std::sort(hn.begin(), hn.end());
-    hn.erase(std::unique(hn.begin(), hn.end()), hn.end());
     o.P = P;
     o.H = (int)halo.size();
     o.NH = (int)hn.size();
@@ -156,6 +172,11 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileScratch &sc, 
     std::vector<size_t> tfirst(P);
     o.cbase.resize(P);
     o.cdeg.resize(P);
+    {
+        int64_t nb = 0;
+        for (int c = 0; c < P; c++) nb += h.adj_ptr[owned[c] + 1] - h.adj_ptr[owned[c]];
+        tasks.reserve(nb);
+    }
     for (int pass = 0; pass < 2; pass++) {
         if (pass == 1) {  // offsets from the counts; the counts restart as fill positions
             int off = 0;
@@ -206,15 +227,12 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileScratch &sc, 
     // one task computes both (the element lists are the same, and so are the
     // products, in the same order).
     {
-        std::vector<std::pair<int32_t, int>> own(P);
-        for (int c = 0; c < P; c++) own[c] = {owned[c], c};
-        std::sort(own.begin(), own.end());
+        for (int c = 0; c < P; c++) sc.opos[owned[c]] = c;
         for (Task &T : tasks) {
             if (T.diag || T.drop) continue;
             const int32_t m = h.adj[o.cbase[T.c] + T.r];
-            auto f = std::lower_bound(own.begin(), own.end(), std::make_pair(m, -1));
-            if (f == own.end() || f->first != m || f->second < T.c) continue;
-            const int c2 = f->second;
+            const int c2 = sc.opos[m];
+            if (c2 < T.c) continue;  // not owned (-1), or its task is the one kept
             const int32_t *rows2 = h.adj.data() + o.cbase[c2];
             const int r2 = (int)(std::lower_bound(rows2, rows2 + o.cdeg[c2], owned[T.c]) - rows2);
             FEASM_REQUIRE(r2 < o.cdeg[c2] && rows2[r2] == owned[T.c], FEASM_ESTATE, "pattern is not symmetric");
@@ -222,6 +240,7 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileScratch &sc, 
             T.r2 = r2;
             tasks[tfirst[c2] + r2].drop = true;
         }
+        for (int c = 0; c < P; c++) sc.opos[owned[c]] = -1;
         tasks.erase(std::remove_if(tasks.begin(), tasks.end(), [](const Task &T) { return T.drop; }), tasks.end());
     }
     for (const Task &T : tasks)  // the kernel reads a task's first item unconditionally
@@ -235,12 +254,20 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileScratch &sc, 
             if (((its[T.off + j] >> 4) & 15u) != (its[T.off + j] & 15u)) T.sym = false;
     }
     // longest first; among equal counts the symmetric diagonal blocks first,
-    // so that they fill wavefronts of their own
-    std::stable_sort(tasks.begin(), tasks.end(), [](const Task &x, const Task &y) {
-        return x.k != y.k ? x.k > y.k : (x.sym && !y.sym);
-    });
-    const int K = tasks.empty() ? 0 : tasks.front().k;
+    // so that they fill wavefronts of their own (a stable counting sort on
+    // (k, sym): the order of std::stable_sort by that key)
+    int K = 0;
+    for (const Task &T : tasks) K = std::max(K, T.k);
     FEASM_REQUIRE(K <= 255, FEASM_EUNSUPPORTED, "a node block is shared by more than 255 elements");
+    {
+        auto bucket = [K](const Task &T) { return 2 * (K - T.k) + (T.sym ? 0 : 1); };
+        std::vector<int> start(2 * K + 3, 0);
+        for (const Task &T : tasks) start[bucket(T) + 1]++;
+        for (size_t b = 1; b < start.size(); b++) start[b] += start[b - 1];
+        std::vector<Task> sorted(tasks.size());
+        for (const Task &T : tasks) sorted[start[bucket(T)]++] = T;
+        tasks.swap(sorted);
+    }
     o.K = K;
     // Wavefront schedule: 64 consecutive tasks run as one wavefront, and the
     // wavefronts of a workgroup go to the 4 SIMDs in turn.  Chunks of 64
@@ -271,6 +298,10 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileScratch &sc, 
     o.toff.clear();
     o.tpair.clear();
     o.item.clear();
+    o.task.reserve(rounds * NS * W);
+    o.toff.reserve(rounds * NS * W);
+    o.tpair.reserve(rounds * NS * W);
+    o.item.reserve(its.size());
     for (size_t rr = 0; rr < rounds; rr++)
         for (int q = 0; q < NS; q++) {
             const int c0 = rr < queue[q].size() ? queue[q][rr] : -1;
