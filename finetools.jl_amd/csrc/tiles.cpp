@@ -158,8 +158,9 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileScratch &sc, 
     for (int i = 0; i < o.H; i++)
         for (int a = 0; a < npe; a++) o.lconn[(size_t)i * npe + a] = (uint16_t)sc.npos[h.conn[(int64_t)halo[i] * npe + a]];
     // block tasks: items of (column c, row position r) in ascending element,
-    // held in one array (task i's items at [off, off + k)); two passes over
-    // the owned nodes' elements: count, then fill
+    // held in one array (task i's items at [off, off + k), tasks in (c, r)
+    // order); per column, the column's items are gathered in element order,
+    // counted per row, then placed
     struct Task {
         int c, r;
         bool sw, diag, sym = false;
@@ -173,54 +174,57 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileScratch &sc, 
     o.cbase.resize(P);
     o.cdeg.resize(P);
     {
-        int64_t nb = 0;
-        for (int c = 0; c < P; c++) nb += h.adj_ptr[owned[c] + 1] - h.adj_ptr[owned[c]];
-        tasks.reserve(nb);
-    }
-    for (int pass = 0; pass < 2; pass++) {
-        if (pass == 1) {  // offsets from the counts; the counts restart as fill positions
-            int off = 0;
-            for (Task &T : tasks) {
-                T.off = off;
-                off += T.k;
-                T.k = 0;
-            }
-            its.resize(off);
-        }
+        int64_t nb = 0, ni = 0;
         for (int c = 0; c < P; c++) {
-            const int32_t node = owned[c];
-            const int64_t b0 = h.adj_ptr[node], deg = h.adj_ptr[node + 1] - b0;
-            const int32_t *rows = h.adj.data() + b0;
-            if (pass == 0) {
-                FEASM_REQUIRE(deg <= 255, FEASM_EUNSUPPORTED, "a node has more than 255 neighbours");
-                o.cbase[c] = b0;
-                o.cdeg[c] = (int32_t)deg;
-                tfirst[c] = tasks.size();
-                for (int r = 0; r < deg; r++) {
-                    Task T;
-                    T.c = c;
-                    T.r = r;
-                    T.sw = rows[r] > node;
-                    T.diag = rows[r] == node;
-                    tasks.push_back(T);
-                }
+            nb += h.adj_ptr[owned[c] + 1] - h.adj_ptr[owned[c]];
+            ni += (h.n2e_ptr[owned[c] + 1] - h.n2e_ptr[owned[c]]) * npe;
+        }
+        tasks.reserve(nb);
+        its.reserve(ni);
+    }
+    std::vector<std::pair<int, uint16_t>> colits;  // (row position, item) of the current column
+    for (int c = 0; c < P; c++) {
+        const int32_t node = owned[c];
+        const int64_t b0 = h.adj_ptr[node], deg = h.adj_ptr[node + 1] - b0;
+        const int32_t *rows = h.adj.data() + b0;
+        FEASM_REQUIRE(deg <= 255, FEASM_EUNSUPPORTED, "a node has more than 255 neighbours");
+        o.cbase[c] = b0;
+        o.cdeg[c] = (int32_t)deg;
+        const size_t tb = tasks.size();
+        tfirst[c] = tb;
+        for (int r = 0; r < deg; r++) {
+            Task T;
+            T.c = c;
+            T.r = r;
+            T.sw = rows[r] > node;
+            T.diag = rows[r] == node;
+            tasks.push_back(T);
+            sc.rpos[rows[r]] = r;
+        }
+        colits.clear();
+        for (int64_t s = h.n2e_ptr[node]; s < h.n2e_ptr[node + 1]; s++) {
+            const int32_t e = (int32_t)(h.n2e[s] >> 4);
+            const int ac = (int)(h.n2e[s] & 15u);
+            const int hl = sc.ehalo[e];
+            for (int a = 0; a < npe; a++) {
+                const int32_t v = h.conn[(int64_t)e * npe + a];
+                const int r = sc.rpos[v];
+                FEASM_REQUIRE(r >= 0 && r < deg && rows[r] == v, FEASM_ESTATE, "pattern misses an element node pair");
+                colits.push_back({r, (uint16_t)((hl << 8) | (ac << 4) | a)});
+                tasks[tb + r].k++;
             }
-            const size_t tb = tfirst[c];
-            for (int r = 0; r < deg; r++) sc.rpos[rows[r]] = r;
-            for (int64_t s = h.n2e_ptr[node]; s < h.n2e_ptr[node + 1]; s++) {
-                const int32_t e = (int32_t)(h.n2e[s] >> 4);
-                const int ac = (int)(h.n2e[s] & 15u);
-                const int hl = sc.ehalo[e];
-                for (int a = 0; a < npe; a++) {
-                    const int32_t v = h.conn[(int64_t)e * npe + a];
-                    const int r = sc.rpos[v];
-                    FEASM_REQUIRE(r >= 0 && r < deg && rows[r] == v, FEASM_ESTATE,
-                                  "pattern misses an element node pair");
-                    Task &T = tasks[tb + r];
-                    if (pass == 1) its[T.off + T.k] = (uint16_t)((hl << 8) | (ac << 4) | a);
-                    T.k++;
-                }
-            }
+        }
+        int off = (int)its.size();
+        for (int r = 0; r < deg; r++) {
+            Task &T = tasks[tb + r];
+            T.off = off;
+            off += T.k;
+            T.k = 0;
+        }
+        its.resize(off);
+        for (const auto &ri : colits) {
+            Task &T = tasks[tb + ri.first];
+            its[T.off + T.k++] = ri.second;
         }
     }
     // A block whose row node is owned too has its transpose in this tile:
@@ -250,7 +254,7 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, TileScratch &sc, 
     // upper triangle from one vector per point.
     for (Task &T : tasks) {
         T.sym = T.diag;
-        for (int j = 0; j < T.k; j++)
+        for (int j = 0; T.sym && j < T.k; j++)
             if (((its[T.off + j] >> 4) & 15u) != (its[T.off + j] & 15u)) T.sym = false;
     }
     // longest first; among equal counts the symmetric diagonal blocks first,
