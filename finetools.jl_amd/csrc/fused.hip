@@ -349,7 +349,8 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
                            RJ(1, 3) * (RJ(2, 1) * RJ(3, 2) - RJ(2, 2) * RJ(3, 1));
         double wj = (FORM == FEASM_FORM_DIFFUSION_ISO) ? (coef[0] * det) * sWq[q] : det * sWq[q];
         if (GRAD) {
-            const double invdet = 1.0 / det;
+            // the inverse scaled by sqrt(|J w|) once: g_a scaled is inv(J)' dN_a times it
+            const double invdet = (1.0 / det) * sqrt(fabs(wj));
             const double iv[9] = {(RJ(2, 2) * RJ(3, 3) - RJ(3, 2) * RJ(2, 3)) * invdet,
                                   -(RJ(1, 2) * RJ(3, 3) - RJ(1, 3) * RJ(3, 2)) * invdet,
                                   (RJ(1, 2) * RJ(2, 3) - RJ(1, 3) * RJ(2, 2)) * invdet,
@@ -359,7 +360,6 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
                                   (RJ(2, 1) * RJ(3, 2) - RJ(3, 1) * RJ(2, 2)) * invdet,
                                   -(RJ(1, 1) * RJ(3, 2) - RJ(3, 1) * RJ(1, 2)) * invdet,
                                   (RJ(1, 1) * RJ(2, 2) - RJ(2, 1) * RJ(1, 2)) * invdet};
-            const R r = (R)sqrt(fabs(wj));
             sSg[h * NQP + q] = (wj < 0.0) ? 1 : 0;
             if (wj < 0.0) atomicOr(sNeg, 1);
             R ivr[9];
@@ -376,9 +376,9 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
                 const int a = (i + q) % NPE;
                 const double2 dd = dxy[a];
                 const R d0 = (R)dd.x, d1 = (R)dd.y, d2 = (R)dz[a];
-                const R u0 = r * (d0 * ivr[0] + d1 * ivr[3] + d2 * ivr[6]);
-                const R u1 = r * (d0 * ivr[1] + d1 * ivr[4] + d2 * ivr[7]);
-                const R u2 = r * (d0 * ivr[2] + d1 * ivr[5] + d2 * ivr[8]);
+                const R u0 = d0 * ivr[0] + d1 * ivr[3] + d2 * ivr[6];
+                const R u1 = d0 * ivr[1] + d1 * ivr[4] + d2 * ivr[7];
+                const R u2 = d0 * ivr[2] + d1 * ivr[5] + d2 * ivr[8];
                 const int cxy = (q / PX) * NPE + a, cz = (NQP / PX) * NPE + (q / PZ) * NPE + a;
                 if constexpr (sizeof(R) == 8)
                     reinterpret_cast<double2 *>(urec)[cxy] = make_double2(u0, u1);
