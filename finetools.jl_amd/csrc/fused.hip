@@ -329,6 +329,11 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
         const double2 *dxy = reinterpret_cast<const double2 *>(sDN) + q * NPE;
         const double *dz = sDN + 2 * NQP * NPE + q * NPE;
         double J[9];
+        // FP64: the point's gradient table, read once (the stores of the
+        // scaled gradients below could alias it, so a second read would be
+        // reloaded); measured no faster for FP32, whose kernels keep the reload
+        constexpr bool TQ = sizeof(R) == 8;
+        double tq[TQ ? 3 * NPE : 1];
 #pragma unroll
         for (int i = 0; i < 9; i++) J[i] = 0.0;
 #pragma unroll
@@ -339,6 +344,11 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
             const double x0 = xy.x, x1 = xy.y, x2 = sZ[n];
             const double2 dd = dxy[a];
             const double d0 = dd.x, d1 = dd.y, d2 = dz[a];
+            if constexpr (TQ) {
+                tq[3 * i] = d0;
+                tq[3 * i + 1] = d1;
+                tq[3 * i + 2] = d2;
+            }
             J[0] += x0 * d0; J[1] += x0 * d1; J[2] += x0 * d2;
             J[3] += x1 * d0; J[4] += x1 * d1; J[5] += x1 * d2;
             J[6] += x2 * d0; J[7] += x2 * d1; J[8] += x2 * d2;
@@ -374,8 +384,17 @@ __device__ __forceinline__ void tile_compute(const TileShape &shape, const TileJ
 #pragma unroll
             for (int i = 0; i < NPE; i++) {
                 const int a = (i + q) % NPE;
-                const double2 dd = dxy[a];
-                const R d0 = (R)dd.x, d1 = (R)dd.y, d2 = (R)dz[a];
+                R d0, d1, d2;
+                if constexpr (TQ) {
+                    d0 = tq[3 * i];
+                    d1 = tq[3 * i + 1];
+                    d2 = tq[3 * i + 2];
+                } else {
+                    const double2 dd = dxy[a];
+                    d0 = (R)dd.x;
+                    d1 = (R)dd.y;
+                    d2 = (R)dz[a];
+                }
                 const R u0 = d0 * ivr[0] + d1 * ivr[3] + d2 * ivr[6];
                 const R u1 = d0 * ivr[1] + d1 * ivr[4] + d2 * ivr[7];
                 const R u2 = d0 * ivr[2] + d1 * ivr[5] + d2 * ivr[8];
