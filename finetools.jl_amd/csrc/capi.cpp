@@ -80,6 +80,18 @@ FormParams feasm::make_form_params(int form, int ndn, const double *coeff, int n
             break;
         default: throw Error(FEASM_EINVAL, "unknown form");
     }
+    // the reference keeps the upper triangle of each element matrix
+    // (add_btdb_ut_only!/add_gkgt_ut_only! + complete_lt!,
+    // src/MatrixUtilityModule.jl:120-216), which for a non-symmetric D or
+    // kappa is not the matrix the node-block kernels form: such coefficients
+    // are refused
+    if (form == FEASM_FORM_DIFFUSION_MAT || form == FEASM_FORM_LIN_ELASTIC) {
+        const int d = form == FEASM_FORM_LIN_ELASTIC ? 6 : 3;
+        for (int i = 0; i < d; i++)
+            for (int j = i + 1; j < d; j++)
+                FEASM_REQUIRE(coeff[i * d + j] == coeff[j * d + i], FEASM_EUNSUPPORTED,
+                              "the material matrix must be symmetric");
+    }
     for (int i = 0; i < ncoeff; i++) fp.coef[i] = coeff[i];
     return fp;
 }
@@ -96,9 +108,7 @@ void check_rule(const feasm_mesh *m, int rule) {
 // the DOF checks of assemble! (src/AssemblyModule.jl:265-273) for a general
 // numbering, in the order the reference meets them: element by element,
 // columns then rows
-void check_dofs_like_assemble(const feasm_mesh *m, const std::vector<int64_t> &dofs1) {
-    int64_t n = m->nalldofs;
-    const int ndn = m->ndn;
+void check_dofs_like_assemble(const feasm_mesh *m, int ndn, int64_t n, const std::vector<int64_t> &dofs1) {
     std::vector<int32_t> conn(m->nelem * m->npe);
     FEASM_HIP(hipMemcpy(conn.data(), m->conn->p, conn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
     // rows == cols == the element's DOFs: the first column is checked, then
@@ -378,7 +388,10 @@ extern "C" int feasm_mesh_destroy(feasm_mesh *mesh) {
 extern "C" int feasm_mesh_set_xyz(feasm_mesh *m, const double *xyz_rowmajor) {
     API_TRY
     FEASM_REQUIRE(m && xyz_rowmajor, FEASM_EINVAL, "null argument");
-    FEASM_HIP(hipMemcpy(m->xyz->p, xyz_rowmajor, m->nnodes * 3 * sizeof(double), hipMemcpyHostToDevice));
+    // ordered after the assemblies already queued on the context stream
+    // (feasm_bilform_into returns before its kernels finish)
+    FEASM_HIP(hipMemcpyAsync(m->xyz->p, xyz_rowmajor, m->nnodes * 3 * sizeof(double), hipMemcpyHostToDevice,
+                             m->ctx->stream));
     for (auto &kv : m->tiles)  // the fused kernel's tiles carry copies of the coordinates
         if (kv.second) refresh_tile_coords(m, *kv.second);
     FEASM_HIP(hipStreamSynchronize(m->ctx->stream));
@@ -416,8 +429,10 @@ extern "C" int feasm_dofs_set(feasm_mesh *m, int ndn, const int64_t *dofnums_col
             rm[n * ndn + k] = d;
             if (d != n * ndn + k + 1) nm = false;
         }
+    // checked before the mesh changes: a rejected numbering leaves the mesh
+    // as it was
+    if (!nm) check_dofs_like_assemble(m, ndn, nalldofs, rm);
     set_dofs(m, ndn, rm, nalldofs, nm);
-    if (!nm) check_dofs_like_assemble(m, rm);
     API_CATCH(m ? m->ctx : nullptr)
 }
 

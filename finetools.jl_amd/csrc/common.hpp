@@ -251,3 +251,38 @@ inline unsigned grid_for(int64_t n, int block, unsigned cap = 1u << 20) {
 }
 
 }  // namespace feasm
+
+namespace feasm {
+
+// coo.hip: the two phases of the device sparse().  Temporaries of one call
+// on nt triples: the sort ping-pongs between (keys, keys_alt) and (idx,
+// idx_alt); afterwards the buffers not holding the sorted data take the
+// segment heads (4 B per triple) and numbers (8 B).
+struct SparseWork {
+    uint64_t *keys = nullptr, *keys_alt = nullptr;
+    uint32_t *idx = nullptr, *idx_alt = nullptr;
+    int64_t *start = nullptr;  // nnz entries (phase B)
+    void *tmp = nullptr;       // sort and scan storage
+    size_t tmp_bytes = 0;
+    unsigned int *flag = nullptr;
+    int64_t *count = nullptr;  // [nnz, flag] read back after phase A
+    // where phase A left the sorted keys / payload and the segment arrays
+    const uint64_t *skeys = nullptr;
+    const uint32_t *sidx = nullptr;
+    int32_t *head = nullptr;
+    int64_t *seg = nullptr;
+};
+int bits_for(uint64_t v);
+// sort + scan storage for nt triples over end_bit key bits
+size_t sort_tmp_bytes(int64_t nt, unsigned end_bit, hipStream_t s);
+// keys + range check, stable sort, segment heads and numbers, one read-back:
+// returns nnz, throws FEASM_EDOF for an index outside [0,m) x [0,n)
+int64_t sparse_phase_a(hipStream_t s, int64_t nt, const int32_t *I, const int32_t *J, int64_t m, int64_t n,
+                       unsigned end_bit, SparseWork &w);
+// segment starts, ordered sums, column pointers
+void sparse_phase_b(hipStream_t s, int64_t nt, const double *V, int64_t m, int64_t n, int64_t nnz,
+                    const SparseWork &w, int64_t *colptr, int32_t *rowval, double *nzval);
+void check_sparse_args(int64_t nt, int64_t m, int64_t n);
+void fill_i64(hipStream_t s, int64_t *p, int64_t n, int64_t v);
+
+}  // namespace feasm

@@ -72,6 +72,7 @@ __global__ __launch_bounds__(256) void k_vec_gather(const int64_t *__restrict__ 
                                                     int ndn, const double *__restrict__ stage,
                                                     const int32_t *__restrict__ dofs, double *__restrict__ F) {
   for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < nnodes; n += (int64_t)gridDim.x * blockDim.x) {
+    if (n2e_ptr[n] == n2e_ptr[n + 1]) continue;  // a node of no element: its DOFs are never touched
     double acc[3] = {0.0, 0.0, 0.0};
     for (int64_t s = n2e_ptr[n]; s < n2e_ptr[n + 1]; s++) {
         const uint32_t v = n2e[s];
@@ -133,6 +134,7 @@ __global__ __launch_bounds__(256) void k_vec_factor_gather(const int64_t *__rest
     __syncthreads();
     for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < nnodes;
          n += (int64_t)gridDim.x * blockDim.x) {
+        if (n2e_ptr[n] == n2e_ptr[n + 1]) continue;  // a node of no element: its DOFs are never touched
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
         for (int64_t s = n2e_ptr[n]; s < n2e_ptr[n + 1]; s++) {
             const uint32_t v = n2e[s];
