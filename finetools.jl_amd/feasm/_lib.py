@@ -65,6 +65,7 @@ SIGNATURES = {
     "feasm_coo_destroy": (I32, [P]),
     "feasm_sparse_device": (I32, [P, I64, P, P, P, I64, I64, PP]),
     "feasm_sparse": (I32, [P, I64, P, P, P, I64, I64, PP]),
+    "feasm_sparse_device_noalloc": (I32, [P, I64, P, P, P, I64, I64, P, P, P, PI64]),
     "feasm_batch_create": (I32, [P, PP]),
     "feasm_batch_add": (I32, [P, P, I32, I32, I32, P, I32, I32]),
     "feasm_batch_prepare": (I32, [P, I32, I32, I32, I32]),

@@ -246,6 +246,20 @@ int feasm_coo_destroy(feasm_coo *coo);
 int feasm_sparse_device(feasm_ctx *ctx, int64_t ntrip, const int32_t *d_I, const int32_t *d_J, const double *d_V,
                         int64_t m, int64_t n, feasm_matrix **out);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  feasm_sparse_device into caller arrays -- d_colptr n+1 int64
+ * (0-based), d_rowval and d_nzval with room for ntrip entries -- with the
+ * same kernels and results (bitwise); *nnz receives the entry count.  Every
+ * temporary lives in one fixed device array per GPU (36 MiB, held until the
+ * process ends): at most FEASM_NOALLOC_MAX_TRIPLES triples per call, more is
+ * FEASM_ENOMEM.  No device memory is allocated or freed.  Calls that overlap
+ * on one GPU share that array and race: serialise them. */
+#define FEASM_NOALLOC_MAX_TRIPLES 1048576
+int feasm_sparse_device_noalloc(feasm_ctx *ctx, int64_t ntrip, const int32_t *d_I, const int32_t *d_J,
+                                const double *d_V, int64_t m, int64_t n, int64_t *d_colptr, int32_t *d_rowval,
+                                double *d_nzval, int64_t *nnz);
+
 /* sparse(I, J, V, m, n) on host triples (1-based int64 indices, as Julia's
  * SparseArrays.sparse takes them at makematrix!, src/AssemblyModule.jl:319-325):
  * uploads them and runs the device sparse(). */
