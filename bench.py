@@ -238,8 +238,9 @@ def bench_t10(feasm, core, n=60, steps=5):
         ctx.set_timing(0)
         nnz = M.nnz
         res[name] = M.csc_arrays(0)[2]
+        gbs = algorithmic_bytes_numeric(10, ne, nn, nnz) / (ms * 1e-3) / 1e9
         out[name] = {"ms": ms, "elements_per_s": ne / (ms * 1e-3), "nnz_per_s": nnz / (ms * 1e-3),
-                     "achieved_GBps": algorithmic_bytes_numeric(10, ne, nn, nnz) / (ms * 1e-3) / 1e9}
+                     "achieved_GBps": gbs, "roofline_frac": gbs / HBM_PEAK_GBS}
         out["nnz"] = nnz
         M.close()
     out["K_mixed"]["rel_fro_vs_fp64"] = rel_fro(res["K_mixed"], res["K_fp64"])
@@ -249,34 +250,35 @@ def bench_t10(feasm, core, n=60, steps=5):
     return out
 
 
-def bench_batched(feasm, core, nmesh=256, seed=4, steps=5, tune_db=None):
+def bench_batched(feasm, core, steps=5, tune_db=None):
     """C4: 256 H8 blocks, nx,ny,nz ~ U{4..64}; half 1-DOF diffusion k~U[0.5,2],
     half 3-DOF elasticity E~U[1e3,1e5] nu~U[0.2,0.45]; mixed precision; one
-    persistent launch replayed from a hipGraph; tile size autotuned."""
-    rng = np.random.default_rng(seed)
+    persistent launch per form replayed from a hipGraph; tile size autotuned.
+    Batched and plain (one bilform_into per mesh, FP64) are compared on the
+    same two clocks: host wall time around `steps` whole passes (each ending
+    in a stream synchronisation), and device time from HIP events."""
+    from feasm import workloads as W
+
     ctx = feasm.Context(0)
-    meshes, specs = [], []
+    specs = W.batched_specs()
+    meshes = []
     t0 = time.perf_counter()
-    ne_tot = 0
-    for i in range(nmesh):
-        nx, ny, nz = (int(v) for v in rng.integers(4, 65, 3))
-        conn, xyz = core.gen_h8block(1.0, 1.0, 1.0, nx, ny, nz)
-        h = 1.0 / max(nx, ny, nz)
-        xyz = core.gen_perturb(xyz, [0, 0, 0], [1, 1, 1], 0.1 * h, 1000 + i)
+    ne_tot = bytes_alg = 0
+    for (n, form, ndn, coeff, seed) in specs:
+        conn, xyz = W.h8_block(n, 0.1, seed)
         m = core.DeviceMesh(ctx, feasm.H8, conn, xyz)
-        if i % 2 == 0:
-            m.set_dofs(1)
-            specs.append((feasm.DIFFUSION_ISO, [float(rng.uniform(0.5, 2.0))]))
-        else:
-            m.set_dofs(3)
-            specs.append((feasm.LIN_ELASTIC, iso_D(float(rng.uniform(1e3, 1e5)), float(rng.uniform(0.2, 0.45)))))
+        m.set_dofs(ndn)
         meshes.append(m)
         ne_tot += conn.shape[0]
-    log("batch: %d meshes, %d elements (setup %.1f s)" % (nmesh, ne_tot, time.perf_counter() - t0))
-    results = {}
+        nnz = ndn * ndn * (3 * n[0] + 1) * (3 * n[1] + 1) * (3 * n[2] + 1)
+        bytes_alg += algorithmic_bytes_numeric(8, conn.shape[0], xyz.shape[0], nnz)
+    log("batch: %d meshes, %d elements (setup %.1f s)" % (len(specs), ne_tot, time.perf_counter() - t0))
+    results = {"workload": "256 H8 blocks, n ~ U{4..64}^3 (seeded), half diffusion half elasticity",
+               "elements": ne_tot, "algorithmic_bytes": bytes_alg}
+    vals = {}
     for label, prec in (("mixed", feasm.MIXED), ("fp64", feasm.FP64)):
         batch = core.Batch(ctx)
-        for m, (form, coeff) in zip(meshes, specs):
+        for m, (n, form, ndn, coeff, _) in zip(meshes, specs):
             batch.add(m, form, feasm.RULE_GAUSS, 2, coeff, precision=prec)
         t0 = time.perf_counter()
         tuned = batch.autotune(db_path=tune_db, key="h8-batch256-%s" % label, reps=3)
@@ -285,34 +287,44 @@ def bench_batched(feasm, core, nmesh=256, seed=4, steps=5, tune_db=None):
         tprep = time.perf_counter() - t0
         batch.run()
         ctx.synchronize()
-        times = []
+        dev = []
+        t0 = time.perf_counter()
         for _ in range(steps):
             batch.run()
-            times.append(batch.last_ms())
+            dev.append(batch.last_ms())  # waits for the replay
+        wall_ms = 1e3 * (time.perf_counter() - t0) / steps
         st = batch.stats()
-        ms = float(np.median(times))
-        results[label] = {"ms": ms, "elements_per_s": st["elements"] / (ms * 1e-3),
-                          "nnz_per_s": st["nnz"] / (ms * 1e-3), "tuned": tuned, "prepare_s": tprep,
-                          "tiles": st["tiles"]}
-        if label == "mixed":
-            vals_mixed = [batch.matrix(i).csc_arrays(0)[2] for i in range(0, nmesh, 16)]
-        else:
-            vals_fp64 = [batch.matrix(i).csc_arrays(0)[2] for i in range(0, nmesh, 16)]
+        ms = float(np.median(dev))
+        results[label] = {"wall_ms": wall_ms, "device_ms": ms, "elements_per_s": st["elements"] / (wall_ms * 1e-3),
+                          "nnz_per_s": st["nnz"] / (wall_ms * 1e-3), "tuned": tuned, "prepare_s": tprep,
+                          "tiles": st["tiles"],
+                          "roofline_frac": bytes_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        vals[label] = [batch.matrix(i).csc_arrays(0)[2] for i in range(0, len(specs), 16)]
         results["nnz"] = st["nnz"]
         batch.close()
-    results["mixed"]["rel_fro_vs_fp64_max"] = max(rel_fro(a, b) for a, b in zip(vals_mixed, vals_fp64))
-    # the plain per-mesh FP64 path, one bilform per mesh, for comparison
-    mats = [m.bilform(form, feasm.RULE_GAUSS, 2, coeff) for m, (form, coeff) in zip(meshes, specs)]
+    results["mixed"]["rel_fro_vs_fp64_max"] = max(rel_fro(a, b) for a, b in zip(vals["mixed"], vals["fp64"]))
+    del vals
+    # the plain per-mesh FP64 path: one bilform_into per mesh, in order
+    mats = [m.bilform(form, feasm.RULE_GAUSS, 2, coeff) for m, (_, form, _, coeff, _) in zip(meshes, specs)]
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        for m, M, (form, coeff) in zip(meshes, mats, specs):
+        for m, M, (_, form, _, coeff, _) in zip(meshes, mats, specs):
+            m.bilform_into(M, form, feasm.RULE_GAUSS, 2, coeff)
+        ctx.synchronize()
+    wall_plain = 1e3 * (time.perf_counter() - t0) / steps
+    # device time in separate passes (the events add host work to the wall time)
+    ctx.set_timing(len(specs) * steps)
+    for _ in range(steps):
+        for m, M, (_, form, _, coeff, _) in zip(meshes, mats, specs):
             m.bilform_into(M, form, feasm.RULE_GAUSS, 2, coeff)
     ctx.synchronize()
-    ms_plain = 1e3 * (time.perf_counter() - t0) / steps
-    results["plain_fp64_per_mesh"] = {"ms": ms_plain, "elements_per_s": ne_tot / (ms_plain * 1e-3)}
-    results["elements"] = ne_tot
-    results["speedup_vs_plain"] = results["mixed"]["elements_per_s"] / results["plain_fp64_per_mesh"]["elements_per_s"]
+    dev_plain = float(ctx.timings()[:, 2].sum()) / steps
+    ctx.set_timing(0)
+    results["plain_fp64_per_mesh"] = {"wall_ms": wall_plain, "device_ms": dev_plain,
+                                      "elements_per_s": ne_tot / (wall_plain * 1e-3)}
+    results["speedup_vs_plain"] = {"wall": wall_plain / results["mixed"]["wall_ms"],
+                                   "device": dev_plain / results["mixed"]["device_ms"]}
     for M in mats:
         M.close()
     for m in meshes:
@@ -321,88 +333,32 @@ def bench_batched(feasm, core, nmesh=256, seed=4, steps=5, tune_db=None):
     return results
 
 
-def bench_served(feasm, core, rate=200.0, duration=3.0, seed=5, pool=48, workers=8):
+def bench_served(feasm, core, rate=200.0, duration=3.0, workers=8):
     """C5: Poisson arrivals (rate/s) of H8/T10 (50/50) block meshes with log-normal
     element counts (median 5e4, sigma 1.0, clipped to [1e3, 2e6]), diffusion or
     elasticity; 30 % re-assemble a resident mesh with new material values.
-    Served (continuous batching, mixed precision) vs the plain per-request FP64
-    path (one request at a time, in arrival order)."""
-    rng = np.random.default_rng(seed)
-    # a pool of distinct meshes generated up front (the requests carry them)
-    meshes = []
+    Served (continuous batching, mixed precision, `workers` streams) vs the
+    plain per-request FP64 path (one request at a time, in arrival order),
+    paced at the arrival times and, for throughput at saturation, all at once."""
+    from feasm import workloads as W
+
     t0 = time.perf_counter()
-    for i in range(pool):
-        ne_target = float(np.clip(np.exp(np.log(5e4) + 1.0 * rng.standard_normal()), 1e3, 2e6))
-        if i % 2 == 0:
-            n = max(1, int(round(ne_target ** (1 / 3))))
-            conn, xyz = core.gen_h8block(1.0, 1.0, 1.0, n, n, n)
-            xyz = core.gen_perturb(xyz, [0, 0, 0], [1, 1, 1], 0.1 / n, 2000 + i)
-            meshes.append((feasm.H8, conn, xyz, (feasm.RULE_GAUSS, 2)))
-        else:
-            n = max(1, int(round((ne_target / 6) ** (1 / 3))))
-            conn, xyz = core.gen_t10block(1.0, 1.0, 1.0, n, n, n, 0.1 / n, 0.03 / n, 2000 + i)
-            meshes.append((feasm.T10, conn, xyz, (feasm.RULE_TET, 4)))
-    log("served: pool of %d meshes generated in %.1f s" % (pool, time.perf_counter() - t0))
-    # the request stream
-    nreq = int(rate * duration)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
-    stream = []
-    for k in range(nreq):
-        diff = rng.random() < 0.5
-        form = feasm.DIFFUSION_ISO if diff else feasm.LIN_ELASTIC
-        coeff = [float(rng.uniform(0.5, 2.0))] if diff else iso_D(float(rng.uniform(1e3, 1e5)),
-                                                                   float(rng.uniform(0.2, 0.45)))
-        stream.append((arrivals[k], int(rng.integers(0, pool)), form, coeff, rng.random() < 0.3))
-
-    def run(precision, workers_, paced=True):
-        srv = core.Server(0, workers=workers_, hbm_fraction=0.6)
-        resident = {}  # (pool index, ndn) -> mesh id
-        pending = []
-        ne_done = 0
-        t_start = time.perf_counter()
-        for (t_arr, pi, form, coeff, re) in stream:
-            while paced and time.perf_counter() - t_start < t_arr:
-                time.sleep(0.0002)
-            et, conn, xyz, rule = meshes[pi]
-            ndn = 1 if form == feasm.DIFFUSION_ISO else 3
-            key = (pi, ndn)
-            if re and key in resident:
-                rid = srv.submit_reassembly(resident[key], form, rule[0], rule[1], coeff, precision)
-            else:
-                rid, mid = srv.submit_mesh(et, conn, xyz, ndn, form, rule[0], rule[1], coeff, precision,
-                                           keep=key not in resident)
-                if key not in resident:
-                    resident[key] = mid
-            pending.append((rid, conn.shape[0]))
-        lats = []
-        for rid, ne in pending:
-            st, lat, _ = srv.wait(rid)
-            if st == 2:
-                lats.append(lat)
-                ne_done += ne
-            srv.release(rid)
-        wall = time.perf_counter() - t_start
-        srv.close()
-        lats = np.array(lats)
-        return {"requests": len(stream), "completed": int(len(lats)), "wall_s": wall,
-                "elements_per_s": ne_done / wall, "p50_ms": float(np.percentile(lats, 50)),
-                "p99_ms": float(np.percentile(lats, 99)), "mean_ms": float(lats.mean())}
-
-    served = run(feasm.MIXED, workers)
-    log("served: %s" % served)
-    plain = run(feasm.FP64, 1)
-    log("plain: %s" % plain)
-    # the same request stream submitted at once: throughput at saturation
-    served_sat = run(feasm.MIXED, workers, paced=False)
-    log("served, saturated: %s" % served_sat)
-    plain_sat = run(feasm.FP64, 1, paced=False)
-    log("plain, saturated: %s" % plain_sat)
-    return {"workload": "Poisson %.0f req/s for %.1f s, H8/T10 50/50, median 5e4 elements (sigma 1.0), 30%% "
-                        "re-assembly of resident meshes" % (rate, duration),
-            "served_mixed": served, "plain_fp64_sequential": plain,
-            "throughput_ratio": served["elements_per_s"] / plain["elements_per_s"],
-            "saturated": {"served_mixed": served_sat, "plain_fp64_sequential": plain_sat,
-                          "throughput_ratio": served_sat["elements_per_s"] / plain_sat["elements_per_s"]}}
+    meshes, rng = W.served_pool()
+    stream = W.served_stream(rng, len(meshes), rate, duration)
+    log("served: pool of %d meshes generated in %.1f s" % (len(meshes), time.perf_counter() - t0))
+    out = {"workload": "Poisson %.0f req/s for %.1f s, H8/T10 50/50, median 5e4 elements (sigma 1.0), 30%% "
+                       "re-assembly of resident meshes" % (rate, duration)}
+    for key, prec, w, paced in (("served_mixed", feasm.MIXED, workers, True),
+                                ("plain_fp64_sequential", feasm.FP64, 1, True),
+                                ("saturated_served_mixed", feasm.MIXED, workers, False),
+                                ("saturated_plain_fp64_sequential", feasm.FP64, 1, False)):
+        rec, wall = W.run_served(stream, meshes, prec, w, paced=paced)
+        out[key] = W.served_summary(rec, wall)
+        log("%s: %s" % (key, out[key]))
+    out["throughput_ratio"] = out["served_mixed"]["elements_per_s"] / out["plain_fp64_sequential"]["elements_per_s"]
+    out["saturated_throughput_ratio"] = (out["saturated_served_mixed"]["elements_per_s"] /
+                                         out["saturated_plain_fp64_sequential"]["elements_per_s"])
+    return out
 
 
 def main():
@@ -413,7 +369,8 @@ def main():
     ap.add_argument("--cells", type=int, default=100, help="cells per direction of the H8 block")
     ap.add_argument("--cpu-cells", type=int, default=50, help="cells per direction of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extras", default="", help="comma list of extra workloads: mixed,vec,t10,batch,served")
+    ap.add_argument("--extras", default="mixed,vec,t10,batch,served",
+                    help="comma list of extra workloads: mixed,vec,t10,batch,served (or 'none')")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB streaming copy (3 launches) to scale traffic counters")
     a = ap.parse_args()
@@ -520,7 +477,7 @@ def main():
     del M
     mesh.close()
     ctx.close()
-    extras = [x for x in a.extras.split(",") if x]
+    extras = [x for x in a.extras.split(",") if x and x != "none"]
     if "mixed" in extras:
         log("C2 mixed ...")
         out["c2_mixed"] = bench_c2_mixed(feasm, core, D, n)

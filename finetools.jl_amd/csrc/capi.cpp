@@ -504,6 +504,14 @@ extern "C" int feasm_matrix_free_block(feasm_matrix *mat, const uint8_t *is_free
     API_CATCH(mat ? mat->ctx.p : nullptr)
 }
 
+extern "C" int feasm_matrix_ff_block(feasm_matrix *mat, int64_t row_nfreedofs, int64_t col_nfreedofs,
+                                     feasm_matrix **out) {
+    API_TRY
+    FEASM_REQUIRE(mat && out, FEASM_EINVAL, "null argument");
+    *out = leading_block(mat->ctx, mat, row_nfreedofs, col_nfreedofs);
+    API_CATCH(mat ? mat->ctx.p : nullptr)
+}
+
 // ======================================================================= lumped mass
 extern "C" int feasm_bilform_dot_lumped(feasm_mesh *m, int rule, int order, const double *coeff, int ncoeff,
                                         feasm_matrix **out) {
@@ -674,13 +682,18 @@ extern "C" int feasm_matrix_destroy(feasm_matrix *mat) {
     return FEASM_OK;
 }
 
-// ======================================================================= generic assembler
-// SysmatAssemblerSparse (src/AssemblyModule.jl:88-329).  Triples are buffered
-// on the host exactly as the reference buffers them (one assemble! call per
-// element matrix arrives from host code) and moved to the device once, at
-// makematrix!, where the device sparse() builds the CSC.
+// ======================================================================= generic assemblers
+// The assembler protocol of src/AssemblyModule.jl (startassembly!, assemble!,
+// makematrix!) for its four sparse-matrix assemblers: SysmatAssemblerSparse
+// (:88-329), SysmatAssemblerSparseSymm (:342-583), SysmatAssemblerSparseDiag
+// (:599-794) and SysmatAssemblerSparseHRZLumpingSymm (:943-1141).  Triples
+// are buffered on the host exactly as the reference buffers them (one
+// assemble! call per element matrix arrives from host code), with the same
+// growth rule and error texts, and moved to the device once, at makematrix!,
+// where the device sparse() builds the CSC.
 struct feasm_coo {
     CtxRef ctx;
+    int kind = FEASM_ASM_SPARSE;
     int64_t buffer_length = 0, buffer_pointer = 0;  // 1-based pointer as the reference
     int64_t row_nalldofs = 0, col_nalldofs = 0;
     bool nomatrixresult = false;
@@ -688,19 +701,26 @@ struct feasm_coo {
     std::vector<double> V;
 };
 
-extern "C" int feasm_coo_create(feasm_ctx *ctx, int nomatrixresult, feasm_coo **out) {
+extern "C" int feasm_coo_create_kind(feasm_ctx *ctx, int kind, int nomatrixresult, feasm_coo **out) {
     API_TRY
     FEASM_REQUIRE(out, FEASM_EINVAL, "null argument");
+    FEASM_REQUIRE(kind >= FEASM_ASM_SPARSE && kind <= FEASM_ASM_HRZ, FEASM_EINVAL, "unknown assembler kind");
     auto *c = new feasm_coo;
     c->ctx = ctx;
+    c->kind = kind;
     c->nomatrixresult = nomatrixresult != 0;
     *out = c;
     API_CATCH(ctx)
 }
 
-// startassembly! (src/AssemblyModule.jl:209-238): the buffers are sized only
-// while the pointer is below 1 (a fresh assembler); afterwards they and the
-// matrix dimensions stay as they are.
+extern "C" int feasm_coo_create(feasm_ctx *ctx, int nomatrixresult, feasm_coo **out) {
+    return feasm_coo_create_kind(ctx, FEASM_ASM_SPARSE, nomatrixresult, out);
+}
+
+// startassembly! (src/AssemblyModule.jl:209-238, :424-454, :671-699,
+// :1031-1061): the buffers are sized only while the pointer is below 1 (a
+// fresh assembler); afterwards they and the matrix dimensions stay as they
+// are.  expectedntriples (:54-59, :357-362, :617-622, :961-966).
 extern "C" int feasm_coo_start(feasm_coo *c, int64_t elem_nrows, int64_t elem_ncols, int64_t n_elem_mats,
                                int64_t row_nalldofs, int64_t col_nalldofs) {
     API_TRY
@@ -708,36 +728,58 @@ extern "C" int feasm_coo_start(feasm_coo *c, int64_t elem_nrows, int64_t elem_nc
     FEASM_REQUIRE(elem_nrows >= 0 && elem_ncols >= 0 && n_elem_mats >= 0, FEASM_EINVAL, "negative size");
     FEASM_REQUIRE(row_nalldofs < (1ll << 31) && col_nalldofs < (1ll << 31), FEASM_EUNSUPPORTED,
                   "more than 2^31 DOFs");
+    if (c->kind == FEASM_ASM_SYMM)
+        FEASM_REQUIRE(elem_nrows == elem_ncols, FEASM_ESIZE,
+                      "Symmetric sparse matrix is assumed to be assembled from square matrices");
+    if (c->kind == FEASM_ASM_HRZ)
+        FEASM_REQUIRE(elem_nrows == elem_ncols, FEASM_ESIZE,
+                      "Diagonal sparse matrix is assumed to be assembled from square matrices");
+    int64_t expected = elem_nrows * elem_ncols * n_elem_mats;
+    if (c->kind == FEASM_ASM_SYMM) expected = (elem_nrows * elem_ncols + elem_nrows) / 2 * n_elem_mats;
+    if (c->kind == FEASM_ASM_DIAG || c->kind == FEASM_ASM_HRZ)
+        expected = std::max(elem_nrows, elem_ncols) * n_elem_mats;
     if (c->buffer_pointer < 1) {
-        c->buffer_length = elem_nrows * elem_ncols * n_elem_mats;
+        c->buffer_length = expected;
         c->I.resize(c->buffer_length);
         c->J.resize(c->buffer_length);
         c->V.resize(c->buffer_length);
         c->buffer_pointer = 1;
+        if (c->kind != FEASM_ASM_SPARSE)
+            FEASM_REQUIRE(row_nalldofs == col_nalldofs, FEASM_EINVAL, "Row and column info do not agree");
         c->row_nalldofs = row_nalldofs;
         c->col_nalldofs = col_nalldofs;
     }
     API_CATCH(c ? c->ctx : nullptr)
 }
 
-static void coo_append(feasm_coo *c, const double *mat, const int64_t *rows, int64_t nrows, const int64_t *cols,
-                       int64_t ncols) {
+namespace {
+
+// _resize_buffers (src/AssemblyModule.jl:61-76)
+void coo_grow(feasm_coo *c, int64_t chunk) {
+    c->buffer_length += chunk;
+    c->I.resize(c->buffer_length);
+    c->J.resize(c->buffer_length);
+    c->V.resize(c->buffer_length);
+}
+
+void check_col(const feasm_coo *c, int64_t dj) {
+    if (dj < 1) throw Error(FEASM_EDOF, "Column degree of freedom < 1");
+    if (dj > c->col_nalldofs) throw Error(FEASM_EDOF, "Column degree of freedom > size");
+}
+
+void check_row(const feasm_coo *c, int64_t di) {
+    if (di < 1) throw Error(FEASM_EDOF, "Row degree of freedom < 1");
+    if (di > c->row_nalldofs) throw Error(FEASM_EDOF, "Row degree of freedom > size");
+}
+
+// assemble!(::SysmatAssemblerSparse) (:250-282): every entry, column-major
+void append_sparse(feasm_coo *c, const double *mat, const int64_t *rows, int64_t nrows, const int64_t *cols,
+                   int64_t ncols) {
     int64_t p = c->buffer_pointer;
-    if (c->buffer_length < p + ncols * nrows - 1) {  // _resize_buffers, :61-76
-        c->buffer_length += ncols * nrows * 1000;
-        c->I.resize(c->buffer_length);
-        c->J.resize(c->buffer_length);
-        c->V.resize(c->buffer_length);
-    }
+    if (c->buffer_length < p + ncols * nrows - 1) coo_grow(c, ncols * nrows * 1000);
     for (int64_t j = 0; j < ncols; j++) {
-        const int64_t dj = cols[j];
-        if (dj < 1) throw Error(FEASM_EDOF, "Column degree of freedom < 1");
-        if (dj > c->col_nalldofs) throw Error(FEASM_EDOF, "Column degree of freedom > size");
-        for (int64_t i = 0; i < nrows; i++) {
-            const int64_t di = rows[i];
-            if (di < 1) throw Error(FEASM_EDOF, "Row degree of freedom < 1");
-            if (di > c->row_nalldofs) throw Error(FEASM_EDOF, "Row degree of freedom > size");
-        }
+        check_col(c, cols[j]);
+        for (int64_t i = 0; i < nrows; i++) check_row(c, rows[i]);
     }
     for (int64_t j = 0; j < ncols; j++)
         for (int64_t i = 0; i < nrows; i++) {
@@ -749,13 +791,78 @@ static void coo_append(feasm_coo *c, const double *mat, const int64_t *rows, int
     c->buffer_pointer = p;
 }
 
+// assemble!(::SysmatAssemblerSparseSymm) (:500-549): the lower triangle of
+// the element matrix in local numbering, column by column
+void append_symm(feasm_coo *c, const double *mat, const int64_t *rows, int64_t n, const int64_t *cols) {
+    int64_t p = c->buffer_pointer;
+    // the reference's growth test compares with a half-integer bound
+    if ((double)c->buffer_length < (double)p + (double)(n * n + n) / 2.0 - 1.0) coo_grow(c, n * n * 1000);
+    for (int64_t j = 0; j < n; j++) {
+        check_col(c, cols[j]);
+        for (int64_t i = j; i < n; i++) check_row(c, rows[i]);
+    }
+    for (int64_t j = 0; j < n; j++)
+        for (int64_t i = j; i < n; i++) {
+            c->V[p - 1] = mat[i + n * j];
+            c->I[p - 1] = (int32_t)(rows[i] - 1);
+            c->J[p - 1] = (int32_t)(cols[j] - 1);
+            p++;
+        }
+    c->buffer_pointer = p;
+}
+
+// assemble! of SysmatAssemblerSparseDiag (:734-760) and
+// SysmatAssemblerSparseHRZLumpingSymm (:1085-1114): one triple per column,
+// (dj, dj) with the diagonal entry -- for HRZ scaled by the element's total
+// over its diagonal total, em2 / dem2, both summed as the reference writes
+// them (column sums added in column order; diagonal in order)
+void append_diag(feasm_coo *c, const double *mat, const int64_t *cols, int64_t n, bool hrz) {
+    int64_t p = c->buffer_pointer;
+    if (c->buffer_length < p + n * n - 1) coo_grow(c, n * n * 1000);
+    double ffactor = 1.0;
+    if (hrz) {
+        double em2 = 0.0, dem2 = 0.0;
+        for (int64_t j = 0; j < n; j++) {
+            double cs = 0.0;
+            for (int64_t i = 0; i < n; i++) cs += mat[i + n * j];
+            em2 += cs;
+        }
+        for (int64_t i = 0; i < n; i++) dem2 += mat[i + n * i];
+        ffactor = em2 / dem2;
+    }
+    for (int64_t j = 0; j < n; j++) check_col(c, cols[j]);
+    for (int64_t j = 0; j < n; j++) {
+        c->V[p - 1] = hrz ? mat[j + n * j] * ffactor : mat[j + n * j];
+        c->I[p - 1] = (int32_t)(cols[j] - 1);
+        c->J[p - 1] = (int32_t)(cols[j] - 1);
+        p++;
+    }
+    c->buffer_pointer = p;
+}
+
+void coo_append(feasm_coo *c, const double *mat, int64_t mat_nrows, int64_t mat_ncols, const int64_t *rows,
+                int64_t nrows, const int64_t *cols, int64_t ncols) {
+    if (c->kind == FEASM_ASM_SPARSE) {
+        FEASM_REQUIRE(mat_nrows == nrows && mat_ncols == ncols, FEASM_ESIZE, "Wrong size of matrix");
+        append_sparse(c, mat, rows, nrows, cols, ncols);
+        return;
+    }
+    FEASM_REQUIRE(nrows == ncols, FEASM_ESIZE, "Size mismatch");
+    FEASM_REQUIRE(mat_nrows == nrows && mat_ncols == ncols, FEASM_ESIZE, "Size mismatch");
+    if (c->kind == FEASM_ASM_SYMM)
+        append_symm(c, mat, rows, nrows, cols);
+    else
+        append_diag(c, mat, cols, ncols, c->kind == FEASM_ASM_HRZ);
+}
+
+}  // namespace
+
 extern "C" int feasm_coo_assemble(feasm_coo *c, const double *mat, int64_t mat_nrows, int64_t mat_ncols,
                                   const int64_t *rows, int64_t nrows, const int64_t *cols, int64_t ncols) {
     API_TRY
     FEASM_REQUIRE(c, FEASM_EINVAL, "null assembler");
     FEASM_REQUIRE(c->buffer_pointer >= 1, FEASM_ESTATE, "startassembly! has not been called");
-    FEASM_REQUIRE(mat_nrows == nrows && mat_ncols == ncols, FEASM_ESIZE, "Wrong size of matrix");
-    coo_append(c, mat, rows, nrows, cols, ncols);
+    coo_append(c, mat, mat_nrows, mat_ncols, rows, nrows, cols, ncols);
     API_CATCH(c ? c->ctx : nullptr)
 }
 
@@ -765,7 +872,7 @@ extern "C" int feasm_coo_assemble_many(feasm_coo *c, int64_t nmats, const double
     FEASM_REQUIRE(c, FEASM_EINVAL, "null assembler");
     FEASM_REQUIRE(c->buffer_pointer >= 1, FEASM_ESTATE, "startassembly! has not been called");
     for (int64_t e = 0; e < nmats; e++)
-        coo_append(c, mats + e * nrows * ncols, rows + e * nrows, nrows, cols + e * ncols, ncols);
+        coo_append(c, mats + e * nrows * ncols, nrows, ncols, rows + e * nrows, nrows, cols + e * ncols, ncols);
     API_CATCH(c ? c->ctx : nullptr)
 }
 
@@ -777,6 +884,9 @@ extern "C" int feasm_coo_set_nomatrixresult(feasm_coo *c, int flag) {
 
 extern "C" int64_t feasm_coo_ntriples(const feasm_coo *c) { return c ? std::max<int64_t>(c->buffer_pointer - 1, 0) : -1; }
 
+// makematrix! (:304-329, :551-583, :779-794, :1125-1141): sparse() of the
+// buffered triples; for the symmetric assembler then S + transpose(S) with
+// the diagonal halved (symm.hip).
 extern "C" int feasm_coo_makematrix(feasm_coo *c, feasm_matrix **out) {
     API_TRY
     FEASM_REQUIRE(c && out, FEASM_EINVAL, "null argument");
@@ -792,7 +902,10 @@ extern "C" int feasm_coo_makematrix(feasm_coo *c, feasm_matrix **out) {
         FEASM_HIP(hipMemcpy(J->p, c->J.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice));
         FEASM_HIP(hipMemcpy(V->p, c->V.data(), nt * sizeof(double), hipMemcpyHostToDevice));
     }
-    *out = sparse_device(ctx, nt, dp<int32_t>(I), dp<int32_t>(J), dp<double>(V), c->row_nalldofs, c->col_nalldofs);
+    std::unique_ptr<feasm_matrix> S(
+        sparse_device(ctx, nt, dp<int32_t>(I), dp<int32_t>(J), dp<double>(V), c->row_nalldofs, c->col_nalldofs));
+    if (c->kind == FEASM_ASM_SYMM) S.reset(symm_plus_transpose(ctx, S.get()));
+    *out = S.release();
     c->buffer_pointer = 1;
     API_CATCH(c ? c->ctx : nullptr)
 }

@@ -212,6 +212,12 @@ feasm_matrix *drop_zeros(feasm_ctx *ctx, const feasm_matrix *a);
 // column), renumbered consecutively
 feasm_matrix *principal_submatrix(feasm_ctx *ctx, const feasm_matrix *a, const uint8_t *keep_host);
 
+// blocks.hip: A[1:r, 1:c] (matrix_blocked_ff)
+feasm_matrix *leading_block(feasm_ctx *ctx, const feasm_matrix *a, int64_t r, int64_t c);
+
+// symm.hip: S + transpose(S) with the diagonal halved and exact zeros dropped
+feasm_matrix *symm_plus_transpose(feasm_ctx *ctx, const feasm_matrix *S);
+
 // pattern.hip
 void build_node_to_element(feasm_mesh *m);
 std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn);

@@ -5,9 +5,12 @@
 // threads, each with its own HIP stream, take the oldest admissible request
 // as soon as they are free: a request is admitted when its estimated device
 // memory fits the server's HBM budget (resident meshes not in use are
-// evicted, least recently used first, to make room).  Work of different
-// requests therefore overlaps on the device, and a request that arrives while
-// others run starts as soon as a worker and memory free up.
+// evicted, least recently used first, to make room; sched.hpp).  A request
+// that could never fit the budget is refused at submission.  Work of
+// different requests therefore overlaps on the device, and a request that
+// arrives while others run starts as soon as a worker and memory free up.
+// A queued request can be cancelled; a running one finishes its device work
+// and is then reported cancelled, its result dropped.
 //
 // Results are deterministic: every kernel sums in a fixed order, so a request
 // gives bitwise the same matrix alone or in any mix.
@@ -21,8 +24,10 @@
 
 #include "common.hpp"
 #include "fused.hpp"
+#include "sched.hpp"
 
 namespace feasm {
+bool fused_supported(int npe, int nqp);
 FormParams make_form_params(int form, int ndn, const double *coeff, int ncoeff);
 void numeric_into_ctx(feasm_ctx *ctx, feasm_mesh *m, const FormParams &fp, int rule, int order, int precision,
                       double *nzval);
@@ -38,13 +43,6 @@ namespace {
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
-
-struct Resident {
-    feasm_mesh *mesh = nullptr;
-    size_t bytes = 0;
-    int users = 0;
-    double last_use = 0;
-};
 
 struct Request {
     int64_t id = 0;
@@ -62,6 +60,7 @@ struct Request {
     size_t est = 0;
     double t_submit = 0, t_start = 0, t_done = 0;
     int status = 0;  // 0 queued, 1 running, 2 done, < 0 failed (feasm status)
+    bool cancel = false;  // cancelled while running: reported cancelled when its work ends
     std::string error;
     std::unique_ptr<feasm_matrix> result;
 };
@@ -90,110 +89,93 @@ size_t estimate_bytes(int npe, int ndn, int64_t nelem, int64_t nnodes, bool keep
 
 struct feasm_server {
     int device = 0;
-    size_t budget = 0;
-    std::atomic<size_t> in_use{0};
     std::vector<std::thread> workers;
     std::vector<feasm_ctx *> ctxs;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<Request *> queue;
+    Admission<Request> adm;  // queue, residents and the memory budget, under mu
     std::map<int64_t, std::unique_ptr<Request>> requests;
-    std::map<int64_t, Resident> residents;
-    std::map<int64_t, int> pending_keep;  // kept meshes still being assembled: mesh id -> DOFs per node
     int64_t next_id = 1, next_mesh = 1;
     bool stop = false;
-    int running = 0;
     int64_t completed = 0;
     std::string last_error;
 
-    // pick the oldest admissible request (looking a few past the head so a
-    // large request does not block small ones); called with mu held
-    Request *pick() {
-        const size_t window = std::min<size_t>(queue.size(), 8);
-        for (size_t k = 0; k < window; k++) {
-            Request *r = queue[k];
-            if (r->reassembly) {
-                auto it = residents.find(r->mesh_id);
-                if (it == residents.end()) {
-                    if (pending_keep.count(r->mesh_id)) continue;  // its mesh is still being assembled
-                    queue.erase(queue.begin() + k);                 // evicted or unknown: fail it
-                    return r;
-                }
-                if (it->second.users > 0) continue;  // one worker per mesh at a time
-            }
-            size_t need = r->est;
-            if (in_use + need > budget) evict(need);
-            if (in_use + need <= budget || running == 0) {
-                queue.erase(queue.begin() + k);
-                return r;
-            }
-        }
-        return nullptr;
-    }
-
-    void evict(size_t need) {
-        while (in_use + need > budget) {
-            int64_t victim = -1;
-            double oldest = 1e300;
-            for (auto &kv : residents)
-                if (kv.second.users == 0 && kv.second.last_use < oldest) {
-                    oldest = kv.second.last_use;
-                    victim = kv.first;
-                }
-            if (victim < 0) return;
-            Resident &v = residents[victim];
-            in_use -= v.bytes;
-            feasm_mesh *m = v.mesh;
-            residents.erase(victim);
-            (void)hipStreamSynchronize(m->ctx->stream);
-            delete m;
-        }
+    // evicted meshes are released outside the lock: no request uses them
+    // (every request synchronises its stream before it finishes) and the
+    // memory pool defers reuse of their blocks past the streams' pending work
+    static void release_meshes(std::vector<void *> &meshes) {
+        for (void *p : meshes) delete static_cast<feasm_mesh *>(p);
+        meshes.clear();
     }
 
     void work(int w) {
         feasm_ctx *ctx = ctxs[w];
-        FEASM_HIP(hipSetDevice(device));
+        (void)hipSetDevice(device);
         for (;;) {
             Request *r = nullptr;
+            std::vector<void *> evicted;
+            bool quit = false;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_work.wait(lk, [&] { return stop || (r = pick()) != nullptr; });
-                if (!r) return;
-                r->status = 1;
-                r->t_start = now_s();
-                running++;
-                if (!r->reassembly) in_use += r->est;
-                if (r->reassembly && residents.count(r->mesh_id)) residents[r->mesh_id].users++;
+                for (;;) {
+                    if (stop) {
+                        quit = true;
+                        break;
+                    }
+                    r = adm.pick(evicted);
+                    if (r || !evicted.empty()) break;
+                    cv_work.wait(lk);
+                }
+                if (r) {
+                    r->status = 1;
+                    r->t_start = now_s();
+                    adm.start(r);
+                }
             }
+            release_meshes(evicted);
+            if (quit) return;
+            if (!r) continue;
             run_one(ctx, r);  // publishes the outcome and the bookkeeping under one lock
             cv_work.notify_all();
             cv_done.notify_all();
         }
     }
 
-    // bookkeeping of a finished request; called with mu held, in the same
-    // critical section that publishes its status
-    void finish_locked(Request *r) {
-        running--;
-        completed++;
-        if (r->reassembly) {
-            auto it = residents.find(r->mesh_id);
-            if (it != residents.end()) {
-                it->second.users--;
-                it->second.last_use = now_s();
+    // outcome of a request whose work has ended; called with mu held
+    void publish_locked(Request *r, int status, const std::string &err, std::unique_ptr<feasm_matrix> M,
+                        feasm_mesh *keep_mesh) {
+        if (r->cancel) {
+            status = FEASM_ECANCELLED;
+            M.reset();
+            if (keep_mesh) {  // a cancelled request keeps nothing resident
+                delete keep_mesh;
+                keep_mesh = nullptr;
             }
         }
+        r->status = status;
+        r->error = (status == FEASM_ECANCELLED) ? std::string("request cancelled") : err;
+        r->result = std::move(M);
+        r->t_done = now_s();
+        adm.finish(r, r->t_done, keep_mesh);
+        completed++;
+        r->conn0.clear();
+        r->conn0.shrink_to_fit();
+        r->xyz.clear();
+        r->xyz.shrink_to_fit();
     }
 
     void run_one(feasm_ctx *ctx, Request *r) {
+        std::unique_ptr<feasm_mesh> owned;
         try {
             feasm_mesh *m = nullptr;
-            std::unique_ptr<feasm_mesh> owned;
             if (r->reassembly) {
                 std::lock_guard<std::mutex> lk(mu);
-                auto it = residents.find(r->mesh_id);
-                FEASM_REQUIRE(it != residents.end(), FEASM_ESTATE, "resident mesh was evicted or never existed");
-                m = it->second.mesh;
+                auto it = adm.residents.find(r->mesh_id);
+                FEASM_REQUIRE(it != adm.residents.end(), FEASM_ESTATE, "resident mesh was evicted or never existed");
+                m = static_cast<feasm_mesh *>(it->second.mesh);
+                // one request at a time runs on a resident mesh: its pattern,
+                // contribution and tile builders use this worker's context
+                m->ctx = ctx;
             } else {
                 owned.reset(new feasm_mesh);
                 m = owned.get();
@@ -226,9 +208,11 @@ struct feasm_server {
             // The fused kernel needs tiles built on the host, which pays off
             // only over repeated assemblies of one mesh: a mesh's first
             // kTileAfter assemblies take the staged path (all on the device);
-            // later ones build the tiles once and drop the staged buffers.
-            m->numeric_path = (m->assemblies < kTileAfter) ? 1 : 0;
-            if (m->assemblies == kTileAfter) {
+            // later ones build the tiles once and drop the staged buffers --
+            // when the element type and rule have a fused kernel at all.
+            const bool fusable = fused_supported(m->npe, device_qtable(ctx, m->etype, r->rule, r->order).nqp);
+            m->numeric_path = (m->assemblies < kTileAfter || !fusable) ? 1 : 0;
+            if (m->assemblies == kTileAfter && fusable) {
                 m->stage.reset();
                 m->pattern->cptr.reset();
                 m->pattern->contrib.reset();
@@ -237,46 +221,13 @@ struct feasm_server {
             numeric_into_ctx(ctx, m, r->fp, r->rule, r->order, r->precision, dp<double>(M->nzval));
             FEASM_HIP(hipStreamSynchronize(ctx->stream));
             std::lock_guard<std::mutex> lk(mu);
-            r->result = std::move(M);
-            r->t_done = now_s();
-            r->status = 2;
-            if (!r->reassembly) {
-                in_use -= r->est;
-                pending_keep.erase(r->mesh_id);
-                if (r->keep) {
-                    Resident res;
-                    res.mesh = owned.release();
-                    res.bytes = r->est;
-                    res.last_use = now_s();
-                    in_use += res.bytes;
-                    residents[r->mesh_id] = res;
-                }
-            }
-            r->conn0.clear();
-            r->conn0.shrink_to_fit();
-            r->xyz.clear();
-            r->xyz.shrink_to_fit();
-            finish_locked(r);
+            publish_locked(r, 2, std::string(), std::move(M), r->keep ? owned.release() : nullptr);
         } catch (const feasm::Error &e) {
             std::lock_guard<std::mutex> lk(mu);
-            if (!r->reassembly) {
-                in_use -= r->est;
-                pending_keep.erase(r->mesh_id);
-            }
-            r->status = e.code;
-            r->error = e.what();
-            r->t_done = now_s();
-            finish_locked(r);
+            publish_locked(r, e.code, e.what(), nullptr, nullptr);
         } catch (const std::exception &e) {
             std::lock_guard<std::mutex> lk(mu);
-            if (!r->reassembly) {
-                in_use -= r->est;
-                pending_keep.erase(r->mesh_id);
-            }
-            r->status = FEASM_EINVAL;
-            r->error = e.what();
-            r->t_done = now_s();
-            finish_locked(r);
+            publish_locked(r, FEASM_EINVAL, e.what(), nullptr, nullptr);
         }
     }
 
@@ -286,7 +237,7 @@ struct feasm_server {
         r->t_submit = now_s();
         Request *p = r.get();
         requests[p->id] = std::move(r);
-        queue.push_back(p);
+        adm.queue.push_back(p);
         cv_work.notify_one();
         return p->id;
     }
@@ -323,7 +274,7 @@ extern "C" int feasm_server_create(int device, int workers, double hbm_fraction,
     FEASM_HIP(hipSetDevice(device));
     size_t fr = 0, tot = 0;
     FEASM_HIP(hipMemGetInfo(&fr, &tot));
-    s->budget = (size_t)(hbm_fraction * (double)fr);
+    s->adm.budget = (size_t)(hbm_fraction * (double)fr);
     for (int w = 0; w < workers; w++) {
         feasm_ctx *c = nullptr;
         const int rc = feasm_ctx_create(device, &c);
@@ -367,8 +318,13 @@ extern "C" int feasm_server_submit_mesh(feasm_server *s, int elem_type, int64_t 
     r->est = estimate_bytes(npe, ndn, nelem, nnodes, keep_resident != 0, precision);
     {
         std::lock_guard<std::mutex> lk(s->mu);
+        // a request larger than the whole budget would wait for ever
+        if (!s->adm.fits_budget(r->est))
+            throw Error(FEASM_ENOMEM, "admission refused: the request needs about " +
+                                          std::to_string(r->est >> 20) + " MiB of device memory, the server's budget is " +
+                                          std::to_string(s->adm.budget >> 20) + " MiB");
         r->mesh_id = s->next_mesh++;
-        if (r->keep) s->pending_keep[r->mesh_id] = ndn;
+        if (r->keep) s->adm.pending_keep[r->mesh_id] = ndn;
     }
     if (mesh_id) *mesh_id = r->mesh_id;
     *request_id = s->enqueue(std::move(r));
@@ -383,12 +339,12 @@ extern "C" int feasm_server_submit_reassembly(feasm_server *s, int64_t mesh_id, 
     int ndn = 0;
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        auto it = s->residents.find(mesh_id);
-        if (it != s->residents.end()) {
-            ndn = it->second.mesh->ndn;
+        auto it = s->adm.residents.find(mesh_id);
+        if (it != s->adm.residents.end()) {
+            ndn = static_cast<feasm_mesh *>(it->second.mesh)->ndn;
         } else {
-            auto pk = s->pending_keep.find(mesh_id);
-            FEASM_REQUIRE(pk != s->pending_keep.end(), FEASM_ESTATE, "no such resident mesh (not kept or evicted)");
+            auto pk = s->adm.pending_keep.find(mesh_id);
+            FEASM_REQUIRE(pk != s->adm.pending_keep.end(), FEASM_ESTATE, "no such resident mesh (not kept or evicted)");
             ndn = pk->second;
         }
     }
@@ -452,17 +408,45 @@ extern "C" int feasm_server_release(feasm_server *s, int64_t request_id) {
     S_CATCH(s)
 }
 
+// Cancel a request: a queued one is withdrawn at once, a running one
+// finishes its device work and is then reported cancelled (its result
+// dropped, its mesh not kept); either way feasm_server_wait gives status
+// FEASM_ECANCELLED.  A request that already ended is FEASM_ESTATE.
+extern "C" int feasm_server_cancel(feasm_server *s, int64_t request_id) {
+    S_TRY
+    FEASM_REQUIRE(s, FEASM_EINVAL, "null server");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->requests.find(request_id);
+        FEASM_REQUIRE(it != s->requests.end(), FEASM_EINVAL, "unknown request");
+        Request *r = it->second.get();
+        FEASM_REQUIRE(r->status == 0 || r->status == 1, FEASM_ESTATE, "request already finished");
+        if (r->status == 0) {
+            FEASM_REQUIRE(s->adm.withdraw(r), FEASM_ESTATE, "request not in the queue");
+            r->status = FEASM_ECANCELLED;
+            r->error = "request cancelled";
+            r->t_done = now_s();
+            s->completed++;
+        } else {
+            r->cancel = true;
+        }
+    }
+    s->cv_done.notify_all();
+    s->cv_work.notify_all();
+    S_CATCH(s)
+}
+
 extern "C" int feasm_server_stats(feasm_server *s, int64_t *queued, int64_t *running, int64_t *completed,
                                   int64_t *residents, int64_t *bytes_in_use, int64_t *budget) {
     S_TRY
     FEASM_REQUIRE(s, FEASM_EINVAL, "null server");
     std::lock_guard<std::mutex> lk(s->mu);
-    if (queued) *queued = (int64_t)s->queue.size();
-    if (running) *running = s->running;
+    if (queued) *queued = (int64_t)s->adm.queue.size();
+    if (running) *running = s->adm.running;
     if (completed) *completed = s->completed;
-    if (residents) *residents = (int64_t)s->residents.size();
-    if (bytes_in_use) *bytes_in_use = (int64_t)s->in_use.load();
-    if (budget) *budget = (int64_t)s->budget;
+    if (residents) *residents = (int64_t)s->adm.residents.size();
+    if (bytes_in_use) *bytes_in_use = (int64_t)s->adm.in_use;
+    if (budget) *budget = (int64_t)s->adm.budget;
     S_CATCH(s)
 }
 
@@ -479,8 +463,8 @@ extern "C" int feasm_server_destroy(feasm_server *s) {
     s->cv_work.notify_all();
     for (auto &t : s->workers) t.join();
     s->requests.clear();
-    for (auto &kv : s->residents) delete kv.second.mesh;
-    s->residents.clear();
+    for (auto &kv : s->adm.residents) delete static_cast<feasm_mesh *>(kv.second.mesh);
+    s->adm.residents.clear();
     for (auto *c : s->ctxs) feasm_ctx_destroy(c);
     delete s;
     return FEASM_OK;

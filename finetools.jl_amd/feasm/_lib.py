@@ -14,6 +14,7 @@ H8, T10 = 0, 1
 DIFFUSION_ISO, DIFFUSION_MAT, LIN_ELASTIC, DOT = 0, 1, 2, 3
 RULE_GAUSS, RULE_TET = 0, 1
 FP64, MIXED = 0, 1
+ASM_SPARSE, ASM_SYMM, ASM_DIAG, ASM_HRZ = 0, 1, 2, 3
 
 P = ctypes.c_void_p
 I32, I64, U64, DBL = ctypes.c_int, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double
@@ -55,7 +56,10 @@ SIGNATURES = {
     "feasm_matrix_copy_csc": (I32, [P, I32, P, P, P]),
     "feasm_matrix_device_csc": (I32, [P, PP, PP, PP]),
     "feasm_matrix_destroy": (I32, [P]),
+    "feasm_matrix_checksum": (I32, [P, ctypes.POINTER(ctypes.c_uint64)]),
     "feasm_coo_create": (I32, [P, I32, PP]),
+    "feasm_coo_create_kind": (I32, [P, I32, I32, PP]),
+    "feasm_matrix_ff_block": (I32, [P, I64, I64, PP]),
     "feasm_coo_start": (I32, [P, I64, I64, I64, I64, I64]),
     "feasm_coo_assemble": (I32, [P, P, I64, I64, P, I64, P, I64]),
     "feasm_coo_assemble_many": (I32, [P, I64, P, P, I64, P, I64]),
@@ -82,6 +86,7 @@ SIGNATURES = {
     "feasm_server_submit_reassembly": (I32, [P, I64, I32, I32, I32, P, I32, I32, PI64]),
     "feasm_server_wait": (I32, [P, I64, DBL, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_double)]),
+    "feasm_server_cancel": (I32, [P, I64]),
     "feasm_server_result": (I32, [P, I64, PP]),
     "feasm_server_release": (I32, [P, I64]),
     "feasm_server_stats": (I32, [P, PI64, PI64, PI64, PI64, PI64, PI64]),

@@ -122,6 +122,12 @@ class DeviceMatrix:
         colptr, rowval, nzval = self.csc_arrays(0)
         return sp.csc_matrix((nzval, rowval, colptr), shape=(m, n))
 
+    def checksum(self):
+        """64-bit digest of colptr, rowval and the value bits (computed on the device)."""
+        d = ctypes.c_uint64()
+        check(lib().feasm_matrix_checksum(self.h, ctypes.byref(d)), self.ctx.h if self.ctx is not None else None)
+        return d.value
+
     def device_ptrs(self):
         c, r, v = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
         check(lib().feasm_matrix_device_csc(self.h, ctypes.byref(c), ctypes.byref(r), ctypes.byref(v)))
@@ -313,6 +319,10 @@ class Server:
         self._check(lib().feasm_server_wait(self.h, rid, timeout, ctypes.byref(st), ctypes.byref(lat),
                                             ctypes.byref(qm)))
         return st.value, lat.value, qm.value
+
+    def cancel(self, rid):
+        """Withdraw a queued request, or have a running one reported cancelled."""
+        self._check(lib().feasm_server_cancel(self.h, rid))
 
     def result(self, rid):
         h = ctypes.c_void_p()

@@ -409,24 +409,22 @@ class FEMMBase:
         return self._mesh
 
 
-# ------------------------------------------------------------------ assembler
-class SysmatAssemblerSparse:
-    """SysmatAssemblerSparse(z=0.0, nomatrixresult=False) on the device.
+# ------------------------------------------------------------------ assemblers
+class _DeviceCOO:
+    """The buffered-triples protocol shared by the sparse-matrix assemblers
+    (feasm_coo of the given kind): startassembly / assemble / makematrix
+    follow the reference methods of that assembler, and makematrix builds the
+    CSC on the device."""
 
-    The generic protocol (startassembly / assemble / makematrix) buffers the
-    triples and builds the CSC with the device sparse().  Passed to a bilform_*
-    function it instead runs the GPU element loop (then ``precision`` selects
-    FP64 or the mixed FP32/FP64 arithmetic)."""
+    KIND = L.ASM_SPARSE
 
-    def __init__(self, z=0.0, nomatrixresult=False, ctx=None, precision=L.FP64):
-        if not isinstance(z, float):
-            raise TypeError("only Float64 assemblers are supported")
+    def _open(self, ctx, nomatrixresult):
         self.ctx = ctx or default_context()
         h = ctypes.c_void_p()
-        check(lib().feasm_coo_create(self.ctx.h, 1 if nomatrixresult else 0, ctypes.byref(h)), self.ctx.h)
+        check(lib().feasm_coo_create_kind(self.ctx.h, self.KIND, 1 if nomatrixresult else 0, ctypes.byref(h)),
+              self.ctx.h)
         self.h = h
         self._nomatrixresult = bool(nomatrixresult)
-        self.precision = precision
         self._dims = None
 
     def close(self):
@@ -440,52 +438,82 @@ class SysmatAssemblerSparse:
         return np.float64
 
 
-class SysmatAssemblerFFBlock:
-    """SysmatAssemblerFFBlock(row_nfreedofs) (src/AssemblyModule.jl:1149-1231):
-    bilform_* then returns the free-free block matrix_blocked_ff of the
-    wrapped SysmatAssemblerSparse's matrix, extracted on the device."""
+class SysmatAssemblerSparse(_DeviceCOO):
+    """SysmatAssemblerSparse(z=0.0, nomatrixresult=False) on the device.
 
-    def __init__(self, row_nfreedofs, col_nfreedofs=None, ctx=None, precision=L.FP64):
-        if col_nfreedofs is not None and col_nfreedofs != row_nfreedofs:
-            raise NotImplementedError("square free-free blocks only")
-        self.nfree = int(row_nfreedofs)
-        self.ctx = ctx or default_context()
-        self.precision = precision
+    The generic protocol (startassembly / assemble / makematrix) buffers the
+    triples and builds the CSC with the device sparse().  Passed to a bilform_*
+    function it instead runs the GPU element loop (then ``precision`` selects
+    FP64 or the mixed FP32/FP64 arithmetic)."""
 
-    def eltype(self):
-        return np.float64
-
-
-class SysmatAssemblerSparseSymm:
-    """SysmatAssemblerSparseSymm(z=0.0) (src/AssemblyModule.jl:342-583), the
-    default assembler of the convenience bilform_* methods.  Passed to a
-    bilform_* function it runs the GPU element loop and returns the full
-    symmetric matrix without the entries that sum to exactly zero, the
-    pattern the reference's S + transpose(S) keeps."""
+    KIND = L.ASM_SPARSE
 
     def __init__(self, z=0.0, nomatrixresult=False, ctx=None, precision=L.FP64):
         if not isinstance(z, float):
             raise TypeError("only Float64 assemblers are supported")
-        if nomatrixresult:
-            raise NotImplementedError("nomatrixresult is not on the symmetric path")
-        self.ctx = ctx or default_context()
+        self._open(ctx, nomatrixresult)
         self.precision = precision
 
-    def eltype(self):
-        return np.float64
+
+class SysmatAssemblerSparseSymm(_DeviceCOO):
+    """SysmatAssemblerSparseSymm(z=0.0) (src/AssemblyModule.jl:342-583), the
+    default assembler of the convenience bilform_* methods.  The generic
+    protocol keeps each element matrix's lower triangle (local numbering) and
+    makematrix forms S + transpose(S) with the diagonal halved on the device.
+    Passed to a bilform_* function it runs the GPU element loop and returns
+    the full symmetric matrix without the entries that sum to exactly zero,
+    the pattern the reference's S + transpose(S) keeps."""
+
+    KIND = L.ASM_SYMM
+
+    def __init__(self, z=0.0, nomatrixresult=False, ctx=None, precision=L.FP64):
+        if not isinstance(z, float):
+            raise TypeError("only Float64 assemblers are supported")
+        self._open(ctx, nomatrixresult)
+        self.precision = precision
 
 
-class SysmatAssemblerSparseHRZLumpingSymm:
-    """SysmatAssemblerSparseHRZLumpingSymm(z=0.0) (src/AssemblyModule.jl:943-1141).
-    Passed to bilform_dot it runs the GPU element loop and returns the
-    HRZ-lumped diagonal matrix (makematrix!)."""
+class SysmatAssemblerSparseDiag(_DeviceCOO):
+    """SysmatAssemblerSparseDiag(z=0.0) (src/AssemblyModule.jl:599-794): the
+    diagonal entries of square element matrices (off-diagonal ones ignored)."""
+
+    KIND = L.ASM_DIAG
 
     def __init__(self, z=0.0, nomatrixresult=False, ctx=None):
         if not isinstance(z, float):
             raise TypeError("only Float64 assemblers are supported")
-        if nomatrixresult:
-            raise NotImplementedError("nomatrixresult is not on the lumped path")
+        self._open(ctx, nomatrixresult)
+
+
+class SysmatAssemblerSparseHRZLumpingSymm(_DeviceCOO):
+    """SysmatAssemblerSparseHRZLumpingSymm(z=0.0) (src/AssemblyModule.jl:943-1141).
+    The generic protocol keeps each element matrix's diagonal scaled by its
+    total over its diagonal total; passed to bilform_dot it runs the GPU
+    element loop and returns the HRZ-lumped diagonal matrix (makematrix!)."""
+
+    KIND = L.ASM_HRZ
+
+    def __init__(self, z=0.0, nomatrixresult=False, ctx=None):
+        if not isinstance(z, float):
+            raise TypeError("only Float64 assemblers are supported")
+        self._open(ctx, nomatrixresult)
+
+
+class SysmatAssemblerFFBlock:
+    """SysmatAssemblerFFBlock(row_nfreedofs, col_nfreedofs=row_nfreedofs)
+    (src/AssemblyModule.jl:1149-1231): delegates startassembly / assemble to
+    a wrapped SysmatAssemblerSparse and makematrix returns the leading
+    free-free block matrix_blocked_ff of its matrix, extracted on the device.
+    Passed to a bilform_* function it runs the GPU element loop and extracts
+    the block."""
+
+    def __init__(self, row_nfreedofs, col_nfreedofs=None, ctx=None, precision=L.FP64):
+        self.row_nfree = int(row_nfreedofs)
+        self.col_nfree = int(row_nfreedofs if col_nfreedofs is None else col_nfreedofs)
+        self.nfree = self.row_nfree
         self.ctx = ctx or default_context()
+        self.precision = precision
+        self._a = SysmatAssemblerSparse(0.0, ctx=self.ctx, precision=precision)
 
     def eltype(self):
         return np.float64
@@ -511,12 +539,19 @@ class SysvecAssembler:
 
 
 def setnomatrixresult(a, flag):
+    if isinstance(a, SysmatAssemblerFFBlock):
+        return setnomatrixresult(a._a, flag)
     a._nomatrixresult = bool(flag)
     check(lib().feasm_coo_set_nomatrixresult(a.h, 1 if flag else 0), a.ctx.h)
     return a
 
 
 def expectedntriples(a, elem_mat_nrows, elem_mat_ncols, n_elem_mats):
+    """expectedntriples (src/AssemblyModule.jl:54-59, :357-362, :617-622, :961-966)."""
+    if isinstance(a, SysmatAssemblerSparseSymm):
+        return (elem_mat_nrows * elem_mat_ncols + elem_mat_nrows) // 2 * n_elem_mats
+    if isinstance(a, (SysmatAssemblerSparseDiag, SysmatAssemblerSparseHRZLumpingSymm)):
+        return max(elem_mat_nrows, elem_mat_ncols) * n_elem_mats
     return elem_mat_nrows * elem_mat_ncols * n_elem_mats
 
 
@@ -525,6 +560,9 @@ def startassembly(a, elem_mat_nrows, elem_mat_ncols=None, n_elem_mats=None, row_
     if isinstance(a, SysvecAssembler):  # startassembly!(self::SysvecAssembler, row_nalldofs)
         a._row_nalldofs = int(elem_mat_nrows)
         a._buffer = np.zeros(a._row_nalldofs)
+        return a
+    if isinstance(a, SysmatAssemblerFFBlock):  # delegated (:1183-1205)
+        startassembly(a._a, elem_mat_nrows, elem_mat_ncols, n_elem_mats, row_nalldofs, col_nalldofs, force_init)
         return a
     check(lib().feasm_coo_start(a.h, elem_mat_nrows, elem_mat_ncols, n_elem_mats, row_nalldofs, col_nalldofs),
           a.ctx.h)
@@ -544,11 +582,14 @@ def assemble(a, mat, dofnums_row, dofnums_col=None):
                 raise ValueError("Row degree of freedom > size")
             a._buffer[gi - 1] += vec[i]
         return a
+    if isinstance(a, SysmatAssemblerFFBlock):
+        assemble(a._a, mat, dofnums_row, dofnums_col)
+        return a
     m = np.asfortranarray(np.asarray(mat, dtype=np.float64))
     if m.ndim == 1:
         m = m.reshape(-1, 1)
     r = np.ascontiguousarray(np.ravel(dofnums_row), dtype=np.int64)
-    c = np.ascontiguousarray(np.ravel(dofnums_col), dtype=np.int64)
+    c = np.ascontiguousarray(np.ravel(dofnums_col if dofnums_col is not None else dofnums_row), dtype=np.int64)
     check(lib().feasm_coo_assemble(a.h, m.ctypes.data_as(ctypes.c_void_p), m.shape[0], m.shape[1], ptr(r), len(r),
                                    ptr(c), len(c)), a.ctx.h)
     return a
@@ -559,13 +600,31 @@ def makevector(a):
     return a._buffer.copy()
 
 
-def makematrix(a):
+def makematrix(a, return_device=False):
+    """makematrix! of the assembler: a scipy CSC matrix (or the DeviceMatrix)."""
+    if isinstance(a, SysmatAssemblerFFBlock):
+        h = ctypes.c_void_p()
+        check(lib().feasm_coo_makematrix(a._a.h, ctypes.byref(h)), a.ctx.h)
+        if not h.value:  # nomatrixresult: the reference's spzeros, then its block
+            m, n = a._a._dims if a._a._dims else (0, 0)
+            if a.row_nfree > m:
+                raise ValueError("The ff block has too many rows")
+            if a.col_nfree > n:
+                raise ValueError("The ff block has too many columns")
+            return sp.csc_matrix((a.row_nfree, a.col_nfree))
+        full = DeviceMatrix(a.ctx, h)
+        h2 = ctypes.c_void_p()
+        check(lib().feasm_matrix_ff_block(full.h, a.row_nfree, a.col_nfree, ctypes.byref(h2)), a.ctx.h)
+        full.close()
+        blk = DeviceMatrix(a.ctx, h2)
+        return blk if return_device else blk.to_scipy()
     h = ctypes.c_void_p()
     check(lib().feasm_coo_makematrix(a.h, ctypes.byref(h)), a.ctx.h)
     if not h.value:
         m, n = a._dims if a._dims else (0, 0)
         return sp.csc_matrix((m, n))
-    return DeviceMatrix(a.ctx, h).to_scipy()
+    M = DeviceMatrix(a.ctx, h)
+    return M if return_device else M.to_scipy()
 
 
 # ------------------------------------------------------------------ bilinear forms
@@ -584,7 +643,11 @@ def _run(femm, assembler, geom, u, form, coeff, return_device=False):
         raise TypeError("the GPU bilinear forms take a SysmatAssemblerSparse, SysmatAssemblerSparseSymm or "
                         "SysmatAssemblerFFBlock")
     kind, order = _rule(femm)
+    if isinstance(assembler, SysmatAssemblerSparseSymm) and assembler._nomatrixresult:
+        raise NotImplementedError("nomatrixresult is not on the symmetric GPU element loop")
     if isinstance(assembler, SysmatAssemblerFFBlock):
+        if assembler.col_nfree != assembler.row_nfree:
+            raise NotImplementedError("the GPU element loop extracts square free-free blocks")
         if assembler.nfree > nalldofs(u):
             raise ValueError("The ff block has too many rows")
         if u._entperm_default and assembler.nfree == nfreedofs(u):

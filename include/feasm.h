@@ -196,6 +196,11 @@ int feasm_matrix_copy_csc(feasm_matrix *mat, int base, int64_t *colptr, int64_t 
 int feasm_matrix_device_csc(feasm_matrix *mat, const int64_t **colptr, const int32_t **rowval,
                             double **nzval);
 int feasm_matrix_destroy(feasm_matrix *mat);
+/* No reference counterpart (verification helper): a 64-bit digest of the
+ * matrix -- column pointers, row indices and the bits of every value, each
+ * hashed with its position and summed modulo 2^64 on the device -- so that
+ * two large results can be compared bitwise without copying them out. */
+int feasm_matrix_checksum(feasm_matrix *mat, uint64_t *digest);
 /* A new matrix holding mat's entries except the stored exact zeros
  * (SparseArrays.dropzeros).  bilform_* into a SysmatAssemblerSparseSymm
  * (src/AssemblyModule.jl:342-583) gives the full symmetric matrix with this
@@ -210,15 +215,39 @@ int feasm_matrix_dropzeros(feasm_matrix *mat, feasm_matrix **out);
  * principal submatrix of mat on the DOFs with is_free[d] != 0 (one flag per
  * column of mat, 0-based), renumbered consecutively.  A new matrix. */
 int feasm_matrix_free_block(feasm_matrix *mat, const uint8_t *is_free, feasm_matrix **out);
+/* matrix_blocked_ff(A, row_nfreedofs, col_nfreedofs)
+ * (src/MatrixUtilityModule.jl:675-690), as SysmatAssemblerFFBlock's
+ * makematrix! applies it to its wrapped assembler's matrix (:1227-1229):
+ * A[1:row_nfreedofs, 1:col_nfreedofs], stored entries kept; spzeros when
+ * either size is 0; "The ff block has too many rows/columns" beyond A.
+ * A new matrix. */
+int feasm_matrix_ff_block(feasm_matrix *mat, int64_t row_nfreedofs, int64_t col_nfreedofs, feasm_matrix **out);
 
-/* ---- generic assembler protocol (SysmatAssemblerSparse) ----------------- */
+/* ---- generic assembler protocol ---------------------------------------- */
+/* assembler kinds of feasm_coo_create_kind (src/AssemblyModule.jl) */
+#define FEASM_ASM_SPARSE 0 /* SysmatAssemblerSparse               :88-329   */
+#define FEASM_ASM_SYMM 1   /* SysmatAssemblerSparseSymm           :342-583  */
+#define FEASM_ASM_DIAG 2   /* SysmatAssemblerSparseDiag           :599-794  */
+#define FEASM_ASM_HRZ 3    /* SysmatAssemblerSparseHRZLumpingSymm :943-1141 */
 /* SysmatAssemblerSparse(0.0, nomatrixresult) (src/AssemblyModule.jl:159-165).
  * ctx may be NULL for host-side buffering only (makematrix! then fails). */
 int feasm_coo_create(feasm_ctx *ctx, int nomatrixresult, feasm_coo **out);
-/* startassembly! (src/AssemblyModule.jl:209-238): on a fresh assembler the
- * buffers are sized for elem_nrows*elem_ncols*n_elem_mats triples and the
- * matrix dimensions recorded; on a used one both are left as they are and
- * only the buffer pointer is reset.  assemble! grows the buffers by
+/* The constructor of the assembler of the given kind (z = 0.0):
+ * SysmatAssemblerSparseSymm(0.0, nomatrixresult) (:393-405), ...Diag (:631-643),
+ * ...HRZLumpingSymm (:975-987).  The protocol calls below follow each kind's
+ * own methods: Symm keeps the lower triangle of each square element matrix
+ * in local numbering (assemble!, :500-549) and its makematrix! forms
+ * S + transpose(S) with the diagonal halved (:551-583; exact zero sums are
+ * not stored, as SparseArrays' `+` stores none); Diag keeps the diagonal
+ * entries (:734-760); HRZLumpingSymm keeps the diagonal scaled by the
+ * element's total over its diagonal total (:1085-1114). */
+int feasm_coo_create_kind(feasm_ctx *ctx, int kind, int nomatrixresult, feasm_coo **out);
+/* startassembly! (src/AssemblyModule.jl:209-238, :424-454, :671-699,
+ * :1031-1061): on a fresh assembler the buffers are sized for
+ * expectedntriples (:54-59, :357-362, :617-622, :961-966) and the matrix
+ * dimensions recorded (the Symm, Diag and HRZ kinds require them equal:
+ * "Row and column info do not agree"); on a used one both are left as they
+ * are and only the buffer pointer is reset.  assemble! grows the buffers by
  * elem_nrows*elem_ncols*1000 triples when full (_resize_buffers, :61-76). */
 int feasm_coo_start(feasm_coo *coo, int64_t elem_nrows, int64_t elem_ncols, int64_t n_elem_mats,
                     int64_t row_nalldofs, int64_t col_nalldofs);
@@ -324,7 +353,8 @@ int feasm_server_create(int device, int workers, double hbm_fraction, feasm_serv
 /* Queue the assembly of a new mesh (arrays are copied; conn 1-based row-major
  * nelem x npe, xyz row-major nnodes x 3, node-major DOFs with ndn per node).
  * keep_resident != 0 keeps mesh, pattern and tiles on the device for later
- * re-assembly under *mesh_id. */
+ * re-assembly under *mesh_id.  A request whose device-memory estimate exceeds
+ * the server's whole budget is refused here with FEASM_ENOMEM. */
 int feasm_server_submit_mesh(feasm_server *s, int elem_type, int64_t nelem, int npe, const int64_t *conn,
                              int64_t nnodes, const double *xyz_rowmajor, int ndn, int form, int rule, int order,
                              const double *coeff, int ncoeff, int precision, int keep_resident, int64_t *request_id,
@@ -337,6 +367,11 @@ int feasm_server_submit_reassembly(feasm_server *s, int64_t mesh_id, int form, i
  * latency and queueing time in ms from submission. */
 int feasm_server_wait(feasm_server *s, int64_t request_id, double timeout_s, int *status, double *latency_ms,
                       double *queue_ms);
+/* Cancel a request: a queued one is withdrawn at once; a running one finishes
+ * its device work and is then reported cancelled (result dropped, mesh not
+ * kept).  feasm_server_wait then gives status FEASM_ECANCELLED.  A request
+ * that already ended is FEASM_ESTATE. */
+int feasm_server_cancel(feasm_server *s, int64_t request_id);
 /* Result of a finished request; owned by the server until released. */
 int feasm_server_result(feasm_server *s, int64_t request_id, feasm_matrix **out);
 int feasm_server_release(feasm_server *s, int64_t request_id);

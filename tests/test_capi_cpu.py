@@ -130,3 +130,65 @@ def test_numberdofs_node_major():
     assert nalldofs(u) == 12
     numberdofs(u, [4, 3, 2, 1])
     assert u.dofnums[3].tolist() == [1, 2, 3]
+
+
+def _host_assembler_kind(kind):
+    h = ctypes.c_void_p()
+    assert L.lib().feasm_coo_create_kind(None, kind, 0, ctypes.byref(h)) == 0
+    return h
+
+
+def test_symm_diag_hrz_protocols_buffer_and_fail_like_reference():
+    """Host side of SysmatAssemblerSparseSymm / Diag / HRZLumpingSymm
+    (src/AssemblyModule.jl:424-549, :671-760, :1031-1114): buffer sizes, the
+    number of triples each assemble! adds, growth, and the error texts."""
+    lib = L.lib()
+    m = np.asfortranarray(np.arange(16, dtype=np.float64).reshape(4, 4))
+    d = np.array([5, 2, 1, 4], np.int64)
+    for kind, per in ((L.ASM_SYMM, 10), (L.ASM_DIAG, 4), (L.ASM_HRZ, 4)):
+        a = _host_assembler_kind(kind)
+        try:
+            if kind != L.ASM_DIAG:  # square element matrices only
+                assert lib.feasm_coo_start(a, 4, 3, 2, 7, 7) == L.ESIZE
+                want = (b"Symmetric sparse matrix is assumed to be assembled from square matrices"
+                        if kind == L.ASM_SYMM else
+                        b"Diagonal sparse matrix is assumed to be assembled from square matrices")
+                assert lib.feasm_last_error(None) == want
+            assert lib.feasm_coo_start(a, 4, 4, 2, 7, 6) == L.EINVAL
+            assert lib.feasm_last_error(None) == b"Row and column info do not agree"
+            b = _host_assembler_kind(kind)
+            assert lib.feasm_coo_start(b, 4, 4, 2, 7, 7) == 0
+            assert lib.feasm_coo_assemble(b, L.ptr(m), 4, 4, L.ptr(d), 4, L.ptr(d), 4) == 0
+            assert lib.feasm_coo_ntriples(b) == per
+            # rows and columns of different lengths
+            assert lib.feasm_coo_assemble(b, L.ptr(m), 4, 4, L.ptr(d), 3, L.ptr(d), 4) == L.ESIZE
+            assert lib.feasm_last_error(None) == b"Size mismatch"
+            bad = np.array([5, 2, 9, 4], np.int64)
+            assert lib.feasm_coo_assemble(b, L.ptr(m), 4, 4, L.ptr(bad), 4, L.ptr(bad), 4) == L.EDOF
+            # Symm meets the rows of the first column before the third column
+            assert lib.feasm_last_error(None) == (b"Row degree of freedom > size" if kind == L.ASM_SYMM
+                                                  else b"Column degree of freedom > size")
+            bad = np.array([0, 2, 1, 4], np.int64)
+            assert lib.feasm_coo_assemble(b, L.ptr(m), 4, 4, L.ptr(bad), 4, L.ptr(bad), 4) == L.EDOF
+            assert lib.feasm_last_error(None) == b"Column degree of freedom < 1"
+            for _ in range(5):  # grows past the 2-element buffer
+                assert lib.feasm_coo_assemble(b, L.ptr(m), 4, 4, L.ptr(d), 4, L.ptr(d), 4) == 0
+            assert lib.feasm_coo_ntriples(b) == 6 * per
+            lib.feasm_coo_destroy(b)
+        finally:
+            lib.feasm_coo_destroy(a)
+    # the symmetric assembler checks every row of the lower triangle, the
+    # diagonal ones only their columns
+    a = _host_assembler_kind(L.ASM_SYMM)
+    b = _host_assembler_kind(L.ASM_DIAG)
+    try:
+        for h in (a, b):
+            assert lib.feasm_coo_start(h, 4, 4, 1, 7, 7) == 0
+        rows = np.array([5, 2, 1, 8], np.int64)
+        cols = np.array([5, 2, 1, 4], np.int64)
+        assert lib.feasm_coo_assemble(a, L.ptr(m), 4, 4, L.ptr(rows), 4, L.ptr(cols), 4) == L.EDOF
+        assert lib.feasm_last_error(None) == b"Row degree of freedom > size"
+        assert lib.feasm_coo_assemble(b, L.ptr(m), 4, 4, L.ptr(rows), 4, L.ptr(cols), 4) == 0
+    finally:
+        lib.feasm_coo_destroy(a)
+        lib.feasm_coo_destroy(b)

@@ -1,0 +1,245 @@
+"""The batched and served workloads on the GPU, at their real sizes.
+
+Batched: 256 seeded H8 blocks (nx, ny, nz ~ U{4..64}, about 9.8 M elements),
+half diffusion and half elasticity, mixed precision, tile shape autotuned,
+one persistent launch per form captured in a hipGraph and replayed.  Every
+replay is bitwise the same (device digests); per mesh the pattern size is the
+closed form ndn^2 (3nx+1)(3ny+1)(3nz+1), constants (diffusion) and
+translations (elasticity) are in the null space, and the values agree with
+the plain FP64 per-mesh assembly within 1e-6; on 8 meshes sampled columns
+match the oracle, and rotations are in the null space.
+
+Served: a Poisson stream (200 requests/s for 1.5 s) of H8/T10 block meshes
+with log-normal sizes (median 5e4 elements) and 30 % re-assemblies, through
+the continuous-batching server with 8 workers: every result is bitwise the
+same as when the same submissions run one at a time (device digests), and
+sampled columns of 8 results match the oracle.  Also: admission refusal and
+cancellation through the C ABI (conditions forced from the host)."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from feasm import _lib as L
+from feasm import core
+from feasm import workloads as W
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+MIXED_TOL = 1e-6
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_fro(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb else 1.0)
+
+
+def sampled_columns_match(etype, form, ndn, coeff, conn, xyz, csc, tol, nsample, seed):
+    """Columns of nsample nodes against the oracle assembled on just the
+    elements that hold them (row indices bit-exact, values within tol)."""
+    cp, rv, nz = csc
+    rng = np.random.default_rng(seed)
+    nodes = np.unique(rng.choice(xyz.shape[0], min(nsample, xyz.shape[0]), replace=False))
+    near = np.nonzero(np.isin(conn, nodes + 1).any(axis=1))[0]
+    ocp, orv, onz = O.assemble_csc(etype, form, conn[near], xyz, ndn, np.ravel(coeff))
+    for node in nodes:
+        for k in range(ndn):
+            j = ndn * node + k
+            got_r, got_v = rv[cp[j] - 1:cp[j + 1] - 1], nz[cp[j] - 1:cp[j + 1] - 1]
+            want_r, want_v = orv[ocp[j] - 1:ocp[j + 1] - 1], onz[ocp[j] - 1:ocp[j + 1] - 1]
+            assert np.array_equal(got_r, want_r), (node, k)
+            err = np.linalg.norm(got_v - want_v) / np.linalg.norm(want_v)
+            assert err <= tol, (node, k, err)
+    return len(nodes)
+
+
+def null_residual(ndn, cp, rv, nz, xyz=None):
+    """max |K u| / (max|K| max|u|) over the constant (ndn 1) or the three
+    translations (ndn 3), and over the three rotations when xyz is given."""
+    kmax = np.abs(nz).max()
+    n = len(cp) - 1
+    col = np.repeat(np.arange(n), np.diff(cp))
+    worst = 0.0
+    modes = []
+    if ndn == 1:
+        modes.append(np.ones(n))
+    else:
+        for k in range(3):
+            u = np.zeros((n // 3, 3))
+            u[:, k] = 1.0
+            modes.append(u.ravel())
+        if xyz is not None:
+            c = xyz - xyz.mean(axis=0)
+            for k in range(3):
+                w = np.zeros(3)
+                w[k] = 1.0
+                modes.append(np.cross(w, c).ravel())
+    for u in modes:
+        r = np.bincount(rv, weights=nz * u[col], minlength=n)
+        worst = max(worst, np.abs(r).max() / (kmax * np.abs(u).max()))
+    return worst
+
+
+def test_c4_batched_workload(ctx, tmp_path):
+    t0 = time.perf_counter()
+    specs = W.batched_specs()
+    meshes = []
+    ne_tot = 0
+    for (n, form, ndn, coeff, seed) in specs:
+        conn, xyz = W.h8_block(n, 0.1, seed)
+        m = core.DeviceMesh(ctx, L.H8, conn, xyz)
+        m.set_dofs(ndn)
+        meshes.append((m, conn, xyz))
+        ne_tot += conn.shape[0]
+    say("C4: %d meshes, %d elements (setup %.1f s)" % (len(specs), ne_tot, time.perf_counter() - t0))
+    assert len(specs) == 256 and 9e6 < ne_tot < 1.1e7
+    batch = core.Batch(ctx)
+    for (m, _, _), (n, form, ndn, coeff, _) in zip(meshes, specs):
+        batch.add(m, form, L.RULE_GAUSS, 2, coeff, precision=L.MIXED)
+    tuned = batch.autotune(db_path=str(tmp_path / "autotune.txt"), key="c4-test", reps=2)
+    say("C4: autotuned", tuned)
+    batch.prepare(use_graph=True, max_nodes=tuned["max_nodes"], budget_kib=tuned["budget_kib"],
+                  threads=tuned["threads"])
+    digests = []
+    for rep in range(3):
+        batch.run()
+        ctx.synchronize()
+        digests.append([batch.matrix(i).checksum() for i in range(len(specs))])
+        # eager work on the same stream between replays
+        E = meshes[1][0].bilform(specs[1][1], L.RULE_GAUSS, 2, specs[1][3])
+        E.close()
+        say("C4: replay", rep, "ms", batch.last_ms())
+    assert digests[0] == digests[1] == digests[2]
+    sampled = set(range(0, 256, 32)) | set(range(1, 256, 32))  # 4 diffusion + ... 16 meshes, both forms
+    worst_err = worst_null = 0.0
+    for i, ((m, conn, xyz), (n, form, ndn, coeff, _)) in enumerate(zip(meshes, specs)):
+        Bm = batch.matrix(i)
+        cp, rv, nz = Bm.csc_arrays(1)
+        want_nnz = ndn * ndn * (3 * n[0] + 1) * (3 * n[1] + 1) * (3 * n[2] + 1)
+        assert len(nz) == want_nnz, (i, len(nz), want_nnz)
+        K64 = m.bilform(form, L.RULE_GAUSS, 2, coeff)
+        cp64, rv64, nz64 = K64.csc_arrays(1)
+        K64.close()
+        assert np.array_equal(cp, cp64) and np.array_equal(rv, rv64)
+        err = rel_fro(nz, nz64)
+        worst_err = max(worst_err, err)
+        assert err <= MIXED_TOL, (i, err)
+        res = null_residual(ndn, cp - 1, rv - 1, nz, xyz if i in sampled else None)
+        worst_null = max(worst_null, res)
+        assert res <= 1e-5, (i, res)
+        if i in sampled:
+            sampled_columns_match(L.H8, form, ndn, coeff, conn, xyz, (cp, rv, nz), MIXED_TOL, 12, 40 + i)
+    say("C4: max mixed vs FP64 %.2e, max null residual %.2e, %.1f s" % (worst_err, worst_null,
+                                                                      time.perf_counter() - t0))
+    batch.close()
+    for m, _, _ in meshes:
+        m.close()
+
+
+def test_c5_served_stream_same_alone_and_in_the_mix():
+    t0 = time.perf_counter()
+    meshes, rng = W.served_pool()
+    stream = W.served_stream(rng, len(meshes), rate=200.0, duration=1.5)
+    # 8 requests to check against the oracle: both element types, both forms,
+    # new meshes and re-assemblies, meshes small enough to copy out quickly
+    small = [k for k, (_, pi, _, _, _) in enumerate(stream) if meshes[pi][1].shape[0] <= 200_000]
+    sample, seen = [], set()
+    for k in small:
+        _, pi, form, _, re = stream[k]
+        kind = (meshes[pi][0], form, re)
+        if kind not in seen:
+            seen.add(kind)
+            sample.append(k)
+    for k in small:
+        if len(sample) >= 8:
+            break
+        if k not in sample:
+            sample.append(k)
+    sample = sorted(sample[:8])
+    say("C5: %d requests, pool of %d meshes (%.1f s); oracle sample %s" % (len(stream), len(meshes),
+                                                                        time.perf_counter() - t0, sample))
+    mix, wall = W.run_served(stream, meshes, L.MIXED, workers=8, paced=True, digests=True,
+                             keep_results=set(sample))
+    summ = W.served_summary(mix, wall)
+    say("C5 in the mix:", summ)
+    alone, wall1 = W.run_served(stream, meshes, L.MIXED, workers=1, paced=False, sequential=True, digests=True,
+                                keep_results=set(sample))
+    say("C5 one at a time: %.2f s" % wall1)
+    assert len(mix) == len(alone) == len(stream)
+    for a, b in zip(mix, alone):
+        assert a["status"] == 2 and b["status"] == 2, (a["index"], a["status"], b["status"])
+        assert a["digest"] == b["digest"], a["index"]
+    for k in sample:
+        a, b = mix[k], alone[k]
+        for x, y in zip(a["csc"], b["csc"]):
+            assert np.array_equal(x, y)
+        _, pi, form, coeff, _ = stream[k]
+        et, conn, xyz, _ = meshes[pi]
+        ndn = 1 if form == L.DIFFUSION_ISO else 3
+        sampled_columns_match(et, form, ndn, coeff, conn, xyz, a["csc"], MIXED_TOL, 10, 70 + k)
+    say("C5: %.1f s" % (time.perf_counter() - t0))
+
+
+def test_server_refuses_requests_beyond_its_budget():
+    srv = core.Server(0, workers=2, hbm_fraction=1e-4)
+    try:
+        budget = srv.stats()["budget"]
+        say("server budget", budget)
+        conn, xyz = W.h8_block((40, 40, 40), 0.1, 3)
+        with pytest.raises(L.FeasmError) as ei:
+            srv.submit_mesh(L.H8, conn, xyz, 3, L.LIN_ELASTIC, L.RULE_GAUSS, 2, W.iso_D(200e3, 0.3).ravel(),
+                            L.FP64)
+        assert ei.value.code == L.ENOMEM and "admission refused" in str(ei.value)
+        assert srv.stats()["queued"] == 0
+        conn, xyz = W.h8_block((3, 3, 3), 0.1, 3)
+        rid, _ = srv.submit_mesh(L.H8, conn, xyz, 1, L.DIFFUSION_ISO, L.RULE_GAUSS, 2, [1.5], L.FP64)
+        st, _, _ = srv.wait(rid)
+        assert st == 2
+        cp, rv, nz = srv.result(rid).csc_arrays(1)
+        ocp, orv, onz = O.assemble_csc(O.H8, O.DIFFUSION_ISO, conn, xyz, 1, [1.5])
+        assert np.array_equal(cp, ocp) and np.array_equal(rv, orv) and rel_fro(nz, onz) <= 1e-12
+        srv.release(rid)
+    finally:
+        srv.close()
+
+
+def test_server_cancels_queued_and_running_requests():
+    srv = core.Server(0, workers=1, hbm_fraction=0.5)
+    try:
+        big = W.h8_block((60, 60, 60), 0.1, 1)
+        small = [W.h8_block((4, 3, 3), 0.1, 10 + i) for i in range(4)]
+        D = W.iso_D(200e3, 0.3).ravel()
+        r0, _ = srv.submit_mesh(L.H8, big[0], big[1], 3, L.LIN_ELASTIC, L.RULE_GAUSS, 2, D, L.FP64)
+        rs = [srv.submit_mesh(L.H8, c, x, 3, L.LIN_ELASTIC, L.RULE_GAUSS, 2, D, L.FP64)[0] for c, x in small]
+        # one worker: r0 runs (or is about to), the small ones are queued behind it
+        srv.cancel(rs[0])
+        srv.cancel(rs[2])
+        srv.cancel(r0)
+        for rid in (rs[0], rs[2], r0):
+            st, _, _ = srv.wait(rid)
+            assert st == L.ECANCELLED, (rid, st)
+            with pytest.raises(L.FeasmError) as ei:
+                srv.result(rid)
+            assert ei.value.code == L.ESTATE
+        for j in (1, 3):
+            st, _, _ = srv.wait(rs[j])
+            assert st == 2
+            cp, rv, nz = srv.result(rs[j]).csc_arrays(1)
+            ocp, orv, onz = O.assemble_csc(O.H8, O.LIN_ELASTIC, small[j][0], small[j][1], 3, D)
+            assert np.array_equal(cp, ocp) and np.array_equal(rv, orv) and rel_fro(nz, onz) <= 1e-12
+            with pytest.raises(L.FeasmError) as ei:
+                srv.cancel(rs[j])  # already finished
+            assert ei.value.code == L.ESTATE
+        st = srv.stats()
+        assert st["completed"] == 5 and st["queued"] == 0 and st["running"] == 0
+        for rid in [r0] + rs:
+            srv.release(rid)
+    finally:
+        srv.close()
