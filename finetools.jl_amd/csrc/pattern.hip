@@ -66,40 +66,80 @@ __global__ void k_n2e_sort(const int64_t *__restrict__ ptr, uint32_t *__restrict
     }
 }
 
-// distinct neighbours of node c (including c), sorted; count pass writes
-// deg[c], fill pass writes adj[adj_ptr[c] ...]
-__global__ void k_adj(const int64_t *__restrict__ n2e_ptr, const uint32_t *__restrict__ n2e,
-                      const int32_t *__restrict__ conn, int npe, int64_t nnodes, const int64_t *__restrict__ adj_ptr,
-                      int32_t *__restrict__ deg, int32_t *__restrict__ adj, int32_t *__restrict__ overflow) {
-    int32_t list[kCap];
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nnodes;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        int len = 0;
-        bool over = false;
-        for (int64_t t = n2e_ptr[c]; t < n2e_ptr[c + 1]; t++) {
-            const int64_t e = n2e[t] >> 4;
-            for (int b = 0; b < npe; b++) {
-                const int32_t m = conn[e * npe + b];
-                // insertion into the sorted unique list
-                int j = len - 1;
-                while (j >= 0 && list[j] > m) j--;
-                if (j >= 0 && list[j] == m) continue;
-                if (len == kCap) {
-                    over = true;
-                    continue;
+// Distinct neighbours of node c (including c), sorted: one wavefront (one
+// 64-thread block) per node.  The node's candidates (the nodes of its
+// elements, npe per element) are gathered into LDS, padded to a power of two
+// and sorted with a bitonic network; the distinct values are the ones that
+// differ from their predecessor, placed by a prefix count over the lanes'
+// runs.  Count pass (adj_ptr == nullptr) writes deg[c]; fill pass writes
+// adj[adj_ptr[c] ...].
+constexpr int kMaxCand = 2048;  // candidates of one node: incident elements x npe
+
+__global__ __launch_bounds__(64) void k_adj(const int64_t *__restrict__ n2e_ptr, const uint32_t *__restrict__ n2e,
+                                            const int32_t *__restrict__ conn, int npe, int64_t nnodes,
+                                            const int64_t *__restrict__ adj_ptr, int32_t *__restrict__ deg,
+                                            int32_t *__restrict__ adj, int32_t *__restrict__ overflow) {
+    __shared__ int32_t cv[kMaxCand];
+    __shared__ int32_t part[64];
+    const int lane = threadIdx.x;
+    constexpr int32_t kPad = 0x7fffffff;  // above every node number
+    for (int64_t c = blockIdx.x; c < nnodes; c += gridDim.x) {
+        const int64_t s0 = n2e_ptr[c];
+        int L = (int)((n2e_ptr[c + 1] - s0) * npe);
+        if (L > kMaxCand) {
+            if (lane == 0) *overflow = 1;
+            L = 0;
+        }
+        int P = 1;
+        while (P < L) P <<= 1;
+        for (int i = lane; i < P; i += 64) {
+            int32_t v = kPad;
+            if (i < L) v = conn[(int64_t)(n2e[s0 + i / npe] >> 4) * npe + i % npe];
+            cv[i] = v;
+        }
+        __syncthreads();
+        for (int k = 2; k <= P; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = lane; i < P; i += 64) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const int32_t x = cv[i], y = cv[ixj];
+                        if ((x > y) == ((i & k) == 0)) {
+                            cv[i] = y;
+                            cv[ixj] = x;
+                        }
+                    }
                 }
-                for (int k = len - 1; k > j; k--) list[k + 1] = list[k];
-                list[j + 1] = m;
-                len++;
+                __syncthreads();
+            }
+        // each lane takes a run of consecutive sorted entries
+        const int per = (P + 63) / 64, beg = min(lane * per, P), end = min(beg + per, P);
+        int cnt = 0;
+        for (int i = beg; i < end; i++) {
+            const int32_t v = cv[i];
+            cnt += (v != kPad && (i == 0 || cv[i - 1] != v)) ? 1 : 0;
+        }
+        part[lane] = cnt;
+        __syncthreads();
+        int off = 0, tot = 0;
+        for (int q = 0; q < 64; q++) {
+            const int pq = part[q];
+            off += (q < lane) ? pq : 0;
+            tot += pq;
+        }
+        if (tot > kCap) {
+            if (lane == 0) *overflow = 1;
+        } else if (adj_ptr == nullptr) {
+            if (lane == 0) deg[c] = tot;
+        } else {
+            const int64_t o = adj_ptr[c] + off;
+            int w = 0;
+            for (int i = beg; i < end; i++) {
+                const int32_t v = cv[i];
+                if (v != kPad && (i == 0 || cv[i - 1] != v)) adj[o + w++] = v;
             }
         }
-        if (over) *overflow = 1;
-        if (adj_ptr == nullptr) {
-            deg[c] = len;
-        } else {
-            const int64_t o = adj_ptr[c];
-            for (int k = 0; k < len; k++) adj[o + k] = list[k];
-        }
+        __syncthreads();  // cv and part are refilled for the next node
     }
 }
 
@@ -109,32 +149,35 @@ __global__ void k_bcol(const int64_t *__restrict__ adj_ptr, int64_t nnodes, int3
         for (int64_t t = adj_ptr[c]; t < adj_ptr[c + 1]; t++) bcol[t] = (int32_t)c;
 }
 
-// contributions of block t = (row node m, column node c): elements holding
-// both nodes, ascending; count pass (cptr == nullptr) or fill pass
-__global__ void k_contrib(const int32_t *__restrict__ bcol, const int32_t *__restrict__ adj, int64_t nblocks,
-                          const int64_t *__restrict__ n2e_ptr, const uint32_t *__restrict__ n2e,
-                          const int32_t *__restrict__ conn, int npe, const int64_t *__restrict__ cptr,
-                          int32_t *__restrict__ ccount, uint32_t *__restrict__ contrib) {
+// Contributions of block t = (row node m, column node c): the elements
+// holding both nodes, ascending -- one thread per block, merging the two
+// nodes' element lists (both sorted by element, then local index).  Every
+// entry (e, ac) of c's list whose element also holds m gives one record, with
+// am the first local index of m in e.  Count pass (cptr == nullptr) or fill
+// pass.
+__global__ __launch_bounds__(256) void k_contrib(const int32_t *__restrict__ bcol, const int32_t *__restrict__ adj,
+                                                 int64_t nblocks, const int64_t *__restrict__ n2e_ptr,
+                                                 const uint32_t *__restrict__ n2e, const int64_t *__restrict__ cptr,
+                                                 int32_t *__restrict__ ccount, uint32_t *__restrict__ contrib) {
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nblocks;
          t += (int64_t)gridDim.x * blockDim.x) {
         const int32_t c = bcol[t], m = adj[t];
         int cnt = 0;
         int64_t w = cptr ? cptr[t] : 0;
+        int64_t j = n2e_ptr[m];
+        const int64_t je = n2e_ptr[m + 1];
         for (int64_t s = n2e_ptr[c]; s < n2e_ptr[c + 1]; s++) {
             const uint32_t u = n2e[s];
-            const int64_t e = u >> 4;
-            const int ac = (int)(u & 15u);
-            int am = -1;
-            for (int b = 0; b < npe; b++)
-                if (conn[e * npe + b] == m) {
-                    am = b;
-                    break;
-                }
-            if (am < 0) continue;
+            const uint32_t e = u >> 4;
+            while (j < je && (n2e[j] >> 4) < e) j++;
+            if (j == je) break;
+            const uint32_t v = n2e[j];
+            if ((v >> 4) != e) continue;
             if (cptr) {
+                const int ac = (int)(u & 15u), am = (int)(v & 15u);
                 const int lo = am < ac ? am : ac, hi = am < ac ? ac : am;
                 const uint32_t p = (uint32_t)(hi * (hi + 1) / 2 + lo);
-                contrib[w++] = ((uint32_t)e << 7) | (p << 1) | (am > ac ? 1u : 0u);
+                contrib[w++] = (e << 7) | (p << 1) | (am > ac ? 1u : 0u);
             }
             cnt++;
         }
@@ -142,20 +185,24 @@ __global__ void k_contrib(const int32_t *__restrict__ bcol, const int32_t *__res
     }
 }
 
-__global__ void k_dof_pattern(const int64_t *__restrict__ adj_ptr, const int32_t *__restrict__ adj, int64_t nnodes,
-                              int ndn, int64_t *__restrict__ colptr, int32_t *__restrict__ rowval) {
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nnodes;
-         c += (int64_t)gridDim.x * blockDim.x) {
+// One wavefront per node column: its ndn DOF columns are ndn^2 deg
+// consecutive rowval entries, written by consecutive lanes.
+__global__ __launch_bounds__(256) void k_dof_pattern(const int64_t *__restrict__ adj_ptr,
+                                                     const int32_t *__restrict__ adj, int64_t nnodes, int ndn,
+                                                     int64_t *__restrict__ colptr, int32_t *__restrict__ rowval) {
+    const int lane = threadIdx.x % 64;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; c < nnodes; c += nw) {
         const int64_t b = adj_ptr[c];
-        const int64_t deg = adj_ptr[c + 1] - b;
+        const int deg = (int)(adj_ptr[c + 1] - b);
         const int64_t base = (int64_t)ndn * ndn * b;
-        for (int l = 0; l < ndn; l++) {
-            const int64_t col0 = base + (int64_t)l * ndn * deg;
-            colptr[c * ndn + l] = col0;
-            for (int64_t r = 0; r < deg; r++)
-                for (int k = 0; k < ndn; k++) rowval[col0 + r * ndn + k] = adj[b + r] * ndn + k;
+        if (lane < ndn) colptr[c * ndn + lane] = base + (int64_t)lane * ndn * deg;
+        if (c == nnodes - 1 && lane == 0) colptr[nnodes * ndn] = base + (int64_t)ndn * ndn * deg;
+        const int per_col = ndn * deg, total = ndn * per_col;
+        for (int idx = lane; idx < total; idx += 64) {
+            const int rem = idx % per_col;
+            rowval[base + idx] = adj[b + rem / ndn] * ndn + rem % ndn;
         }
-        if (c == nnodes - 1) colptr[nnodes * ndn] = base + (int64_t)ndn * ndn * deg;
     }
 }
 
@@ -229,8 +276,7 @@ void build_contributions(feasm_mesh *m, Pattern &P_) {
     FEASM_HIP(hipMemsetAsync(ccount->p, 0, (P->nblocks + 1) * sizeof(int32_t), ctx->stream));
     hipLaunchKernelGGL(k_contrib, dim3(grid_for(P->nblocks, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(P->bcol),
                        dp<int32_t>(P->adj), P->nblocks, dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e),
-                       dp<int32_t>(m->conn), m->npe, (const int64_t *)nullptr, dp<int32_t>(ccount),
-                       (uint32_t *)nullptr);
+                       (const int64_t *)nullptr, dp<int32_t>(ccount), (uint32_t *)nullptr);
     FEASM_HIP(hipGetLastError());
     P->cptr = dev_alloc((P->nblocks + 1) * sizeof(int64_t));
     exclusive_scan_i32_to_i64(ctx, dp<int32_t>(ccount), dp<int64_t>(P->cptr), P->nblocks + 1);
@@ -238,8 +284,7 @@ void build_contributions(feasm_mesh *m, Pattern &P_) {
     DevPtr contrib = dev_alloc(std::max<size_t>(P->ncontrib * sizeof(uint32_t), 4));
     hipLaunchKernelGGL(k_contrib, dim3(grid_for(P->nblocks, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(P->bcol),
                        dp<int32_t>(P->adj), P->nblocks, dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e),
-                       dp<int32_t>(m->conn), m->npe, dp<int64_t>(P->cptr), (int32_t *)nullptr,
-                       dp<uint32_t>(contrib));
+                       dp<int64_t>(P->cptr), (int32_t *)nullptr, dp<uint32_t>(contrib));
     FEASM_HIP(hipGetLastError());
     FEASM_HIP(hipStreamSynchronize(ctx->stream));
     P->contrib = contrib;
@@ -257,7 +302,8 @@ std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn) {
     DevPtr overflow = dev_alloc(sizeof(int32_t));
     FEASM_HIP(hipMemsetAsync(overflow->p, 0, sizeof(int32_t), ctx->stream));
     FEASM_HIP(hipMemsetAsync(deg->p, 0, (nn + 1) * sizeof(int32_t), ctx->stream));
-    hipLaunchKernelGGL(k_adj, dim3(grid_for(nn, 64)), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr),
+    const unsigned gadj = grid_for(nn, 1, 1 << 17);
+    hipLaunchKernelGGL(k_adj, dim3(gadj), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr),
                        dp<uint32_t>(m->n2e), dp<int32_t>(m->conn), m->npe, nn, (const int64_t *)nullptr,
                        dp<int32_t>(deg), (int32_t *)nullptr, dp<int32_t>(overflow));
     FEASM_HIP(hipGetLastError());
@@ -266,9 +312,10 @@ std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn) {
     P->nblocks = read_i64(ctx, dp<int64_t>(P->adj_ptr) + nn);
     int32_t ovf = 0;
     FEASM_HIP(hipMemcpy(&ovf, overflow->p, sizeof(int32_t), hipMemcpyDeviceToHost));
-    FEASM_REQUIRE(ovf == 0, FEASM_EUNSUPPORTED, "a node has more than 256 distinct neighbours");
+    FEASM_REQUIRE(ovf == 0, FEASM_EUNSUPPORTED,
+                  "a node has more than 256 distinct neighbours or more than 2048 (element, node) incidences");
     P->adj = dev_alloc(P->nblocks * sizeof(int32_t));
-    hipLaunchKernelGGL(k_adj, dim3(grid_for(nn, 64)), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr),
+    hipLaunchKernelGGL(k_adj, dim3(gadj), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr),
                        dp<uint32_t>(m->n2e), dp<int32_t>(m->conn), m->npe, nn, dp<int64_t>(P->adj_ptr),
                        (int32_t *)nullptr, dp<int32_t>(P->adj), dp<int32_t>(overflow));
     P->bcol = dev_alloc(P->nblocks * sizeof(int32_t));
@@ -282,7 +329,7 @@ std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn) {
     P->colptr = dev_alloc((P->n + 1) * sizeof(int64_t));
     P->rowval = dev_alloc(std::max<size_t>(P->nnz * sizeof(int32_t), 4));
     FEASM_HIP(hipMemsetAsync(P->colptr->p, 0, (P->n + 1) * sizeof(int64_t), ctx->stream));
-    hipLaunchKernelGGL(k_dof_pattern, dim3(grid_for(nn, 256)), dim3(256), 0, ctx->stream, dp<int64_t>(P->adj_ptr),
+    hipLaunchKernelGGL(k_dof_pattern, dim3(grid_for((nn + 3) / 4, 1, 65536)), dim3(256), 0, ctx->stream, dp<int64_t>(P->adj_ptr),
                        dp<int32_t>(P->adj), nn, ndn, dp<int64_t>(P->colptr), dp<int32_t>(P->rowval));
     FEASM_HIP(hipGetLastError());
     FEASM_HIP(hipStreamSynchronize(ctx->stream));

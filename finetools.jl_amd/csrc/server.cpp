@@ -52,6 +52,9 @@ struct Request {
     int64_t nelem = 0, nnodes = 0;
     std::vector<int32_t> conn0;
     std::vector<double> xyz;
+    // borrowed submissions: the caller's arrays, converted by the worker
+    const int64_t *bconn = nullptr;
+    const double *bxyz = nullptr;
     bool keep = false;
     // both
     int64_t mesh_id = -1;
@@ -184,6 +187,17 @@ struct feasm_server {
                 m->npe = r->npe;
                 m->nelem = r->nelem;
                 m->nnodes = r->nnodes;
+                if (r->bconn) {  // borrowed arrays: converted and checked here, in the worker
+                    r->conn0.resize(r->nelem * r->npe);
+                    for (int64_t i = 0; i < r->nelem * r->npe; i++) {
+                        FEASM_REQUIRE(r->bconn[i] >= 1 && r->bconn[i] <= r->nnodes, FEASM_EINVAL,
+                                      "connectivity refers to a missing node");
+                        r->conn0[i] = (int32_t)(r->bconn[i] - 1);
+                    }
+                    r->xyz.assign(r->bxyz, r->bxyz + 3 * r->nnodes);
+                    r->bconn = nullptr;
+                    r->bxyz = nullptr;
+                }
                 m->conn = dev_alloc(std::max<size_t>(r->conn0.size() * 4, 4));
                 m->xyz = dev_alloc(std::max<size_t>(r->xyz.size() * 8, 8));
                 FEASM_HIP(hipMemcpyAsync(m->conn->p, r->conn0.data(), r->conn0.size() * 4, hipMemcpyHostToDevice,
@@ -287,10 +301,10 @@ extern "C" int feasm_server_create(int device, int workers, double hbm_fraction,
     S_CATCH(nullptr)
 }
 
-extern "C" int feasm_server_submit_mesh(feasm_server *s, int elem_type, int64_t nelem, int npe, const int64_t *conn,
-                                        int64_t nnodes, const double *xyz_rowmajor, int ndn, int form, int rule,
-                                        int order, const double *coeff, int ncoeff, int precision,
-                                        int keep_resident, int64_t *request_id, int64_t *mesh_id) {
+static int submit_mesh(feasm_server *s, bool borrowed, int elem_type, int64_t nelem, int npe, const int64_t *conn,
+                       int64_t nnodes, const double *xyz_rowmajor, int ndn, int form, int rule, int order,
+                       const double *coeff, int ncoeff, int precision, int keep_resident, int64_t *request_id,
+                       int64_t *mesh_id) {
     S_TRY
     FEASM_REQUIRE(s && conn && xyz_rowmajor && request_id, FEASM_EINVAL, "null argument");
     FEASM_REQUIRE(npe == (elem_type == FEASM_H8 ? 8 : 10) && (elem_type == FEASM_H8 || elem_type == FEASM_T10),
@@ -304,12 +318,17 @@ extern "C" int feasm_server_submit_mesh(feasm_server *s, int elem_type, int64_t 
     r->ndn = ndn;
     r->nelem = nelem;
     r->nnodes = nnodes;
-    r->conn0.resize(nelem * npe);
-    for (int64_t i = 0; i < nelem * npe; i++) {
-        FEASM_REQUIRE(conn[i] >= 1 && conn[i] <= nnodes, FEASM_EINVAL, "connectivity refers to a missing node");
-        r->conn0[i] = (int32_t)(conn[i] - 1);
+    if (borrowed) {
+        r->bconn = conn;
+        r->bxyz = xyz_rowmajor;
+    } else {
+        r->conn0.resize(nelem * npe);
+        for (int64_t i = 0; i < nelem * npe; i++) {
+            FEASM_REQUIRE(conn[i] >= 1 && conn[i] <= nnodes, FEASM_EINVAL, "connectivity refers to a missing node");
+            r->conn0[i] = (int32_t)(conn[i] - 1);
+        }
+        r->xyz.assign(xyz_rowmajor, xyz_rowmajor + 3 * nnodes);
     }
-    r->xyz.assign(xyz_rowmajor, xyz_rowmajor + 3 * nnodes);
     r->fp = make_form_params(form, ndn, coeff, ncoeff);
     r->rule = rule;
     r->order = order;
@@ -329,6 +348,23 @@ extern "C" int feasm_server_submit_mesh(feasm_server *s, int elem_type, int64_t 
     if (mesh_id) *mesh_id = r->mesh_id;
     *request_id = s->enqueue(std::move(r));
     S_CATCH(s)
+}
+
+extern "C" int feasm_server_submit_mesh(feasm_server *s, int elem_type, int64_t nelem, int npe, const int64_t *conn,
+                                        int64_t nnodes, const double *xyz_rowmajor, int ndn, int form, int rule,
+                                        int order, const double *coeff, int ncoeff, int precision,
+                                        int keep_resident, int64_t *request_id, int64_t *mesh_id) {
+    return submit_mesh(s, false, elem_type, nelem, npe, conn, nnodes, xyz_rowmajor, ndn, form, rule, order, coeff,
+                       ncoeff, precision, keep_resident, request_id, mesh_id);
+}
+
+extern "C" int feasm_server_submit_mesh_borrowed(feasm_server *s, int elem_type, int64_t nelem, int npe,
+                                                 const int64_t *conn, int64_t nnodes, const double *xyz_rowmajor,
+                                                 int ndn, int form, int rule, int order, const double *coeff,
+                                                 int ncoeff, int precision, int keep_resident, int64_t *request_id,
+                                                 int64_t *mesh_id) {
+    return submit_mesh(s, true, elem_type, nelem, npe, conn, nnodes, xyz_rowmajor, ndn, form, rule, order, coeff,
+                       ncoeff, precision, keep_resident, request_id, mesh_id);
 }
 
 extern "C" int feasm_server_submit_reassembly(feasm_server *s, int64_t mesh_id, int form, int rule, int order,

@@ -83,6 +83,8 @@ SIGNATURES = {
     "feasm_batch_destroy": (I32, [P]),
     "feasm_server_create": (I32, [I32, I32, DBL, PP]),
     "feasm_server_submit_mesh": (I32, [P, I32, I64, I32, P, I64, P, I32, I32, I32, I32, P, I32, I32, I32, PI64, PI64]),
+    "feasm_server_submit_mesh_borrowed": (I32, [P, I32, I64, I32, P, I64, P, I32, I32, I32, I32, P, I32, I32, I32, PI64,
+                                                PI64]),
     "feasm_server_submit_reassembly": (I32, [P, I64, I32, I32, I32, P, I32, I32, PI64]),
     "feasm_server_wait": (I32, [P, I64, DBL, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_double)]),

@@ -284,6 +284,7 @@ class Server:
         if rc != L.OK:
             raise L.FeasmError(rc, lib().feasm_server_last_error(None).decode())
         self.h = h
+        self._borrowed = {}  # request id -> arrays the server reads until the request ends
 
     def _check(self, rc):
         if rc != L.OK:
@@ -293,18 +294,23 @@ class Server:
         if getattr(self, "h", None):
             lib().feasm_server_destroy(self.h)
             self.h = None
+        self._borrowed = {}
 
     __del__ = close
 
-    def submit_mesh(self, etype, conn, xyz, ndn, form, rule, order, coeff, precision=L.FP64, keep=False):
+    def submit_mesh(self, etype, conn, xyz, ndn, form, rule, order, coeff, precision=L.FP64, keep=False,
+                    borrow=False):
+        """borrow: the server reads conn and xyz in its worker, without a copy;
+        the arrays are held here until the request is released."""
         conn = np.ascontiguousarray(conn, dtype=np.int64)
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)
         c = np.ascontiguousarray(np.atleast_1d(coeff), dtype=np.float64).ravel()
         rid, mid = ctypes.c_int64(), ctypes.c_int64()
-        self._check(lib().feasm_server_submit_mesh(self.h, etype, conn.shape[0], conn.shape[1], ptr(conn),
-                                                   xyz.shape[0], ptr(xyz), ndn, form, rule, order, ptr(c), len(c),
-                                                   precision, 1 if keep else 0, ctypes.byref(rid),
-                                                   ctypes.byref(mid)))
+        fn = lib().feasm_server_submit_mesh_borrowed if borrow else lib().feasm_server_submit_mesh
+        self._check(fn(self.h, etype, conn.shape[0], conn.shape[1], ptr(conn), xyz.shape[0], ptr(xyz), ndn, form, rule,
+                       order, ptr(c), len(c), precision, 1 if keep else 0, ctypes.byref(rid), ctypes.byref(mid)))
+        if borrow:
+            self._borrowed[rid.value] = (conn, xyz)
         return rid.value, mid.value
 
     def submit_reassembly(self, mesh_id, form, rule, order, coeff, precision=L.FP64):
@@ -331,6 +337,7 @@ class Server:
 
     def release(self, rid):
         self._check(lib().feasm_server_release(self.h, rid))
+        self._borrowed.pop(rid, None)
 
     def stats(self):
         v = [ctypes.c_int64() for _ in range(6)]

@@ -120,7 +120,7 @@ def run_served(stream, meshes, precision, workers, paced=True, sequential=False,
                 rid = srv.submit_reassembly(resident[key], form, rule[0], rule[1], coeff, precision)
             else:
                 rid, mid = srv.submit_mesh(et, conn, xyz, ndn, form, rule[0], rule[1], coeff, precision,
-                                           keep=key not in resident)
+                                           keep=key not in resident, borrow=True)
                 if key not in resident:
                     resident[key] = mid
             if sequential:

@@ -359,6 +359,15 @@ int feasm_server_submit_mesh(feasm_server *s, int elem_type, int64_t nelem, int 
                              int64_t nnodes, const double *xyz_rowmajor, int ndn, int form, int rule, int order,
                              const double *coeff, int ncoeff, int precision, int keep_resident, int64_t *request_id,
                              int64_t *mesh_id);
+/* The same without copying: conn and xyz_rowmajor are borrowed and must stay
+ * valid and unchanged until feasm_server_wait reports the request finished;
+ * the worker that takes the request converts and checks them (an entry of
+ * conn outside 1..nnodes fails the request with FEASM_EINVAL), so the
+ * submitting thread does no per-element work. */
+int feasm_server_submit_mesh_borrowed(feasm_server *s, int elem_type, int64_t nelem, int npe, const int64_t *conn,
+                                      int64_t nnodes, const double *xyz_rowmajor, int ndn, int form, int rule,
+                                      int order, const double *coeff, int ncoeff, int precision, int keep_resident,
+                                      int64_t *request_id, int64_t *mesh_id);
 /* Queue a re-assembly of a resident mesh with new coefficients (pattern and
  * tiles reused: numeric phase only). */
 int feasm_server_submit_reassembly(feasm_server *s, int64_t mesh_id, int form, int rule, int order,
