@@ -361,6 +361,34 @@ def bench_served(feasm, core, rate=200.0, duration=3.0, workers=8):
     return out
 
 
+def bench_gram(feasm, ne=262144, reps=50, rounds=2):
+    """The per-element node-pair Gram (8 nodes, 8 points, 3 x 3 blocks for the
+    36 pairs a <= b) from element records in LDS, on the FP64 vector ALUs and
+    on the FP64 matrix cores (v_mfma_f64_16x16x4f64, the 24 x 8 gradient
+    table padded to 32 rows: six MFMA per element), interleaved rounds."""
+    import ctypes
+
+    ctx = feasm.Context(0)
+    lib = feasm.lib()
+    out = {"workload": "%d synthetic H8 element records x %d passes, LDS resident" % (ne, reps)}
+    useful = 36 * 72 * 2  # flops of one element's upper node-pair Gram
+    issued = {"vector": useful, "mfma": 6 * 16 * 16 * 4 * 2}
+    best = {}
+    for _ in range(rounds):
+        for name, v in (("vector", 0), ("mfma", 1)):
+            ms, cs = ctypes.c_double(), ctypes.c_double()
+            feasm._lib.check(lib.feasm_probe_gram(ctx.h, v, ne, reps, ctypes.byref(ms), ctypes.byref(cs), None, 0),
+                             ctx.h)
+            best[name] = min(best.get(name, 1e30), ms.value)
+    for name, ms in best.items():
+        rate = ne * reps / (ms * 1e-3)
+        out[name] = {"ms": ms, "element_grams_per_s": rate, "useful_fp64_tflops": rate * useful / 1e12,
+                     "issued_fp64_tflops": rate * issued[name] / 1e12}
+    out["mfma_over_vector"] = best["vector"] / best["mfma"]
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -369,8 +397,8 @@ def main():
     ap.add_argument("--cells", type=int, default=100, help="cells per direction of the H8 block")
     ap.add_argument("--cpu-cells", type=int, default=50, help="cells per direction of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extras", default="mixed,vec,t10,batch,served",
-                    help="comma list of extra workloads: mixed,vec,t10,batch,served (or 'none')")
+    ap.add_argument("--extras", default="mixed,vec,t10,batch,served,gram",
+                    help="comma list of extra workloads: mixed,vec,t10,batch,served,gram (or 'none')")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB streaming copy (3 launches) to scale traffic counters")
     a = ap.parse_args()
@@ -495,6 +523,9 @@ def main():
     if "served" in extras:
         log("served ...")
         out["served"] = bench_served(feasm, core)
+    if "gram" in extras:
+        log("element Gram, vector vs matrix cores ...")
+        out["gram_probe"] = bench_gram(feasm)
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(a.cpu_cells, 2, D)

@@ -112,7 +112,29 @@ struct TileSet {
     DevPtr hnodes;  // int32 distinct nodes of each tile's halo, ascending
 };
 
+// The mesh arrays the tile builder reads (host copies of the device ones):
+// 0-based connectivity, coordinates, node -> element lists and the node
+// adjacency of the pattern.
+struct TileHostMesh {
+    int npe = 0;
+    int64_t nn = 0, ne = 0;
+    std::vector<int32_t> conn;
+    std::vector<double> xyz;
+    std::vector<int64_t> n2e_ptr, adj_ptr;
+    std::vector<uint32_t> n2e;
+    std::vector<int32_t> adj;
+};
+// The tiles as built on the host, before the upload.
+struct HostTiles {
+    int64_t ntiles = 0, nhalo = 0, ntasks = 0, nitems = 0, blob_bytes = 0;
+    int max_nodes = 0, max_halo = 0, max_k = 0;
+    std::vector<TileHdr> hdr;
+    std::unique_ptr<unsigned char[]> blob;
+    size_t blob_size = 0;
+    std::vector<int32_t> hnodes;
+};
 // max_threads bounds the host threads building the per-tile arrays
+void build_tiles_host(const TileHostMesh &h, const TileShape &shape, int max_threads, HostTiles &out);
 std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int max_threads = 16);
 // rewrites the coordinates in every tile's blob from the mesh's coordinates
 // (on the mesh's context stream)

@@ -19,7 +19,10 @@
 // the LDS budget; the cut is then moved back to the coarsest Morton boundary
 // in the second half of the tile, which keeps tiles box-shaped.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -42,15 +45,7 @@ uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
     return v;
 }
 
-struct HostMesh {
-    int npe = 0;
-    int64_t nn = 0, ne = 0;
-    std::vector<int32_t> conn;
-    std::vector<double> xyz;
-    std::vector<int64_t> n2e_ptr, adj_ptr;
-    std::vector<uint32_t> n2e;
-    std::vector<int32_t> adj;
-};
+using HostMesh = TileHostMesh;
 
 // Node order along a Z-curve of a lattice whose spacing is the mean node
 // spacing, so that runs of the order are nearly box-shaped.
@@ -447,27 +442,24 @@ void make_tile(const HostMesh &h, const int32_t *owned, int P, const TileShape &
 
 }  // namespace
 
-std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int max_threads) {
-    const Pattern &p = *m->pattern;
-    FEASM_HIP(hipStreamSynchronize(m->ctx->stream));  // mesh uploads may still be in flight on it
-    HostMesh h;
-    h.npe = m->npe;
-    h.nn = m->nnodes;
-    h.ne = m->nelem;
-    h.conn.resize(h.ne * h.npe);
-    h.xyz.resize(h.nn * 3);
-    h.n2e_ptr.resize(h.nn + 1);
-    h.n2e.resize(h.ne * h.npe);
-    h.adj_ptr.resize(h.nn + 1);
-    FEASM_HIP(hipMemcpy(h.conn.data(), m->conn->p, h.conn.size() * 4, hipMemcpyDeviceToHost));
-    FEASM_HIP(hipMemcpy(h.xyz.data(), m->xyz->p, h.xyz.size() * 8, hipMemcpyDeviceToHost));
-    FEASM_HIP(hipMemcpy(h.n2e_ptr.data(), m->n2e_ptr->p, (h.nn + 1) * 8, hipMemcpyDeviceToHost));
-    FEASM_HIP(hipMemcpy(h.n2e.data(), m->n2e->p, h.n2e.size() * 4, hipMemcpyDeviceToHost));
-    FEASM_HIP(hipMemcpy(h.adj_ptr.data(), p.adj_ptr->p, (h.nn + 1) * 8, hipMemcpyDeviceToHost));
-    h.adj.resize(h.adj_ptr[h.nn]);
-    FEASM_HIP(hipMemcpy(h.adj.data(), p.adj->p, h.adj.size() * 4, hipMemcpyDeviceToHost));
+namespace {
+// FEASM_TILE_TIMING=1 prints the tile builder's phase times to stderr
+struct PhaseClock {
+    bool on = std::getenv("FEASM_TILE_TIMING") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void lap(const char *what) {
+        if (!on) return;
+        const auto n = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "tiles: %-24s %8.2f ms\n", what, std::chrono::duration<double, std::milli>(n - t).count());
+        t = n;
+    }
+};
+}  // namespace
 
+void build_tiles_host(const TileHostMesh &h, const TileShape &shape, int max_threads, HostTiles &out) {
+    PhaseClock clk;
     const auto key = morton_order(h);
+    clk.lap("morton order");
     const int64_t nn = h.nn;
     const int npe = h.npe;
 
@@ -584,6 +576,7 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int 
         for (int w = 0; w < nt; w++)
             if (codes[w]) throw Error(codes[w], errs[w]);
     }
+    clk.lap("partition");
     std::vector<int64_t> starts;
     for (const auto &cs : chunk_starts) starts.insert(starts.end(), cs.begin(), cs.end());
     starts.push_back(nn);
@@ -616,11 +609,12 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int 
     for (auto &th : pool) th.join();
     for (int w = 0; w < nthreads; w++)
         if (codes[w]) throw Error(codes[w], errs[w]);
+    clk.lap("per-tile arrays");
 
-    auto T = std::make_shared<TileSet>();
-    T->shape = shape;
+    HostTiles *T = &out;
     T->ntiles = ntiles;
-    std::vector<TileHdr> hdr(ntiles);
+    std::vector<TileHdr> &hdr = T->hdr;
+    hdr.assign(ntiles, TileHdr{});
     int64_t nX = 0, nB = 0;
     for (int64_t t = 0; t < ntiles; t++) {
         const TileOut &o = outs[t];
@@ -649,14 +643,17 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int 
     FEASM_REQUIRE(nX < INT32_MAX, FEASM_EUNSUPPORTED, "mesh too large for 32-bit tile offsets");
     T->blob_bytes = nB;
     const size_t blob_size = (size_t)std::max<int64_t>(nB, 16);
-    std::unique_ptr<unsigned char[]> blob(new unsigned char[blob_size]);  // every byte is written below
-    std::memset(blob.get() + nB, 0, blob_size - (size_t)nB);
-    std::vector<int32_t> hn(std::max<int64_t>(nX, 1));
+    T->blob.reset(new unsigned char[blob_size]);  // every byte is written below
+    T->blob_size = blob_size;
+    unsigned char *const blob_base = T->blob.get();
+    std::memset(blob_base + nB, 0, blob_size - (size_t)nB);
+    std::vector<int32_t> &hn = T->hnodes;
+    hn.assign(std::max<int64_t>(nX, 1), 0);
     auto fill = [&](int64_t t) {
         const TileOut &o = outs[t];
         const TileHdr &d = hdr[t];
         const TileShape::Layout L = shape.layout(d.H, d.P, d.NH, d.K, d.NT, d.NI);
-        unsigned char *b = blob.get() + d.blob;
+        unsigned char *b = blob_base + d.blob;
         std::memset(b, 0, (size_t)d.nbytes);  // alignment gaps
         auto put = [&](size_t off, const void *src, size_t bytes) {
             if (bytes) std::memcpy(b + (off - L.cbase), src, bytes);
@@ -684,12 +681,50 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int 
             });
         for (auto &th : fillers) th.join();
     }
-    T->blob = dev_alloc(blob_size);
-    FEASM_HIP(hipMemcpy(T->blob->p, blob.get(), blob_size, hipMemcpyHostToDevice));
-    T->hnodes = dev_alloc(hn.size() * 4);
-    FEASM_HIP(hipMemcpy(T->hnodes->p, hn.data(), hn.size() * 4, hipMemcpyHostToDevice));
-    T->hdr = dev_alloc(std::max<size_t>(hdr.size() * sizeof(TileHdr), 16));
-    FEASM_HIP(hipMemcpy(T->hdr->p, hdr.data(), hdr.size() * sizeof(TileHdr), hipMemcpyHostToDevice));
+    clk.lap("blobs");
+}
+
+std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int max_threads) {
+    PhaseClock clk;
+    const Pattern &p = *m->pattern;
+    FEASM_HIP(hipStreamSynchronize(m->ctx->stream));  // mesh uploads may still be in flight on it
+    TileHostMesh h;
+    h.npe = m->npe;
+    h.nn = m->nnodes;
+    h.ne = m->nelem;
+    h.conn.resize(h.ne * h.npe);
+    h.xyz.resize(h.nn * 3);
+    h.n2e_ptr.resize(h.nn + 1);
+    h.n2e.resize(h.ne * h.npe);
+    h.adj_ptr.resize(h.nn + 1);
+    FEASM_HIP(hipMemcpy(h.conn.data(), m->conn->p, h.conn.size() * 4, hipMemcpyDeviceToHost));
+    FEASM_HIP(hipMemcpy(h.xyz.data(), m->xyz->p, h.xyz.size() * 8, hipMemcpyDeviceToHost));
+    FEASM_HIP(hipMemcpy(h.n2e_ptr.data(), m->n2e_ptr->p, (h.nn + 1) * 8, hipMemcpyDeviceToHost));
+    FEASM_HIP(hipMemcpy(h.n2e.data(), m->n2e->p, h.n2e.size() * 4, hipMemcpyDeviceToHost));
+    FEASM_HIP(hipMemcpy(h.adj_ptr.data(), p.adj_ptr->p, (h.nn + 1) * 8, hipMemcpyDeviceToHost));
+    h.adj.resize(h.adj_ptr[h.nn]);
+    FEASM_HIP(hipMemcpy(h.adj.data(), p.adj->p, h.adj.size() * 4, hipMemcpyDeviceToHost));
+    clk.lap("copies to host");
+    HostTiles ht;
+    build_tiles_host(h, shape, max_threads, ht);
+    clk.t = std::chrono::steady_clock::now();
+    auto T = std::make_shared<TileSet>();
+    T->shape = shape;
+    T->ntiles = ht.ntiles;
+    T->nhalo = ht.nhalo;
+    T->ntasks = ht.ntasks;
+    T->nitems = ht.nitems;
+    T->blob_bytes = ht.blob_bytes;
+    T->max_nodes = ht.max_nodes;
+    T->max_halo = ht.max_halo;
+    T->max_k = ht.max_k;
+    T->blob = dev_alloc(ht.blob_size);
+    FEASM_HIP(hipMemcpy(T->blob->p, ht.blob.get(), ht.blob_size, hipMemcpyHostToDevice));
+    T->hnodes = dev_alloc(ht.hnodes.size() * 4);
+    FEASM_HIP(hipMemcpy(T->hnodes->p, ht.hnodes.data(), ht.hnodes.size() * 4, hipMemcpyHostToDevice));
+    T->hdr = dev_alloc(std::max<size_t>(ht.hdr.size() * sizeof(TileHdr), 16));
+    FEASM_HIP(hipMemcpy(T->hdr->p, ht.hdr.data(), ht.hdr.size() * sizeof(TileHdr), hipMemcpyHostToDevice));
+    clk.lap("upload");
     return T;
 }
 

@@ -33,6 +33,8 @@ SIGNATURES = {
     "feasm_timings": (I32, [P, P, I32, ctypes.POINTER(ctypes.c_int)]),
     "feasm_timings_reset": (I32, [P]),
     "feasm_copy_calibration": (I32, [P, I64, I32, ctypes.POINTER(ctypes.c_double)]),
+    "feasm_probe_gram": (I32, [P, I32, I64, I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                               P, I64]),
     "feasm_mesh_create": (I32, [P, I32, I64, I32, P, I64, P, PP]),
     "feasm_mesh_create_device": (I32, [P, I32, I64, I32, P, I64, P, PP]),
     "feasm_mesh_destroy": (I32, [P]),

@@ -100,6 +100,18 @@ int feasm_timings_reset(feasm_ctx *ctx);
  * hardware traffic counters to bytes. */
 int feasm_copy_calibration(feasm_ctx *ctx, int64_t nbytes, int repeats, double *ms_per_copy);
 
+/* Measurement helper (no reference counterpart): throughput of the
+ * per-element node-pair Gram M_ab = sum_q u_a(q) u_b(q)^T of an 8-node,
+ * 8-point element (the product inside add_btdb_ut_only!,
+ * src/MatrixUtilityModule.jl:189-216) from element records in LDS, on the
+ * FP64 vector ALUs (variant 0) or the FP64 matrix cores
+ * (v_mfma_f64_16x16x4f64, variant 1), nelem synthetic elements (a multiple of
+ * 16) each computed reps times.  *ms the kernel time, *checksum a weighted
+ * sum of every Gram entry; grams_host (nkeep x 36 x 9, optional) the upper
+ * node-pair blocks of the first nkeep elements. */
+int feasm_probe_gram(feasm_ctx *ctx, int variant, int64_t nelem, int reps, double *ms, double *checksum,
+                     double *grams_host, int64_t nkeep);
+
 /* ---- mesh + DOF numbering (device resident) ----------------------------- */
 /* Replaces what FENodeSet/FESetH8/FESetT10 hold (src/FENodeSetModule.jl:26-32,
  * src/FESetModule.jl:57-93) and the geometry NodalField (src/NodalFieldModule.jl:
