@@ -142,6 +142,14 @@ void *ensure_stage(feasm_mesh *m, int form, int precision) {
 
 }  // namespace
 
+TileShape feasm::mesh_tile_shape(feasm_mesh *m, int nqp, int form, int precision) {
+    TileShape shape = tile_shape_for(m->ctx, m->npe, nqp, form, precision);
+    if (m->tile_nodes > 0) shape.max_nodes = std::min(255, m->tile_nodes);
+    if (m->tile_kib > 0) shape.budget = std::min(shape.budget, (size_t)m->tile_kib * 1024);
+    if (m->tile_threads == 256 || m->tile_threads == 512 || m->tile_threads == 1024) shape.threads = m->tile_threads;
+    return shape;
+}
+
 void feasm::numeric_into_ctx(feasm_ctx *ctx, feasm_mesh *m, const FormParams &fp, int rule, int order,
                              int precision, double *nzval) {
     const DevQTable &qt = device_qtable(ctx, m->etype, rule, order);
@@ -150,11 +158,7 @@ void feasm::numeric_into_ctx(feasm_ctx *ctx, feasm_mesh *m, const FormParams &fp
                   "no fused kernel for this element type and rule");
     std::shared_ptr<TileSet> tiles;
     if (fused) {
-        TileShape shape = tile_shape_for(ctx, m->npe, qt.nqp, fp.form, precision);
-        if (m->tile_nodes > 0) shape.max_nodes = std::min(255, m->tile_nodes);
-        if (m->tile_kib > 0) shape.budget = std::min(shape.budget, (size_t)m->tile_kib * 1024);
-        if (m->tile_threads == 256 || m->tile_threads == 512 || m->tile_threads == 1024)
-            shape.threads = m->tile_threads;
+        const TileShape shape = mesh_tile_shape(m, qt.nqp, fp.form, precision);
         auto it = m->tiles.find(shape.key());
         if (it == m->tiles.end()) {
             try {
@@ -575,6 +579,19 @@ extern "C" int feasm_mesh_tile_info(feasm_mesh *m, int64_t *ntiles, int64_t *nha
     }
     if (ntiles) *ntiles = t;
     if (nhalo) *nhalo = h;
+    API_CATCH(m ? m->ctx : nullptr)
+}
+
+extern "C" int feasm_tiles_compare(feasm_mesh *m, int form, int precision, int64_t *ntiles, int64_t *first_diff,
+                                   double *device_ms) {
+    API_TRY
+    FEASM_REQUIRE(m && ntiles && first_diff && device_ms, FEASM_EINVAL, "null argument");
+    FEASM_REQUIRE(m->pattern, FEASM_ESTATE, "no pattern built");
+    FEASM_REQUIRE(form >= FEASM_FORM_DIFFUSION_ISO && form <= FEASM_FORM_DOT, FEASM_EINVAL, "unknown form");
+    FEASM_REQUIRE(precision == FEASM_FP64 || precision == FEASM_MIXED, FEASM_EINVAL, "unknown precision");
+    FEASM_REQUIRE(m->npe == 8 || m->npe == 10, FEASM_EUNSUPPORTED, "no fused kernel for this element type");
+    const TileShape shape = mesh_tile_shape(m, m->npe == 8 ? 8 : 4, form, precision);
+    *first_diff = compare_tile_builders(m, shape, ntiles, device_ms);
     API_CATCH(m ? m->ctx : nullptr)
 }
 

@@ -140,6 +140,88 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int 
 // (on the mesh's context stream)
 void refresh_tile_coords(feasm_mesh *m, TileSet &t);
 
+// ---- Tile partition (shared by the host and the device builders, which
+// therefore produce the same tiles byte for byte).
+//
+// Nodes are ordered along a Z-curve (Morton code of their coordinates on a
+// lattice whose spacing is the mean node spacing; ties by node number).  A
+// box is a run [s0, s1) of that order; the whole order is the first box.  A
+// box becomes a tile when its exact sizes fit (box_fits); otherwise it is
+// split where the highest code bit that varies inside it flips (both halves
+// are then boxes of the Z-curve: box-shaped in space), or in the middle when
+// all its codes are equal.  A single node that does not fit is an error.
+
+// 21 bits -> every third bit of a 63-bit Morton code
+__host__ __device__ inline uint64_t morton_spread3(uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+// the lattice spacing for nn nodes in the bounding box [lo, hi] (host only,
+// so that both builders quantise with the same number)
+double morton_step(const double lo[3], const double hi[3], int64_t nn);
+// the Morton code of a point (identical arithmetic on host and device)
+__host__ __device__ inline uint64_t morton_code(const double *x, const double lo[3], double step) {
+    uint64_t code = 0;
+    for (int k = 0; k < 3; k++) {
+        const double t = floor((x[k] - lo[k]) / step + 0.5);
+        const double c = t < 0.0 ? 0.0 : (t > 2097151.0 ? 2097151.0 : t);
+        code |= morton_spread3((uint64_t)c) << k;
+    }
+    return code;
+}
+
+// What decides whether a box fits: owned nodes P, element incidences of the
+// owned nodes (cand), distinct halo elements H, distinct halo nodes NH, the
+// largest node -> element degree K, and the block tasks NT and items NI
+// before transposed blocks are folded (upper bounds of the tile's own).
+struct BoxSizes {
+    int64_t P = 0, cand = 0, H = 0, NH = 0, K = 0, NT = 0, NI = 0;
+};
+// most element incidences one box may gather (the device builder sorts
+// them in LDS)
+constexpr int64_t kMaxBoxCand = 4096;
+__host__ __device__ inline int box_max_nodes(const TileShape &s) { return s.max_nodes < 255 ? s.max_nodes : 255; }
+__host__ __device__ inline bool box_fits(const TileShape &s, const BoxSizes &b) {
+    if (b.P > box_max_nodes(s) || b.cand > kMaxBoxCand || b.K > 255 || b.H > s.max_halo || b.H > 255 ||
+        b.NH > 65535)
+        return false;
+    return s.layout(b.H, (int)b.P, b.NH, (int)b.K, b.NT + kPadTasks, b.NI).total <= s.budget;
+}
+// where a box that does not fit is cut: the first position whose code has
+// the highest varying bit set, or the middle when all codes are equal
+// (code: the sorted Morton codes; s1 - s0 >= 2)
+__host__ __device__ inline int64_t box_split(const uint64_t *code, int64_t s0, int64_t s1) {
+    const uint64_t x = code[s0] ^ code[s1 - 1];
+    if (!x) return s0 + (s1 - s0) / 2;
+    int b = 63;
+    while (!((x >> b) & 1)) b--;
+    int64_t lo = s0 + 1, hi = s1 - 1;  // code[s0] has the bit clear, code[s1 - 1] set
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if ((code[mid] >> b) & 1)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// the device builder (csrc/tiles_gpu.hip): the same tiles as
+// build_tiles_host, built from the mesh's device arrays on its stream
+std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shape);
+// host copies of the arrays the host builder reads
+void tile_host_mesh(feasm_mesh *m, TileHostMesh &h);
+// builds the tiles with both builders and compares headers, blobs and halo
+// node lists byte for byte: the first tile that differs, or -1
+int64_t compare_tile_builders(feasm_mesh *m, const TileShape &shape, int64_t *ntiles, double *device_ms);
+// the fused kernel's tile shape for this mesh (its tile parameters applied)
+TileShape mesh_tile_shape(feasm_mesh *m, int nqp, int form, int precision);
+
 // one mesh of a batched launch
 struct BatchMesh {
     feasm_mesh *mesh;

@@ -15,9 +15,11 @@
 // same length, and the wavefronts are dealt to the SIMDs so that each gets a
 // similar share of the work.
 //
-// Tiles are grown along the Morton order until the next node would overflow
-// the LDS budget; the cut is then moved back to the coarsest Morton boundary
-// in the second half of the tile, which keeps tiles box-shaped.
+// Tiles are boxes of the Morton order, split in two along the highest
+// varying code bit until each fits the LDS budget (fused.hpp: box_fits,
+// box_split).  This host builder and the device one (tiles_gpu.hip) follow
+// the same rules and produce the same bytes; the assembly uses the device
+// one, and this one is the reference it is checked against.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -27,6 +29,7 @@
 #include <memory>
 #include <numeric>
 #include <thread>
+#include <utility>
 
 #include "common.hpp"
 #include "fused.hpp"
@@ -35,20 +38,9 @@ namespace feasm {
 
 namespace {
 
-uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
-    v &= 0x1fffff;
-    v = (v | v << 32) & 0x1f00000000ffffull;
-    v = (v | v << 16) & 0x1f0000ff0000ffull;
-    v = (v | v << 8) & 0x100f00f00f00f00full;
-    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
-    v = (v | v << 2) & 0x1249249249249249ull;
-    return v;
-}
-
 using HostMesh = TileHostMesh;
 
-// Node order along a Z-curve of a lattice whose spacing is the mean node
-// spacing, so that runs of the order are nearly box-shaped.
+// Node order along the Z-curve (see morton_code), ties by node number.
 std::vector<std::pair<uint64_t, int32_t>> morton_order(const HostMesh &h) {
     const int64_t nn = h.nn;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -57,27 +49,9 @@ std::vector<std::pair<uint64_t, int32_t>> morton_order(const HostMesh &h) {
             lo[k] = std::min(lo[k], h.xyz[3 * n + k]);
             hi[k] = std::max(hi[k], h.xyz[3 * n + k]);
         }
-    double vol = 1, span = 0;
-    int dims = 0;
-    for (int k = 0; k < 3; k++) {
-        span = std::max(span, hi[k] - lo[k]);
-        if (hi[k] - lo[k] > 0) {
-            vol *= hi[k] - lo[k];
-            dims++;
-        }
-    }
-    double step = (dims > 0 && nn > 1) ? std::pow(vol / (double)nn, 1.0 / dims) : 1.0;
-    if (!(step > 0) || span / step > 2e6) step = (span > 0 ? span : 1.0) / 2e6;
+    const double step = morton_step(lo, hi, nn);
     std::vector<std::pair<uint64_t, int32_t>> key(nn);
-    for (int64_t n = 0; n < nn; n++) {
-        uint64_t code = 0;
-        for (int k = 0; k < 3; k++) {
-            const double t = std::floor((h.xyz[3 * n + k] - lo[k]) / step + 0.5);
-            const uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, t));
-            code |= spread3(q) << k;
-        }
-        key[n] = {code, (int32_t)n};
-    }
+    for (int64_t n = 0; n < nn; n++) key[n] = {morton_code(&h.xyz[3 * n], lo, step), (int32_t)n};
     // stable LSD radix sort over the codes' significant bits (11 bits a
     // pass): the order of std::sort on (code, node), nodes ascending within
     // equal codes
@@ -463,122 +437,57 @@ void build_tiles_host(const TileHostMesh &h, const TileShape &shape, int max_thr
     const int64_t nn = h.nn;
     const int npe = h.npe;
 
-    // ---- partition the Morton order into tiles.  The order is cut into a
-    // fixed number of equal node ranges partitioned independently (in
-    // parallel; no tile straddles a cut, and the tiles do not depend on the
-    // thread count).
-    auto level = [&](int64_t j) {  // coarseness of the Morton boundary before sorted position j
-        const uint64_t x = key[j - 1].first ^ key[j].first;
-        return x ? 64 - __builtin_clzll(x) : 0;
-    };
-    auto fits = [&](int64_t P, int64_t h2, int64_t nh2, int64_t k2, int64_t nt2, int64_t ni2) {
-        return P <= shape.max_nodes && P <= 255 && h2 <= shape.max_halo && h2 <= 255 && nh2 <= 65535 &&
-               k2 <= 255 && shape.lds_bytes(h2, (int)P, nh2, (int)k2, nt2, ni2) <= shape.budget;
-    };
-    // tile starts within [s0, s1)
-    auto partition_range = [&](int64_t s0, int64_t s1, std::vector<int64_t> &starts) {
-        std::vector<int32_t> estamp(h.ne, -1), nstamp(nn, -1);
-        std::vector<int64_t> pstamp(nn, -1);
-        int32_t epoch = 0;
-        int64_t pepoch = 0;
-        int64_t H = 0, NH = 0, K = 0, NT = 0, NI = 0;
-        auto add_node = [&](int32_t n) {
+    // ---- partition the Morton order into tiles (fused.hpp: box_fits and
+    // box_split), depth first, so that the tiles come out in Morton order
+    std::vector<uint64_t> code(nn);
+    for (int64_t i = 0; i < nn; i++) code[i] = key[i].first;
+    std::vector<int32_t> estamp(h.ne, -1), nstamp(nn, -1);
+    int32_t epoch = 0;
+    auto sizes = [&](int64_t s0, int64_t s1) {  // exact sizes of the box [s0, s1)
+        BoxSizes b;
+        b.P = s1 - s0;
+        if (b.P > box_max_nodes(shape)) return b;  // does not fit, whatever the rest
+        epoch++;
+        for (int64_t i = s0; i < s1; i++) {
+            const int32_t n = key[i].second;
             const int64_t d = h.n2e_ptr[n + 1] - h.n2e_ptr[n];
-            K = std::max(K, d);
-            NT += h.adj_ptr[n + 1] - h.adj_ptr[n];
-            NI += d * npe;
+            b.cand += d;
+            b.K = std::max(b.K, d);
+            b.NT += h.adj_ptr[n + 1] - h.adj_ptr[n];
             for (int64_t q = h.n2e_ptr[n]; q < h.n2e_ptr[n + 1]; q++) {
                 const int32_t e = (int32_t)(h.n2e[q] >> 4);
                 if (estamp[e] == epoch) continue;
                 estamp[e] = epoch;
-                H++;
+                b.H++;
                 for (int a = 0; a < npe; a++) {
                     const int32_t v = h.conn[(int64_t)e * npe + a];
                     if (nstamp[v] != epoch) {
                         nstamp[v] = epoch;
-                        NH++;
+                        b.NH++;
                     }
                 }
             }
-        };
-        // what the tile would hold with node n added (tile state left
-        // unchanged; pstamp marks the nodes this probe has counted)
-        auto probe = [&](int32_t n, int64_t &h2, int64_t &nh2, int64_t &k2) {
-            h2 = H;
-            nh2 = NH;
-            k2 = std::max(K, h.n2e_ptr[n + 1] - h.n2e_ptr[n]);
-            pepoch++;
-            for (int64_t q = h.n2e_ptr[n]; q < h.n2e_ptr[n + 1]; q++) {
-                const int32_t e = (int32_t)(h.n2e[q] >> 4);
-                if (estamp[e] == epoch) continue;
-                h2++;
-                for (int a = 0; a < npe; a++) {
-                    const int32_t v = h.conn[(int64_t)e * npe + a];
-                    if (nstamp[v] != epoch && pstamp[v] != pepoch) {
-                        pstamp[v] = pepoch;
-                        nh2++;
-                    }
-                }
-            }
-        };
-        int64_t s = s0;
-        while (s < s1) {
-            epoch++;
-            H = NH = K = NT = NI = 0;
-            int64_t i = s;
-            for (; i < s1; i++) {
-                const int32_t n = key[i].second;
-                int64_t h2, nh2, k2;
-                probe(n, h2, nh2, k2);
-                const int64_t nt2 = NT + (h.adj_ptr[n + 1] - h.adj_ptr[n]) + kPadTasks;
-                const int64_t ni2 = NI + (h.n2e_ptr[n + 1] - h.n2e_ptr[n]) * npe;
-                if (!fits(i - s + 1, h2, nh2, k2, nt2, ni2)) break;
-                add_node(n);
-            }
-            FEASM_REQUIRE(i > s, FEASM_EUNSUPPORTED, "a node has too many incident elements for the fused kernel");
-            int64_t cut = i;
-            if (i < s1 && i - s >= 4) {  // move the cut back to the coarsest boundary in the second half
-                int best = -1;
-                for (int64_t j = i; j >= s + (i - s + 1) / 2; j--) {
-                    const int l = level(j);
-                    if (l > best) {
-                        best = l;
-                        cut = j;
-                    }
-                }
-            }
-            starts.push_back(s);
-            s = cut;
         }
+        b.NI = b.cand * npe;
+        return b;
     };
-    const int nchunks = (int)std::max<int64_t>(1, std::min<int64_t>(16, nn / 65536));
-    std::vector<std::vector<int64_t>> chunk_starts(nchunks);
-    {
-        const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(std::max(1, max_threads), nchunks),
-                                                                   std::thread::hardware_concurrency()));
-        std::vector<std::thread> pool;
-        std::vector<std::string> errs(nt);
-        std::vector<int> codes(nt, 0);
-        for (int w = 0; w < nt; w++)
-            pool.emplace_back([&, w]() {
-                try {
-                    for (int c = w; c < nchunks; c += nt)
-                        partition_range(nn * c / nchunks, nn * (c + 1) / nchunks, chunk_starts[c]);
-                } catch (const Error &e) {
-                    codes[w] = e.code;
-                    errs[w] = e.what();
-                } catch (const std::exception &e) {
-                    codes[w] = FEASM_ENOMEM;
-                    errs[w] = e.what();
-                }
-            });
-        for (auto &th : pool) th.join();
-        for (int w = 0; w < nt; w++)
-            if (codes[w]) throw Error(codes[w], errs[w]);
+    std::vector<int64_t> starts;
+    if (nn > 0) {
+        std::vector<std::pair<int64_t, int64_t>> stack{{0, nn}};
+        while (!stack.empty()) {
+            const auto [s0, s1] = stack.back();
+            stack.pop_back();
+            if (box_fits(shape, sizes(s0, s1))) {
+                starts.push_back(s0);
+                continue;
+            }
+            FEASM_REQUIRE(s1 - s0 > 1, FEASM_EUNSUPPORTED, "a node has too many incident elements for the fused kernel");
+            const int64_t mid = box_split(code.data(), s0, s1);
+            stack.push_back({mid, s1});
+            stack.push_back({s0, mid});
+        }
     }
     clk.lap("partition");
-    std::vector<int64_t> starts;
-    for (const auto &cs : chunk_starts) starts.insert(starts.end(), cs.begin(), cs.end());
     starts.push_back(nn);
     const int64_t ntiles = (int64_t)starts.size() - 1;
 
@@ -684,11 +593,24 @@ void build_tiles_host(const TileHostMesh &h, const TileShape &shape, int max_thr
     clk.lap("blobs");
 }
 
-std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int max_threads) {
-    PhaseClock clk;
+double morton_step(const double lo[3], const double hi[3], int64_t nn) {
+    double vol = 1, span = 0;
+    int dims = 0;
+    for (int k = 0; k < 3; k++) {
+        span = std::max(span, hi[k] - lo[k]);
+        if (hi[k] - lo[k] > 0) {
+            vol *= hi[k] - lo[k];
+            dims++;
+        }
+    }
+    double step = (dims > 0 && nn > 1) ? std::pow(vol / (double)nn, 1.0 / dims) : 1.0;
+    if (!(step > 0) || span / step > 2e6) step = (span > 0 ? span : 1.0) / 2e6;
+    return step;
+}
+
+void tile_host_mesh(feasm_mesh *m, TileHostMesh &h) {
     const Pattern &p = *m->pattern;
     FEASM_HIP(hipStreamSynchronize(m->ctx->stream));  // mesh uploads may still be in flight on it
-    TileHostMesh h;
     h.npe = m->npe;
     h.nn = m->nnodes;
     h.ne = m->nelem;
@@ -704,28 +626,47 @@ std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int 
     FEASM_HIP(hipMemcpy(h.adj_ptr.data(), p.adj_ptr->p, (h.nn + 1) * 8, hipMemcpyDeviceToHost));
     h.adj.resize(h.adj_ptr[h.nn]);
     FEASM_HIP(hipMemcpy(h.adj.data(), p.adj->p, h.adj.size() * 4, hipMemcpyDeviceToHost));
-    clk.lap("copies to host");
+}
+
+int64_t compare_tile_builders(feasm_mesh *m, const TileShape &shape, int64_t *ntiles, double *device_ms) {
+    FEASM_HIP(hipStreamSynchronize(m->ctx->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    const std::shared_ptr<TileSet> D = build_tiles_device(m, shape);
+    FEASM_HIP(hipStreamSynchronize(m->ctx->stream));
+    *device_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *ntiles = D->ntiles;
+    TileHostMesh h;
+    tile_host_mesh(m, h);
     HostTiles ht;
-    build_tiles_host(h, shape, max_threads, ht);
-    clk.t = std::chrono::steady_clock::now();
-    auto T = std::make_shared<TileSet>();
-    T->shape = shape;
-    T->ntiles = ht.ntiles;
-    T->nhalo = ht.nhalo;
-    T->ntasks = ht.ntasks;
-    T->nitems = ht.nitems;
-    T->blob_bytes = ht.blob_bytes;
-    T->max_nodes = ht.max_nodes;
-    T->max_halo = ht.max_halo;
-    T->max_k = ht.max_k;
-    T->blob = dev_alloc(ht.blob_size);
-    FEASM_HIP(hipMemcpy(T->blob->p, ht.blob.get(), ht.blob_size, hipMemcpyHostToDevice));
-    T->hnodes = dev_alloc(ht.hnodes.size() * 4);
-    FEASM_HIP(hipMemcpy(T->hnodes->p, ht.hnodes.data(), ht.hnodes.size() * 4, hipMemcpyHostToDevice));
-    T->hdr = dev_alloc(std::max<size_t>(ht.hdr.size() * sizeof(TileHdr), 16));
-    FEASM_HIP(hipMemcpy(T->hdr->p, ht.hdr.data(), ht.hdr.size() * sizeof(TileHdr), hipMemcpyHostToDevice));
-    clk.lap("upload");
-    return T;
+    build_tiles_host(h, shape, 16, ht);
+    std::vector<TileHdr> dh(D->ntiles);
+    std::vector<unsigned char> db(D->blob_bytes);
+    std::vector<int32_t> dn(ht.hnodes.size());
+    if (D->ntiles) FEASM_HIP(hipMemcpy(dh.data(), D->hdr->p, dh.size() * sizeof(TileHdr), hipMemcpyDeviceToHost));
+    if (D->blob_bytes) FEASM_HIP(hipMemcpy(db.data(), D->blob->p, db.size(), hipMemcpyDeviceToHost));
+    const int64_t nt = std::min(D->ntiles, ht.ntiles);
+    const size_t nx = (size_t)std::min<int64_t>((int64_t)dn.size(), (int64_t)(D->hnodes->bytes / 4));
+    if (nx) FEASM_HIP(hipMemcpy(dn.data(), D->hnodes->p, nx * 4, hipMemcpyDeviceToHost));
+    for (int64_t t = 0; t < nt; t++) {
+        const TileHdr &a = dh[t], &b = ht.hdr[t];
+        if (std::memcmp(&a, &b, sizeof(TileHdr)) != 0) return t;
+        if (a.blob + a.nbytes > D->blob_bytes ||
+            std::memcmp(db.data() + a.blob, ht.blob.get() + b.blob, (size_t)a.nbytes) != 0)
+            return t;
+        if ((size_t)a.x0 + a.NH > nx ||
+            std::memcmp(dn.data() + a.x0, ht.hnodes.data() + b.x0, (size_t)a.NH * 4) != 0)
+            return t;
+    }
+    if (D->ntiles != ht.ntiles) return nt;
+    if (D->nhalo != ht.nhalo || D->ntasks != ht.ntasks || D->nitems != ht.nitems || D->blob_bytes != ht.blob_bytes ||
+        D->max_nodes != ht.max_nodes || D->max_halo != ht.max_halo || D->max_k != ht.max_k)
+        return std::max<int64_t>(nt - 1, 0);
+    return -1;
+}
+
+std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int max_threads) {
+    (void)max_threads;
+    return build_tiles_device(m, shape);
 }
 
 }  // namespace feasm

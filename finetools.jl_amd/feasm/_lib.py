@@ -52,6 +52,7 @@ SIGNATURES = {
     "feasm_pattern_build": (I32, [P]),
     "feasm_mesh_set_numeric_path": (I32, [P, I32]),
     "feasm_mesh_tile_info": (I32, [P, PI64, PI64]),
+    "feasm_tiles_compare": (I32, [P, I32, I32, PI64, PI64, ctypes.POINTER(DBL)]),
     "feasm_mesh_set_tile_params": (I32, [P, I32, I32, I32]),
     "feasm_pattern_info": (I32, [P, PI64, PI64]),
     "feasm_matrix_dims": (I32, [P, PI64, PI64, PI64]),

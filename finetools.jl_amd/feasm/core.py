@@ -193,6 +193,15 @@ class DeviceMesh:
     def build_pattern(self):
         check(lib().feasm_pattern_build(self.h), self.ctx.h)
 
+    def tiles_compare(self, form, precision=L.FP64):
+        """Builds the fused kernel's tiles with the device and the host
+        builders and compares them byte for byte: (ntiles, first differing
+        tile or -1, device build ms)."""
+        nt, diff, ms = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+        check(lib().feasm_tiles_compare(self.h, form, precision, ctypes.byref(nt), ctypes.byref(diff),
+                                        ctypes.byref(ms)), self.ctx.h)
+        return nt.value, diff.value, ms.value
+
     def bilform(self, form, rule, order, coeff, precision=L.FP64, path="auto"):
         c = np.ascontiguousarray(np.atleast_1d(coeff), dtype=np.float64).ravel()
         h = ctypes.c_void_p()

@@ -167,6 +167,15 @@ int feasm_mesh_set_numeric_path(feasm_mesh *mesh, int path);
 int feasm_mesh_set_tile_params(feasm_mesh *mesh, int max_nodes, int budget_kib, int threads);
 /* Tiles built for the fused kernel so far (summed over forms/precisions). */
 int feasm_mesh_tile_info(feasm_mesh *mesh, int64_t *ntiles, int64_t *nhalo);
+/* Tile builders, for testing (no reference counterpart): builds the fused
+ * kernel's tiles for this mesh, form and precision (with the mesh's tile
+ * parameters) with the device builder the assembly uses and with the host
+ * builder, and compares headers, blobs and halo-node lists byte for byte.
+ * *ntiles: tiles of the device build; *first_diff: the first tile that
+ * differs, -1 when none; *device_ms: the device build's wall time.  Needs
+ * the mesh's pattern (feasm_pattern_build). */
+int feasm_tiles_compare(feasm_mesh *mesh, int form, int precision, int64_t *ntiles, int64_t *first_diff,
+                        double *device_ms);
 /* Build (or rebuild) the device CSC pattern for the mesh's DOFs now. */
 int feasm_pattern_build(feasm_mesh *mesh);
 /* Pattern statistics: nnz of the DOF-level CSC and number of node blocks. */
