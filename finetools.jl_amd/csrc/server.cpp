@@ -68,11 +68,16 @@ struct Request {
     std::unique_ptr<feasm_matrix> result;
 };
 
-// Device bytes a request may hold: pattern and values always, plus the staged
-// element blocks and contribution lists (every mesh's first assemblies) or,
-// for a resident mesh re-assembled often enough, the fused kernel's tiles.
+// A mesh's first kTileAfter assemblies take the staged path: building the
+// fused kernel's tiles (about 10-20 ms per million H8 elements on the device)
+// costs more than the staged path saves on one assembly (about 2.4 ms per
+// million), so tiles pay off only for meshes re-assembled several times.
 constexpr int kTileAfter = 4;
 
+// Device bytes a request may hold: pattern and values always, plus the staged
+// element blocks and contribution lists or, for a resident mesh, whichever
+// is larger of those and the device tile builder's peak (slots, packed blobs
+// and its per-node work arrays).
 size_t estimate_bytes(int npe, int ndn, int64_t nelem, int64_t nnodes, bool keep, int precision) {
     const double deg = (npe == 8) ? 27.0 : 26.0;  // upper bound of the mean node degree of block meshes
     const double nb = deg * (double)nnodes, nnz = nb * ndn * ndn;
@@ -84,7 +89,7 @@ size_t estimate_bytes(int npe, int ndn, int64_t nelem, int64_t nnodes, bool keep
                + 12.0 * nnz           // rowval + nzval
                + 8.0 * nnodes * ndn;  // colptr
     const double staged = npairs * 9.0 * rb * nelem + 4.0 * npe * npe * nelem + 8.0 * nb;  // stage, contributions
-    b += keep ? std::max(staged, 600.0 * nnodes) : staged;
+    b += keep ? std::max(staged, 1400.0 * nnodes) : staged;
     return (size_t)(1.2 * b) + (8u << 20);
 }
 
@@ -219,11 +224,10 @@ struct feasm_server {
             M->pattern = m->pattern;
             M->form_class = m->ndn;
             M->nzval = dev_alloc(std::max<size_t>(M->nnz * 8, 8));
-            // The fused kernel needs tiles built on the host, which pays off
-            // only over repeated assemblies of one mesh: a mesh's first
-            // kTileAfter assemblies take the staged path (all on the device);
-            // later ones build the tiles once and drop the staged buffers --
-            // when the element type and rule have a fused kernel at all.
+            // staged for a mesh's first kTileAfter assemblies (all on the
+            // device); later ones build the tiles once (on the device) and
+            // drop the staged buffers -- when the element type and rule have a
+            // fused kernel at all
             const bool fusable = fused_supported(m->npe, device_qtable(ctx, m->etype, r->rule, r->order).nqp);
             m->numeric_path = (m->assemblies < kTileAfter || !fusable) ? 1 : 0;
             if (m->assemblies == kTileAfter && fusable) {
@@ -288,7 +292,9 @@ extern "C" int feasm_server_create(int device, int workers, double hbm_fraction,
     FEASM_HIP(hipSetDevice(device));
     size_t fr = 0, tot = 0;
     FEASM_HIP(hipMemGetInfo(&fr, &tot));
-    s->adm.budget = (size_t)(hbm_fraction * (double)fr);
+    // blocks the library's memory pool holds for reuse are free to the
+    // server's requests too
+    s->adm.budget = (size_t)(hbm_fraction * (double)(fr + pool_cached_bytes(device)));
     for (int w = 0; w < workers; w++) {
         feasm_ctx *c = nullptr;
         const int rc = feasm_ctx_create(device, &c);

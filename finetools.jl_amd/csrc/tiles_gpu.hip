@@ -122,8 +122,95 @@ __device__ uint32_t block_excl_scan(uint32_t *a, int n, uint32_t *part) {
     return total;
 }
 
-// ascending bitonic sort of a[0, n), n a power of two >= 2
+// Ascending bitonic sort of a[0, n), n a power of two, 2 <= n <= 4096.
+// Up to 1024 entries the entries stay in registers: thread t holds the entries t + kTB m
+// (m < E = n / kTB; n below kTB is padded with 0xffffffff to kTB).  A stage
+// whose partners are 256 or more apart pairs two registers of one thread,
+// one whose partners are less than 64 apart pairs the same register of two
+// lanes of one wavefront (a lane shuffle), and the stages in between go
+// through LDS with workgroup barriers.
+template <int E, int JJ>
+__device__ __forceinline__ void sort_reg_stage(uint32_t (&x)[E], int t, int k) {
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        const int mp = m ^ JJ;
+        if (mp > m && mp < E) {
+            const bool up = ((t + kTB * m) & k) == 0;
+            const uint32_t lo = min(x[m], x[mp]), hi = max(x[m], x[mp]);
+            x[m] = up ? lo : hi;
+            x[mp] = up ? hi : lo;
+        }
+    }
+}
+
+template <int E>
+__device__ void block_sort_e(uint32_t *a, int n) {
+    const int t = threadIdx.x;
+    uint32_t x[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) x[m] = a[t + kTB * m];
+    for (int k = 2; k <= n; k <<= 1) {
+        int j = k >> 1;
+        for (; j >= kTB; j >>= 1) {
+            const int jj = j / kTB;
+            if (jj == 1) sort_reg_stage<E, 1>(x, t, k);
+            if (jj == 2) sort_reg_stage<E, 2>(x, t, k);
+            if (jj == 4) sort_reg_stage<E, 4>(x, t, k);
+            if (jj == 8) sort_reg_stage<E, 8>(x, t, k);
+        }
+        if (j >= 64) {
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < E; m++) a[t + kTB * m] = x[m];
+            __syncthreads();
+            for (; j >= 64; j >>= 1) {
+#pragma unroll
+                for (int m = 0; m < E; m++) {
+                    const int i = t + kTB * m, ixj = i ^ j;
+                    if (ixj > i) {
+                        const uint32_t p = a[i], q = a[ixj];
+                        if ((p > q) == ((i & k) == 0)) {
+                            a[i] = q;
+                            a[ixj] = p;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int m = 0; m < E; m++) x[m] = a[t + kTB * m];
+        }
+        for (; j > 0; j >>= 1) {
+#pragma unroll
+            for (int m = 0; m < E; m++) {
+                const int i = t + kTB * m;
+                const uint32_t y = (uint32_t)__shfl_xor((int)x[m], j, 64);
+                const bool take_min = ((i & k) == 0) == ((i & j) == 0);
+                x[m] = take_min ? min(x[m], y) : max(x[m], y);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; m++) a[t + kTB * m] = x[m];
+    __syncthreads();
+}
+
+// a[0, n): n a power of two >= 2 and <= 4096; entries [n, kTB) of a are
+// overwritten when n < kTB (the buffers hold at least kTB entries)
 __device__ void block_sort(uint32_t *a, int n) {
+    if (n < kTB) {
+        for (int i = n + threadIdx.x; i < kTB; i += kTB) a[i] = 0xffffffffu;
+        __syncthreads();
+        n = kTB;
+    }
+    switch (n / kTB) {
+        case 1: block_sort_e<1>(a, n); return;
+        case 2: block_sort_e<2>(a, n); return;
+        case 4: block_sort_e<4>(a, n); return;
+        default: break;
+    }
+    // larger sorts stay in LDS (more entries per thread would cost registers)
     for (int k = 2; k <= n; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int i = threadIdx.x; i < n; i += kTB) {
@@ -254,6 +341,28 @@ __global__ void k_bbox(const double *xyz, int64_t nn, double *part) {
     }
 }
 
+// the nb partial boxes of k_bbox reduced to one (one block of 256 threads)
+__global__ void k_bbox_final(const double *part, int nb, double *out) {
+    __shared__ double s[6][256];
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = part[6 * b + k] < lo[k] ? part[6 * b + k] : lo[k];
+            hi[k] = part[6 * b + 3 + k] > hi[k] ? part[6 * b + 3 + k] : hi[k];
+        }
+    for (int k = 0; k < 3; k++) {
+        s[k][threadIdx.x] = lo[k];
+        s[3 + k][threadIdx.x] = hi[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        double r = s[k][0];
+        for (int i = 1; i < 256; i++) r = k < 3 ? (s[k][i] < r ? s[k][i] : r) : (s[k][i] > r ? s[k][i] : r);
+        out[k] = r;
+    }
+}
+
 __global__ void k_morton(const double *xyz, int64_t nn, double lo0, double lo1, double lo2, double step, uint64_t *code,
                          int32_t *id) {
     const double lo[3] = {lo0, lo1, lo2};
@@ -263,47 +372,51 @@ __global__ void k_morton(const double *xyz, int64_t nn, double lo0, double lo1, 
     }
 }
 
-__global__ void k_inverse(const int32_t *order, int64_t nn, int32_t *inv) {
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nn; p += (int64_t)gridDim.x * blockDim.x)
-        inv[order[p]] = (int32_t)p;
-}
-
 // ---- 2. the box tree, one level
 
 // info per tile start: H, NH, K, NT, NI (the sizes box_fits saw)
 constexpr int kInfo = 5;
 
-__global__ __launch_bounds__(kTB) void k_part_level(const int64_t *in, int64_t *out, unsigned *nout, unsigned cap,
-                                                    const uint64_t *code, const int32_t *order, MeshView mv,
-                                                    TileShape shape, uint8_t *start, int64_t *info, unsigned *err) {
+// level l of the tree: the boxes in[0, count[l]) are split or marked as
+// tiles; the halves go to out[0, count[l + 1]).  The grid strides over the
+// level's boxes, so the host launches levels without knowing their sizes.
+__global__ __launch_bounds__(kTB) void k_part_level(const int64_t *in, int64_t *out, unsigned *count, int l,
+                                                    unsigned cap, const uint64_t *code, const int32_t *order,
+                                                    MeshView mv, TileShape shape, uint8_t *start, int64_t *info,
+                                                    unsigned *err) {
     __shared__ EvalLds L;
-    const int64_t s0 = in[2 * (int64_t)blockIdx.x], s1 = in[2 * (int64_t)blockIdx.x + 1];
-    const int64_t P = s1 - s0;
-    bool fits = false;
-    BoxSizes bs;
-    if (P <= box_max_nodes(shape)) fits = eval_box(mv, order, s0, (int)P, shape, L, bs);
-    if (threadIdx.x != 0) return;
-    if (fits) {
-        start[s0] = 1;
-        int64_t *f = info + kInfo * s0;
-        f[0] = bs.H;
-        f[1] = bs.NH;
-        f[2] = bs.K;
-        f[3] = bs.NT;
-        f[4] = bs.NI;
-    } else if (P == 1) {
-        atomicOr(err, kErrNodeTooBig);
-    } else {
-        const int64_t mid = box_split(code, s0, s1);
-        const unsigned slot = atomicAdd(nout, 2u);
-        if (slot + 2 > cap) {
-            atomicOr(err, kErrQueue);
-            return;
+    const unsigned nin = count[l];
+    for (unsigned bx = blockIdx.x; bx < nin; bx += gridDim.x) {
+        const int64_t s0 = in[2 * (int64_t)bx], s1 = in[2 * (int64_t)bx + 1];
+        const int64_t P = s1 - s0;
+        bool fits = false;
+        BoxSizes bs;
+        if (P <= box_max_nodes(shape)) fits = eval_box(mv, order, s0, (int)P, shape, L, bs);
+        if (threadIdx.x == 0) {
+            if (fits) {
+                start[s0] = 1;
+                int64_t *f = info + kInfo * s0;
+                f[0] = bs.H;
+                f[1] = bs.NH;
+                f[2] = bs.K;
+                f[3] = bs.NT;
+                f[4] = bs.NI;
+            } else if (P == 1) {
+                atomicOr(err, kErrNodeTooBig);
+            } else {
+                const int64_t mid = box_split(code, s0, s1);
+                const unsigned slot = atomicAdd(&count[l + 1], 2u);
+                if (slot + 2 > cap) {
+                    atomicOr(err, kErrQueue);
+                } else {
+                    out[2 * (int64_t)slot] = s0;
+                    out[2 * (int64_t)slot + 1] = mid;
+                    out[2 * (int64_t)slot + 2] = mid;
+                    out[2 * (int64_t)slot + 3] = s1;
+                }
+            }
         }
-        out[2 * (int64_t)slot] = s0;
-        out[2 * (int64_t)slot + 1] = mid;
-        out[2 * (int64_t)slot + 2] = mid;
-        out[2 * (int64_t)slot + 3] = s1;
+        __syncthreads();  // LDS is reused by the next box
     }
 }
 
@@ -321,8 +434,32 @@ struct PrepStats {
     unsigned long long ntu_max, niu_max, k_max;
 };
 
-__global__ void k_tile_prep(const int64_t *ts, int64_t ntiles, const int64_t *info, TileShape shape, int64_t *nh,
-                            int64_t *slot, PrepStats *st) {
+// largest of v over the workgroup (256 threads), added to *out with one atomic
+__device__ void block_max_to(unsigned long long v, unsigned long long *out, unsigned long long *sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 128; d > 0; d >>= 1) {
+        if ((int)threadIdx.x < d) sh[threadIdx.x] = max(sh[threadIdx.x], sh[threadIdx.x + d]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, sh[0]);
+    __syncthreads();
+}
+__device__ void block_sum_to(unsigned long long v, unsigned long long *out, unsigned long long *sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 128; d > 0; d >>= 1) {
+        if ((int)threadIdx.x < d) sh[threadIdx.x] += sh[threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, sh[0]);
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_tile_prep(const int64_t *ts, int64_t ntiles, const int64_t *info,
+                                                   TileShape shape, int64_t *nh, int64_t *slot, PrepStats *st) {
+    __shared__ unsigned long long sh[256];
+    unsigned long long ntu = 0, niu = 0, km = 0;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t <= ntiles; t += (int64_t)gridDim.x * blockDim.x) {
         if (t == ntiles) {
             nh[t] = 0;
@@ -335,27 +472,35 @@ __global__ void k_tile_prep(const int64_t *ts, int64_t ntiles, const int64_t *in
         const TileShape::Layout L = shape.layout(f[0], P, f[1], (int)f[2], f[3] + kPadTasks, f[4]);
         nh[t] = f[1];
         slot[t] = (int64_t)(L.U - L.cbase);
-        atomicMax(&st->ntu_max, (unsigned long long)f[3]);
-        atomicMax(&st->niu_max, (unsigned long long)f[4]);
-        atomicMax(&st->k_max, (unsigned long long)f[2]);
+        ntu = max(ntu, (unsigned long long)f[3]);
+        niu = max(niu, (unsigned long long)f[4]);
+        km = max(km, (unsigned long long)f[2]);
     }
+    block_max_to(ntu, &st->ntu_max, sh);
+    block_max_to(niu, &st->niu_max, sh);
+    block_max_to(km, &st->k_max, sh);
 }
 
 // ---- 3. one tile
 
 // capacities of the build kernel's dynamic LDS (maxima over the tiles)
 struct BuildCaps {
-    int ntu;    // unfolded block tasks
-    int niu;    // unfolded items
-    int nsets;  // bank-spread read sets
-    int spread; // H8 FP64 bank-spread numbering
+    int ntu;     // unfolded block tasks
+    int niu;     // unfolded items
+    int nsets;   // bank-spread read sets (< 4096: the read entries are 16-bit)
+    int spread;  // H8 FP64 bank-spread numbering
 };
 
 constexpr int kSlotsExtra = kPadTasks + 256;  // schedule slots beyond the tasks
 
+// Dynamic LDS of the build kernel.  The task arrays (word .. rows) are dead
+// once the records and items are written: the bank spread's load counters
+// reuse them, and its read entries reuse the evaluation's sort buffer when
+// they fit there.
 struct BuildLds {
-    size_t word, tpair, sorted, bcount, cq, qtab, ibase, lconn, items, sinfo, gk, occcnt, occfill, occ, load, perm,
-        order, slotl, total;
+    size_t word, tpair, sorted, rows, load, bcount, cq, csum, qtab, ibase, lconn, ownc, items, sinfo, gk, occcnt,
+        occfill, occ, perm, order, slotl, total;
+    bool occ_in_buf;
 };
 
 __host__ __device__ inline size_t bl_take(size_t &o, size_t bytes) {
@@ -367,49 +512,58 @@ __host__ __device__ inline size_t bl_take(size_t &o, size_t bytes) {
 __host__ __device__ inline BuildLds build_lds(const BuildCaps &c, int npe) {
     BuildLds B;
     size_t o = 0;
-    const size_t ns = (size_t)c.ntu + kSlotsExtra;      // schedule slots
-    const size_t nch = (size_t)c.ntu / 64 + 2;           // chunks of 64 tasks
-    const size_t nsg = nch + 8;                          // slot groups
+    const size_t ns = (size_t)c.ntu + kSlotsExtra;  // schedule slots
+    const size_t nch = (size_t)c.ntu / 64 + 2;       // chunks of 64 tasks
+    const size_t nsg = nch + 8;                      // slot groups
     B.word = bl_take(o, (size_t)c.ntu * 4);
     B.tpair = bl_take(o, (size_t)c.ntu * 2);
     B.sorted = bl_take(o, (size_t)c.ntu * 2);
+    B.rows = bl_take(o, (size_t)c.ntu * 2);
+    B.load = B.word;
+    if (c.spread && (size_t)c.nsets * 16 > o) o = align16((size_t)c.nsets * 16);
     B.bcount = bl_take(o, 513 * 4);
     B.cq = bl_take(o, nch * 4);
+    B.csum = bl_take(o, nch * 4);
     B.qtab = bl_take(o, nsg * 4);
     B.ibase = bl_take(o, (nsg + 1) * 4);
     B.lconn = bl_take(o, (size_t)255 * npe * 2);
+    B.ownc = bl_take(o, (size_t)255 * npe * 2);
+    B.occ_in_buf = (size_t)c.niu * 2 * 2 <= (size_t)kMaxBoxCand * 4;
     if (c.spread) {
         B.items = bl_take(o, (size_t)c.niu * 2);
         B.sinfo = bl_take(o, ns * 4);
         B.gk = bl_take(o, (ns / 16 + 2) * 4);
         B.occcnt = bl_take(o, 257 * 4);
         B.occfill = bl_take(o, 257 * 4);
-        B.occ = bl_take(o, (size_t)c.niu * 2 * 4);
-        B.load = bl_take(o, (size_t)c.nsets * 16);
+        B.occ = B.occ_in_buf ? 0 : bl_take(o, (size_t)c.niu * 2 * 2);
         B.perm = bl_take(o, 256 * 4);
         B.order = bl_take(o, 256 * 4);
         B.slotl = bl_take(o, (256 + 32) * 4);
     } else {
-        B.items = B.sinfo = B.gk = B.occcnt = B.occfill = B.occ = B.load = B.perm = B.order = B.slotl = 0;
+        B.items = B.sinfo = B.gk = B.occcnt = B.occfill = B.occ = B.perm = B.order = B.slotl = 0;
     }
     B.total = o;
     return B;
 }
 
-// the elements holding both nodes v (column) and m (row), in ascending
-// element then local index of v, then local index of m; emit(e, ac, am, j)
-// for the j-th; returns their number
+// The tile's node incidence lists in LDS: keys[nptr[j] .. nptr[j + 1]) are
+// the (halo element h, local index a) of halo node j as h << 4 | a in the low
+// 16 bits, ascending (j << 16 in the high bits).  merge_halo walks the
+// elements holding both halo nodes ja (column) and jb (row) in ascending
+// element, then local index of ja, then of jb -- for the owned column node
+// every element is a halo element, so these are all the elements the two
+// nodes share -- calling emit(h, ac, am, j) for the j-th; returns their
+// number.
 template <class F>
-__device__ int merge_elements(const MeshView &mv, int32_t v, int32_t m, F &&emit) {
-    const int64_t a1 = mv.n2e_ptr[v + 1], b1 = mv.n2e_ptr[m + 1];
-    int64_t j = mv.n2e_ptr[m];
-    int k = 0;
-    for (int64_t i = mv.n2e_ptr[v]; i < a1; i++) {
-        const uint32_t ce = mv.n2e[i];
+__device__ int merge_halo(const uint32_t *keys, const uint16_t *nptr, int ja, int jb, F &&emit) {
+    const int a1 = nptr[ja + 1], b1 = nptr[jb + 1];
+    int j = nptr[jb], k = 0;
+    for (int i = nptr[ja]; i < a1; i++) {
+        const uint32_t ce = keys[i] & 0xffffu;
         const uint32_t e = ce >> 4;
-        while (j < b1 && (mv.n2e[j] >> 4) < e) j++;
-        for (int64_t q = j; q < b1 && (mv.n2e[q] >> 4) == e; q++) {
-            emit(e, ce & 15u, mv.n2e[q] & 15u, k);
+        while (j < b1 && ((keys[j] & 0xffffu) >> 4) < e) j++;
+        for (int q = j; q < b1 && ((keys[q] & 0xffffu) >> 4) == e; q++) {
+            emit(e, ce & 15u, keys[q] & 15u, k);
             k++;
         }
     }
@@ -424,12 +578,49 @@ __device__ __forceinline__ int task_bucket(uint32_t w, int K) {
 
 constexpr uint32_t kDropBit = 1u << 27;  // builder-internal: computed by the transposed block's task
 
+// the owned column whose node block list starts at or before task u
+__device__ __forceinline__ int column_of(const uint32_t *coff, int P, int u) {
+    int lo = 0, hi = P;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (coff[mid] <= (uint32_t)u)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// the 16 entries of a quarter-wavefront's read row (32-byte aligned), as 8 words
+__device__ __forceinline__ void load_row16(const uint16_t *row, uint32_t (&w)[8]) {
+    const uint4 a = reinterpret_cast<const uint4 *>(row)[0], b = reinterpret_cast<const uint4 *>(row)[1];
+    w[0] = a.x;
+    w[1] = a.y;
+    w[2] = a.z;
+    w[3] = a.w;
+    w[4] = b.x;
+    w[5] = b.y;
+    w[6] = b.z;
+    w[7] = b.w;
+}
+// no entry x < n of the row equals r
+__device__ __forceinline__ bool first_in_row(const uint32_t (&w)[8], int n, uint32_t r) {
+    bool f = true;
+#pragma unroll
+    for (int x = 0; x < 16; x++) {
+        const uint32_t v = (w[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+        f = f && !(x < n && v == r);
+    }
+    return f;
+}
+
 __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *xyz, const int32_t *order,
-                                                    const int32_t *inv, const int32_t *inc, const int64_t *ts,
-                                                    const int64_t *info, const int64_t *x0, const int64_t *slot_off,
-                                                    TileShape shape, BuildCaps caps, unsigned char *slots,
-                                                    int32_t *hnodes, int32_t *tout, unsigned *err) {
-    __shared__ EvalLds L;
+                                                    const int64_t *ts, const int64_t *info, const int64_t *x0,
+                                                    const int64_t *slot_off, TileShape shape, BuildCaps caps,
+                                                    unsigned char *slots, int32_t *hnodes, int32_t *tout,
+                                                    unsigned *err, unsigned long long *tim) {
+    __shared__ __align__(16) EvalLds L;
+    __shared__ uint16_t ownj[256];  // halo-node index of each owned column (0xffff: no element)
     __shared__ int sh_fail, sh_K, sh_nch, sh_nsg, sh_NT;
     __shared__ uint32_t sh_NI;
     extern __shared__ __align__(16) unsigned char dyn[];
@@ -437,18 +628,30 @@ __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *x
     uint32_t *word = reinterpret_cast<uint32_t *>(dyn + B.word);
     uint16_t *tpair = reinterpret_cast<uint16_t *>(dyn + B.tpair);
     uint16_t *sorted = reinterpret_cast<uint16_t *>(dyn + B.sorted);
+    uint16_t *rows = reinterpret_cast<uint16_t *>(dyn + B.rows);
     uint32_t *bcount = reinterpret_cast<uint32_t *>(dyn + B.bcount);
     int32_t *cq = reinterpret_cast<int32_t *>(dyn + B.cq);
+    uint32_t *csum = reinterpret_cast<uint32_t *>(dyn + B.csum);
     int32_t *qtab = reinterpret_cast<int32_t *>(dyn + B.qtab);
     uint32_t *ibase = reinterpret_cast<uint32_t *>(dyn + B.ibase);
     uint16_t *lconn = reinterpret_cast<uint16_t *>(dyn + B.lconn);
+    int16_t *ownc = reinterpret_cast<int16_t *>(dyn + B.ownc);  // owned column of each halo node, or -1
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // phase clock (tim != nullptr): thread 0 adds each phase's wall-clock
+    // ticks to tim[phase]
+    long long t_mark = tim ? wall_clock64() : 0;
+    auto mark = [&](int k) {
+        if (tim && tid == 0) {
+            const long long now = wall_clock64();
+            atomicAdd(&tim[k], (unsigned long long)(now - t_mark));
+            t_mark = now;
+        }
+    };
     const int npe = mv.npe;
     const int64_t t = blockIdx.x;
     const int64_t s0 = ts[t];
     const int P = (int)(ts[t + 1] - s0);
-    const int32_t tile = (int32_t)t;
 
     // halo lists (the same evaluation the partition made)
     BoxSizes bs;
@@ -459,18 +662,40 @@ __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *x
         if (tid == 0) atomicOr(err, kErrSizes);
         return;
     }
+    mark(0);
     const int H = (int)bs.H, NH = (int)bs.NH, NTu = (int)bs.NT;
-    for (int i = tid; i < NH; i += kTB) hnodes[x0[t] + i] = L.hn[i];
-    for (int i = tid; i < H * npe; i += kTB) {
+    const int nc = H * npe;
+    for (int i = tid; i < NH; i += kTB) {
+        hnodes[x0[t] + i] = L.hn[i];
+        ownc[i] = -1;
+    }
+    for (int i = tid; i < nc; i += kTB) {
         const int32_t v = mv.conn[(int64_t)L.halo[i / npe] * npe + i % npe];
         lconn[i] = (uint16_t)lower_bound_i32(L.hn, NH, v);
     }
-    // columns: coff becomes the first task of each owned column
+    if (tid < P) {
+        const int j = lower_bound_i32(L.hn, NH, L.owned[tid]);
+        ownj[tid] = (j < NH && L.hn[j] == L.owned[tid]) ? (uint16_t)j : (uint16_t)0xffff;
+    }
     if (tid == 0) {
         sh_fail = 0;
         sh_K = 0;
     }
     __syncthreads();
+    // node incidence lists (merge_halo)
+    uint32_t *keys = L.buf;
+    for (int i = tid; i < nc; i += kTB) keys[i] = ((uint32_t)lconn[i] << 16) | ((uint32_t)(i / npe) << 4) | (uint32_t)(i % npe);
+    int n2 = 2;
+    while (n2 < nc) n2 <<= 1;
+    for (int i = nc + tid; i < n2; i += kTB) keys[i] = 0xffffffffu;
+    if (tid < P && ownj[tid] != 0xffff) ownc[ownj[tid]] = (int16_t)tid;
+    __syncthreads();
+    block_sort(keys, n2);
+    uint16_t *nptr = reinterpret_cast<uint16_t *>(keys + nc);  // [NH + 1], after the sorted keys
+    for (int i = tid; i < nc; i += kTB)
+        if (i == 0 || (keys[i] >> 16) != (keys[i - 1] >> 16)) nptr[keys[i] >> 16] = (uint16_t)i;
+    if (tid == 0) nptr[NH] = (uint16_t)nc;
+    // columns: coff becomes the first task of each owned column
     if (tid < P) {
         const int32_t v = L.owned[tid];
         const int64_t d = mv.adj_ptr[v + 1] - mv.adj_ptr[v];
@@ -483,36 +708,48 @@ __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *x
     if (tid == P) L.coff[P] = 0;
     __syncthreads();
     if (sh_fail) return;
+    mark(1);
     block_excl_scan(L.coff, P + 1, L.part);
+    // the row node of every task, as a halo-node index
+    for (int u = tid; u < NTu; u += kTB) {
+        const int c = column_of(L.coff, P, u), r = u - (int)L.coff[c];
+        const int32_t m = mv.adj[mv.adj_ptr[L.owned[c]] + r];
+        const int j = lower_bound_i32(L.hn, NH, m);
+        if (j >= NH || L.hn[j] != m) {  // a block the tile's elements do not reach
+            atomicOr(err, kErrEmptyBlock);
+            sh_fail = 1;
+        }
+        rows[u] = (uint16_t)j;
+    }
+    __syncthreads();
+    if (sh_fail) return;
+    mark(2);
 
     // block tasks in (column, row) order
     for (int u = tid; u < NTu; u += kTB) {
-        int lo = 0, hi = P;  // the last column whose first task is <= u
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (L.coff[mid] <= (uint32_t)u)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        const int c = lo, r = u - (int)L.coff[c];
-        const int32_t v = L.owned[c];
-        const int32_t m = mv.adj[mv.adj_ptr[v] + r];
-        const bool diag = m == v, sw = m > v;
-        const int32_t pm = inv[m];
-        const bool same = !diag && inc[pm] - 1 == tile;
-        const int c2 = same ? (int)(pm - s0) : -1;
+        const int c = column_of(L.coff, P, u), r = u - (int)L.coff[c];
+        const int jc = ownj[c], jm = rows[u];
+        const bool diag = jm == jc, sw = jm > jc;  // halo-node indices ascend with the node numbers
+        const int c2 = ownc[jm];
+        const bool same = !diag && c2 >= 0;
         const bool drop = same && c2 < c, pair = same && c2 > c;
         bool sym = diag;
-        const int k = drop ? 0 : merge_elements(mv, v, m, [&](uint32_t, uint32_t ac, uint32_t am, int) {
+        const int k = drop ? 0 : merge_halo(keys, nptr, jc, jm, [&](uint32_t, uint32_t ac, uint32_t am, int) {
             if (ac != am) sym = false;
         });
         int r2 = 0;
         if (pair) {
-            const int64_t b0 = mv.adj_ptr[m];
-            const int dm = (int)(mv.adj_ptr[m + 1] - b0);
-            r2 = lower_bound_i32(mv.adj + b0, dm, v);
-            if (r2 >= dm || mv.adj[b0 + r2] != v) {
+            const int b0 = (int)L.coff[c2], dm = (int)(L.coff[c2 + 1] - L.coff[c2]);
+            int lo = 0, hi = dm;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (rows[b0 + mid] < jc)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            r2 = lo;
+            if (r2 >= dm || rows[b0 + r2] != jc) {
                 atomicOr(err, kErrAsym);
                 sh_fail = 1;
             }
@@ -533,6 +770,7 @@ __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *x
     }
     __syncthreads();
     if (sh_fail) return;
+    mark(3);
     const int K = sh_K;
 
     // stable counting sort of the kept tasks by task_bucket
@@ -545,31 +783,31 @@ __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *x
     }
     __syncthreads();
     const int NTk = (int)block_excl_scan(bcount, nb, L.part);
-    if (wave == 0) {
+    // each bucket's tasks in task order, one wavefront per bucket
+    for (int b = wave; b < nb; b += kTB / 64) {
+        uint32_t pos = bcount[b];
         for (int base = 0; base < NTu; base += 64) {
             const int u = base + lane;
             const uint32_t w = u < NTu ? word[u] : kDropBit;
-            const bool valid = !(w & kDropBit);
-            const int b = valid ? task_bucket(w, K) : -1;
-            uint64_t rem = __ballot(valid);
-            while (rem) {
-                const int leader = __ffsll((unsigned long long)rem) - 1;
-                const int bv = __shfl(b, leader, 64);
-                const uint64_t mask = __ballot(valid && b == bv);
-                const uint32_t first = bcount[bv];
-                if (valid && b == bv) sorted[first + __popcll(mask & ((1ull << lane) - 1ull))] = (uint16_t)u;
-                if (lane == leader) bcount[bv] = first + (uint32_t)__popcll(mask);
-                rem &= ~mask;
-            }
+            const bool hit = !(w & kDropBit) && task_bucket(w, K) == b;
+            const uint64_t mask = __ballot(hit);
+            if (hit) sorted[pos + __popcll(mask & ((1ull << lane) - 1ull))] = (uint16_t)u;
+            pos += (uint32_t)__popcll(mask);
         }
     }
     __syncthreads();
+    mark(4);
 
     // wavefront schedule: chunks of 64 sorted tasks dealt to the 4 SIMDs,
     // the costliest first, each to the least loaded (first on ties); plain
     // order when that needs more than kPadTasks null tasks
+    const int nch = (NTk + 63) / 64;
+    for (int ch = tid; ch < nch; ch += kTB) {
+        uint32_t sum = 0;
+        for (int i = ch * 64; i < min(NTk, ch * 64 + 64); i++) sum += (word[sorted[i]] >> 16) & 255u;
+        csum[ch] = sum;
+    }
     if (tid == 0) {
-        const int nch = (NTk + 63) / 64;
         double load[4] = {0.0, 0.0, 0.0, 0.0};
         int qn[4] = {0, 0, 0, 0};
         for (int ch = 0; ch < nch; ch++) {
@@ -587,12 +825,11 @@ __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *x
             for (int ch = 0; ch < nch; ch++) cq[ch] = ch;
             rounds = (nch + 3) / 4;
         }
-        sh_nch = nch;
         sh_nsg = rounds * 4;
         sh_NT = 0;
     }
     __syncthreads();
-    const int nch = sh_nch, nsg = sh_nsg;
+    const int nsg = sh_nsg;
     for (int i = tid; i < nsg; i += kTB) qtab[i] = -1;
     __syncthreads();
     for (int ch = tid; ch < nch; ch += kTB) {
@@ -604,14 +841,13 @@ __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *x
         uint32_t acc = 0;
         for (int sg = 0; sg < nsg; sg++) {
             ibase[sg] = acc;
-            const int ch = qtab[sg];
-            if (ch < 0) continue;
-            for (int i = ch * 64; i < min(NTk, ch * 64 + 64); i++) acc += (word[sorted[i]] >> 16) & 255u;
+            if (qtab[sg] >= 0) acc += csum[qtab[sg]];
         }
         ibase[nsg] = acc;
         sh_NI = acc;
     }
     __syncthreads();
+    mark(5);
     const int NT = sh_NT;
     const uint32_t NI = sh_NI;
     if (NI > 65535u) {
@@ -650,16 +886,14 @@ __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *x
             if (sinfo) sinfo[s] = valid ? (toff | (k << 16)) : 0xffffffffu;
         }
         if (valid) {
-            const int c = (int)(w & 255u), r = (int)((w >> 8) & 255u);
-            const int32_t v = L.owned[c];
-            const int32_t m = mv.adj[mv.adj_ptr[v] + r];
-            merge_elements(mv, v, m, [&](uint32_t e, uint32_t ac, uint32_t am, int j) {
-                const int hl = lower_bound_i32(L.halo, H, (int32_t)e);
-                items[toff + j] = (uint16_t)((hl << 8) | (ac << 4) | am);
+            const int c = (int)(w & 255u);
+            merge_halo(keys, nptr, ownj[c], rows[u], [&](uint32_t h, uint32_t ac, uint32_t am, int j) {
+                items[toff + j] = (uint16_t)((h << 8) | (ac << 4) | am);
             });
         }
     }
     __syncthreads();
+    mark(6);
 
     if (caps.spread && H > 1) {
         // Halo numbering for the LDS banks (the host's spread_halo_banks):
@@ -668,7 +902,7 @@ __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *x
         uint32_t *gk = reinterpret_cast<uint32_t *>(dyn + B.gk);
         uint32_t *occcnt = reinterpret_cast<uint32_t *>(dyn + B.occcnt);
         uint32_t *occfill = reinterpret_cast<uint32_t *>(dyn + B.occfill);
-        uint32_t *occ = reinterpret_cast<uint32_t *>(dyn + B.occ);
+        uint16_t *occ = B.occ_in_buf ? reinterpret_cast<uint16_t *>(L.buf) : reinterpret_cast<uint16_t *>(dyn + B.occ);
         uint8_t *load = reinterpret_cast<uint8_t *>(dyn + B.load);
         int32_t *perm = reinterpret_cast<int32_t *>(dyn + B.perm);
         int32_t *eorder = reinterpret_cast<int32_t *>(dyn + B.order);
@@ -688,48 +922,56 @@ __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *x
             return;
         }
         for (int i = tid; i < nsets * 16; i += kTB) load[i] = 0;
-        // the distinct reads of set si, in slot order
-        auto set_reads = [&](int si, uint16_t *rs) {
-            int lo = 0, hi = ngroups;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (gk[mid] <= (uint32_t)si)
-                    lo = mid;
-                else
-                    hi = mid;
+        // The reads of each set: one quarter-wavefront per group of 16 slots,
+        // lane l for slot 16 g + l, one set after the other; a read counts
+        // once per set (at its first slot).  Pass 0 counts each element's
+        // reads, pass 1 places them.
+        uint16_t *rq = reinterpret_cast<uint16_t *>(L.part) + (tid & ~15);  // 16 entries per quarter
+        const int q = tid >> 4, ql = tid & 15;
+        for (int pass = 0; pass < 2; pass++) {
+            if (pass == 1) {
+                block_excl_scan(occcnt, H + 1, L.part);
+                for (int i = tid; i <= H; i += kTB) occfill[i] = occcnt[i];
+                __syncthreads();
             }
-            const int g = lo, local = si - (int)gk[g], j = local >> 1, side = local & 1;
-            int n = 0;
-            for (int s = 16 * g; s < min(NT, 16 * g + 16); s++) {
-                const uint32_t in = sinfo[s];
-                if (in == 0xffffffffu || (uint32_t)j >= (in >> 16)) continue;
-                const uint16_t it = items[(in & 0xffffu) + j];
-                const uint16_t rd = (uint16_t)(((it >> 8) << 4) | (side ? (it & 15u) : ((it >> 4) & 15u)));
-                bool seen = false;
-                for (int q = 0; q < n; q++) seen = seen || rs[q] == rd;
-                if (!seen) rs[n++] = rd;
+            for (int g = q; g < ngroups; g += kTB / 16) {
+                const int sl = 16 * g + ql;
+                const uint32_t in = sl < NT ? sinfo[sl] : 0xffffffffu;
+                const uint32_t kl = in == 0xffffffffu ? 0u : (in >> 16);
+                const int g0 = (int)gk[g], ng = (int)(gk[g + 1] - gk[g]);  // the group's 2 kmax sets
+                for (int j = 0; j < ng / 2; j++) {  // sets g0 + 2 j (column side), g0 + 2 j + 1 (row side)
+                    const bool valid = (uint32_t)j < kl;
+                    uint16_t r0 = 0xffff, r1 = 0xffff;
+                    if (valid) {
+                        const uint16_t it = items[(in & 0xffffu) + j];
+                        r0 = (uint16_t)(((it >> 8) << 4) | ((it >> 4) & 15u));
+                        r1 = (uint16_t)(((it >> 8) << 4) | (it & 15u));
+                    }
+                    rq[ql] = r0;
+                    rq[256 + ql] = r1;
+                    __builtin_amdgcn_wave_barrier();
+                    uint32_t w0[8], w1[8];
+                    load_row16(rq, w0);
+                    load_row16(rq + 256, w1);
+                    const bool f0 = valid && first_in_row(w0, ql, r0), f1 = valid && first_in_row(w1, ql, r1);
+                    if (pass == 0) {
+                        if (f0) atomicAdd(&occcnt[r0 >> 4], 1u);
+                        if (f1) atomicAdd(&occcnt[r1 >> 4], 1u);
+                    } else {
+                        const uint32_t si = (uint32_t)(g0 + 2 * j);
+                        if (f0) occ[atomicAdd(&occfill[r0 >> 4], 1u)] = (uint16_t)(si * 16u + (r0 & 15u));
+                        if (f1) occ[atomicAdd(&occfill[r1 >> 4], 1u)] = (uint16_t)((si + 1u) * 16u + (r1 & 15u));
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
             }
-            return n;
-        };
-        __syncthreads();
-        for (int si = tid; si < nsets; si += kTB) {
-            uint16_t rs[16];
-            const int n = set_reads(si, rs);
-            for (int q = 0; q < n; q++) atomicAdd(&occcnt[rs[q] >> 4], 1u);
-        }
-        __syncthreads();
-        block_excl_scan(occcnt, H + 1, L.part);
-        for (int i = tid; i <= H; i += kTB) occfill[i] = occcnt[i];
-        __syncthreads();
-        for (int si = tid; si < nsets; si += kTB) {
-            uint16_t rs[16];
-            const int n = set_reads(si, rs);
-            for (int q = 0; q < n; q++) occ[atomicAdd(&occfill[rs[q] >> 4], 1u)] = (uint32_t)si * 16u + (rs[q] & 15u);
+            __syncthreads();
         }
         // elements by read count, descending, stable; slots per residue class
         const int uc = shape.uchunks();
         if (tid < 16) slotl[tid] = 0;
         __syncthreads();
+        mark(7);
         for (int e = tid; e < H; e += kTB) {
             const uint32_t ce = occcnt[e + 1] - occcnt[e];
             int rank = 0;
@@ -757,58 +999,71 @@ __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *x
             slotl[16 + slotl[cls] + rank] = sl;
         }
         __syncthreads();
+        mark(8);
         if (wave == 0) {
-            const int c = lane & 15, sub = lane >> 4;
+            // The four lanes 4 c .. 4 c + 3 hold class c (its first slot, its
+            // slot count, the slots used) and sum its cost over every fourth
+            // read of the element; the class chosen is the minimum of (cost,
+            // -remaining slots, class) over the classes with a slot left.
+            const int c = lane >> 2, sub = lane & 3;
             const int cstart = slotl[c];
             const int ccount = (c < 15 ? slotl[c + 1] : H) - cstart;
             int used = 0;
+            int e = eorder[0];
+            int o0 = (int)occcnt[e], o1 = (int)occcnt[e + 1];
             for (int idx = 0; idx < H; idx++) {
-                const int e = eorder[idx];
-                const int o0 = (int)occcnt[e], o1 = (int)occcnt[e + 1];
+                // the next element's reads (independent of this one's choice)
+                const int en = idx + 1 < H ? eorder[idx + 1] : 0;
+                const int n0 = (int)occcnt[en], n1 = (int)occcnt[en + 1];
                 uint32_t part = 0;
-                for (int i = o0 + sub; i < o1; i += 4) {
-                    const uint32_t x = occ[i];
-                    part += load[(x >> 4) * 16 + (((x & 15u) + (uint32_t)c) & 15u)];
-                }
-                part += __shfl_xor(part, 16, 64);
-                part += __shfl_xor(part, 32, 64);
-                const uint32_t cost = part & 0xffffu;
-                const int rem = ccount - used;
-                int best = -1, brem = 0;
-                uint32_t bc = 0;
-                for (int cc = 0; cc < 16; cc++) {
-                    const uint32_t costc = __shfl(cost, cc, 64);
-                    const int remc = __shfl(rem, cc, 64);
-                    if (remc == 0) continue;
-                    if (best < 0 || costc < bc || (costc == bc && remc > brem)) {
-                        best = cc;
-                        bc = costc;
-                        brem = remc;
+                for (int i0 = o0 + sub; i0 < o1; i0 += 32) {  // eight reads at a time, no branches
+                    uint32_t x[8];
+#pragma unroll
+                    for (int r = 0; r < 8; r++) x[r] = occ[i0 + 4 * r < o1 ? i0 + 4 * r : o0];
+#pragma unroll
+                    for (int r = 0; r < 8; r++) {
+                        const uint32_t v = load[(x[r] >> 4) * 16 + (((x[r] & 15u) + (uint32_t)c) & 15u)];
+                        part += i0 + 4 * r < o1 ? v : 0u;
                     }
                 }
-                const int ub = __shfl(used, best, 64);
-                const int bstart = __shfl(cstart, best, 64);
-                if (lane == 0) perm[e] = slotl[16 + bstart + ub];
-                if (lane == best) used++;
+                // sum over the quad, then the minimum key over the 16 quads
+                part += (uint32_t)__builtin_amdgcn_mov_dpp((int)part, 0xB1, 0xf, 0xf, false);  // quad_perm 1,0,3,2
+                part += (uint32_t)__builtin_amdgcn_mov_dpp((int)part, 0x4E, 0xf, 0xf, false);  // quad_perm 2,3,0,1
+                const int rem = ccount - used;
+                uint32_t key = rem > 0 ? (((part & 0xffffu) << 12) | ((uint32_t)(255 - rem) << 4) | (uint32_t)c)
+                                       : 0xffffffffu;
+                key = min(key, (uint32_t)__builtin_amdgcn_mov_dpp((int)key, 0x141, 0xf, 0xf, false));  // row_half_mirror
+                key = min(key, (uint32_t)__builtin_amdgcn_mov_dpp((int)key, 0x140, 0xf, 0xf, false));  // row_mirror
+                key = min(key, (uint32_t)__shfl_xor((int)key, 16, 64));
+                key = min(key, (uint32_t)__shfl_xor((int)key, 32, 64));
+                const int best = (int)((uint32_t)__builtin_amdgcn_readfirstlane((int)key) & 15u);
+                if (c == best) {
+                    if (sub == 0) perm[e] = slotl[16 + cstart + used];
+                    used++;
+                }
                 for (int i = o0 + lane; i < o1; i += 64) {
-                    const uint32_t x = occ[i];
-                    uint8_t &ld = load[(x >> 4) * 16 + (((x & 15u) + (uint32_t)best) & 15u)];
+                    const uint32_t xx = occ[i];
+                    uint8_t &ld = load[(xx >> 4) * 16 + (((xx & 15u) + (uint32_t)best) & 15u)];
                     if (ld < 255) ld = (uint8_t)(ld + 1);
                 }
+                e = en;
+                o0 = n0;
+                o1 = n1;
             }
         }
         __syncthreads();
+        mark(9);
         for (uint32_t i = tid; i < NI; i += kTB) {
             const uint16_t it = items[i];
             gitem[i] = (uint16_t)((perm[it >> 8] << 8) | (it & 255u));
         }
         uint16_t *glconn = reinterpret_cast<uint16_t *>(blob + (Lay.lconn - Lay.cbase));
-        for (int i = tid; i < H * npe; i += kTB) glconn[perm[i / npe] * npe + i % npe] = lconn[i];
+        for (int i = tid; i < nc; i += kTB) glconn[perm[i / npe] * npe + i % npe] = lconn[i];
     } else {
         if (caps.spread)
             for (uint32_t i = tid; i < NI; i += kTB) gitem[i] = items[i];
         uint16_t *glconn = reinterpret_cast<uint16_t *>(blob + (Lay.lconn - Lay.cbase));
-        for (int i = tid; i < H * npe; i += kTB) glconn[i] = lconn[i];
+        for (int i = tid; i < nc; i += kTB) glconn[i] = lconn[i];
     }
 
     // column pointers, coordinates, the alignment gaps
@@ -829,7 +1084,7 @@ __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *x
     if (tid < 6) {
         const size_t ends[6] = {Lay.cbase + (size_t)P * 8,  Lay.cdeg + (size_t)P * 4,
                                 Lay.task + (size_t)NT * 8,  Lay.item + (size_t)NI * 2,
-                                Lay.lconn + (size_t)H * npe * 2, Lay.X + (size_t)NH * 24};
+                                Lay.lconn + (size_t)nc * 2, Lay.X + (size_t)NH * 24};
         const size_t next[6] = {Lay.cdeg, Lay.task, Lay.item, Lay.lconn, Lay.X, Lay.U};
         for (size_t b = ends[tid]; b < next[tid]; b++) blob[b - Lay.cbase] = 0;
     }
@@ -839,6 +1094,8 @@ __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *x
         tout[4 * t + 2] = (int32_t)NI;
         tout[4 * t + 3] = (int32_t)(Lay.U - Lay.cbase);
     }
+    __syncthreads();
+    mark(10);
 }
 
 // ---- 4. packing and headers
@@ -854,7 +1111,7 @@ __global__ void k_tile_sizes(const int32_t *tout, int64_t ntiles, int64_t *nbyte
 
 __global__ void k_tile_pack(const int64_t *ts, const int64_t *info, const int32_t *tout, const int64_t *x0,
                             const int64_t *slot_off, const int64_t *boff, const unsigned char *slots,
-                            unsigned char *blob, TileHdr *hdr, TileStats *st) {
+                            unsigned char *blob, TileHdr *hdr) {
     const int64_t t = blockIdx.x;
     const int64_t nbytes = tout[4 * t + 3];
     const uint4 *src = reinterpret_cast<const uint4 *>(slots + slot_off[t]);
@@ -873,13 +1130,27 @@ __global__ void k_tile_pack(const int64_t *ts, const int64_t *info, const int32_
         h.nbytes = (int32_t)nbytes;
         h.blob = boff[t];
         hdr[t] = h;
-        atomicAdd(&st->nhalo, (unsigned long long)h.H);
-        atomicAdd(&st->ntasks, (unsigned long long)h.NT);
-        atomicAdd(&st->nitems, (unsigned long long)h.NI);
-        atomicMax(&st->max_nodes, (unsigned long long)h.P);
-        atomicMax(&st->max_halo, (unsigned long long)h.H);
-        atomicMax(&st->max_k, (unsigned long long)h.K);
     }
+}
+
+__global__ __launch_bounds__(256) void k_tile_stats(const TileHdr *hdr, int64_t ntiles, TileStats *st) {
+    __shared__ unsigned long long sh[256];
+    unsigned long long nh = 0, nt = 0, ni = 0, mp = 0, mh = 0, mk = 0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntiles; t += (int64_t)gridDim.x * blockDim.x) {
+        const TileHdr h = hdr[t];
+        nh += h.H;
+        nt += h.NT;
+        ni += h.NI;
+        mp = max(mp, (unsigned long long)h.P);
+        mh = max(mh, (unsigned long long)h.H);
+        mk = max(mk, (unsigned long long)h.K);
+    }
+    block_sum_to(nh, &st->nhalo, sh);
+    block_sum_to(nt, &st->ntasks, sh);
+    block_sum_to(ni, &st->nitems, sh);
+    block_max_to(mp, &st->max_nodes, sh);
+    block_max_to(mh, &st->max_halo, sh);
+    block_max_to(mk, &st->max_k, sh);
 }
 
 template <class T>
@@ -916,6 +1187,8 @@ void raise_tile_errors(unsigned e) {
     if (e & kErrAsym) throw Error(FEASM_ESTATE, "pattern is not symmetric");
     if (e & kErrEmptyBlock) throw Error(FEASM_ESTATE, "a pattern block has no element");
     if (e & kErrBudget) throw Error(FEASM_ESTATE, "a tile exceeds the LDS budget it was sized for");
+    if (e & kErrSets)
+        throw Error(FEASM_EUNSUPPORTED, "a tile's bank-spread read sets exceed the device tile builder's capacity");
     throw Error(FEASM_ESTATE, "device tile builder: inconsistent sizes (code " + std::to_string(e) + ")");
 }
 
@@ -939,29 +1212,28 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
         return T;
     }
     Clock clk(s);
+    clk.lap("stream work before");
     MeshView mv{dp<int32_t>(m->conn), dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e), dp<int64_t>(pat.adj_ptr),
                 dp<int32_t>(pat.adj), npe};
     const double *xyz = dp<double>(m->xyz);
 
     // 1. Morton order
     const unsigned nbb = grid_for(nn, 256, 1024);
-    DevPtr bb = dev_alloc((size_t)nbb * 6 * sizeof(double));
+    DevPtr bb = dev_alloc((size_t)(nbb + 1) * 6 * sizeof(double));
     hipLaunchKernelGGL(k_bbox, dim3(nbb), dim3(256), 0, s, xyz, nn, dp<double>(bb));
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, s, dp<double>(bb), (int)nbb, dp<double>(bb) + 6 * nbb);
     FEASM_HIP(hipGetLastError());
-    std::vector<double> hb((size_t)nbb * 6);
-    FEASM_HIP(hipMemcpyAsync(hb.data(), bb->p, hb.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    double hb[6];
+    FEASM_HIP(hipMemcpyAsync(hb, dp<double>(bb) + 6 * nbb, sizeof(hb), hipMemcpyDeviceToHost, s));
     FEASM_HIP(hipStreamSynchronize(s));
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (unsigned b = 0; b < nbb; b++)
-        for (int k = 0; k < 3; k++) {
-            lo[k] = std::min(lo[k], hb[6 * b + k]);
-            hi[k] = std::max(hi[k], hb[6 * b + 3 + k]);
-        }
+    const double lo[3] = {hb[0], hb[1], hb[2]}, hi[3] = {hb[3], hb[4], hb[5]};
     const double step = morton_step(lo, hi, nn);
+    clk.lap("bounding box");
     DevPtr code_a = dev_alloc(nn * 8), code_b = dev_alloc(nn * 8), id_a = dev_alloc(nn * 4), id_b = dev_alloc(nn * 4);
     hipLaunchKernelGGL(k_morton, dim3(grid_for(nn, 256, 65536)), dim3(256), 0, s, xyz, nn, lo[0], lo[1], lo[2], step,
                        dp<uint64_t>(code_a), dp<int32_t>(id_a));
     FEASM_HIP(hipGetLastError());
+    clk.lap("codes");
     const uint64_t *code;
     const int32_t *order;
     {
@@ -974,32 +1246,39 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
         code = kb.current();
         order = vb.current();
     }
-    DevPtr inv = dev_alloc(nn * 4);
-    hipLaunchKernelGGL(k_inverse, dim3(grid_for(nn, 256, 65536)), dim3(256), 0, s, order, nn, dp<int32_t>(inv));
-    FEASM_HIP(hipGetLastError());
-    clk.lap("morton order");
+    clk.lap("sort");
 
     // 2. the box tree
     DevPtr start = dev_alloc(nn), info = dev_alloc((size_t)nn * kInfo * 8);
     DevPtr qa = dev_alloc((size_t)(nn + 2) * 16), qb = dev_alloc((size_t)(nn + 2) * 16);
-    DevPtr ctr = dev_alloc(16);  // [boxes of the next level, error bits]
+    constexpr int kMaxLevels = 256;  // each level halves boxes at least once: depth < 64 + log2(nn)
+    DevPtr ctr = dev_alloc((kMaxLevels + 2) * sizeof(unsigned));  // [error bits, boxes per level...]
     FEASM_HIP(hipMemsetAsync(start->p, 0, nn, s));
-    FEASM_HIP(hipMemsetAsync(ctr->p, 0, 16, s));
+    FEASM_HIP(hipMemsetAsync(ctr->p, 0, (kMaxLevels + 2) * sizeof(unsigned), s));
     const int64_t root[2] = {0, nn};
+    const unsigned one = 1;
     FEASM_HIP(hipMemcpyAsync(qa->p, root, sizeof(root), hipMemcpyHostToDevice, s));
-    unsigned *d_nout = dp<unsigned>(ctr), *d_err = dp<unsigned>(ctr) + 1;
-    unsigned hc[2] = {1, 0};
+    unsigned *d_err = dp<unsigned>(ctr), *d_count = dp<unsigned>(ctr) + 1;
+    FEASM_HIP(hipMemcpyAsync(d_count, &one, sizeof(one), hipMemcpyHostToDevice, s));
+    unsigned hc[2] = {0, 0};
     int levels = 0;
-    for (unsigned nin = 1; nin > 0; levels++) {
-        FEASM_HIP(hipMemsetAsync(d_nout, 0, sizeof(unsigned), s));
-        hipLaunchKernelGGL(k_part_level, dim3(nin), dim3(kTB), 0, s, dp<int64_t>(qa), dp<int64_t>(qb), d_nout,
-                           (unsigned)(nn + 2), code, order, mv, shape, dp<uint8_t>(start), dp<int64_t>(info), d_err);
-        FEASM_HIP(hipGetLastError());
-        FEASM_HIP(hipMemcpyAsync(hc, d_nout, sizeof(hc), hipMemcpyDeviceToHost, s));
+    for (bool more = true; more;) {
+        // a batch of levels between read-backs (a level with no boxes costs
+        // one launch)
+        for (int b = 0; b < 8; b++, levels++) {
+            FEASM_REQUIRE(levels < kMaxLevels, FEASM_ESTATE, "device tile builder: box tree too deep");
+            const unsigned grid = (unsigned)std::min<int64_t>(std::min<int64_t>(int64_t(1) << std::min(levels, 20), nn), 16384);
+            hipLaunchKernelGGL(k_part_level, dim3(grid), dim3(kTB), 0, s, dp<int64_t>(qa), dp<int64_t>(qb), d_count,
+                               levels, (unsigned)(nn + 2), code, order, mv, shape, dp<uint8_t>(start),
+                               dp<int64_t>(info), d_err);
+            FEASM_HIP(hipGetLastError());
+            std::swap(qa, qb);
+        }
+        FEASM_HIP(hipMemcpyAsync(&hc[0], d_err, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        FEASM_HIP(hipMemcpyAsync(&hc[1], d_count + levels, sizeof(unsigned), hipMemcpyDeviceToHost, s));
         FEASM_HIP(hipStreamSynchronize(s));
-        raise_tile_errors(hc[1]);
-        nin = hc[0];
-        std::swap(qa, qb);
+        raise_tile_errors(hc[0]);
+        more = hc[1] > 0;
     }
     qa.reset();
     qb.reset();
@@ -1027,7 +1306,7 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
     DevPtr x0 = dev_alloc((ntiles + 1) * 8), slot_off = dev_alloc((ntiles + 1) * 8);
     DevPtr pst = dev_alloc(sizeof(PrepStats));
     FEASM_HIP(hipMemsetAsync(pst->p, 0, sizeof(PrepStats), s));
-    hipLaunchKernelGGL(k_tile_prep, dim3(grid_for(ntiles + 1, 256, 65536)), dim3(256), 0, s, dp<int64_t>(ts), ntiles,
+    hipLaunchKernelGGL(k_tile_prep, dim3(grid_for(ntiles + 1, 256, 1024)), dim3(256), 0, s, dp<int64_t>(ts), ntiles,
                        dp<int64_t>(info), shape, dp<int64_t>(nh), dp<int64_t>(slot), dp<PrepStats>(pst));
     FEASM_HIP(hipGetLastError());
     scan_excl<int64_t>(s, dp<int64_t>(nh), dp<int64_t>(x0), ntiles + 1);
@@ -1045,7 +1324,7 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
     caps.spread = (shape.grad && shape.rbytes == 8 && npe == 8) ? 1 : 0;
     // sets of the bank spread: per chunk of 64 sorted tasks at most
     // 2 * (K + (chunk items) / 16), the groups' first tasks
-    caps.nsets = caps.spread ? 2 * ((caps.ntu / 64 + 2) * (int)ps.k_max + caps.niu / 16 + 16) : 0;
+    caps.nsets = caps.spread ? std::min(4095, 2 * ((caps.ntu / 64 + 2) * (int)ps.k_max + caps.niu / 16 + 16)) : 0;
     const BuildLds B = build_lds(caps, npe);
     int maxlds = 0;
     FEASM_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
@@ -1057,20 +1336,36 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
     clk.lap("sizes");
 
     // 3. the tiles
+    DevPtr tim;
+    if (clk.on) {
+        tim = dev_alloc(16 * 8);
+        FEASM_HIP(hipMemsetAsync(tim->p, 0, 16 * 8, s));
+    }
     if (ntiles > 0) {
         if (B.total > 0)
             FEASM_HIP(hipFuncSetAttribute((const void *)k_tile_build, hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)B.total));
         hipLaunchKernelGGL(k_tile_build, dim3((unsigned)ntiles), dim3(kTB), B.total, s, mv, xyz, order,
-                           dp<int32_t>(inv), dp<int32_t>(inc), dp<int64_t>(ts), dp<int64_t>(info), dp<int64_t>(x0),
-                           dp<int64_t>(slot_off), shape, caps, dp<unsigned char>(slots), dp<int32_t>(T->hnodes),
-                           dp<int32_t>(tout), d_err);
+                           dp<int64_t>(ts), dp<int64_t>(info), dp<int64_t>(x0), dp<int64_t>(slot_off), shape, caps,
+                           dp<unsigned char>(slots), dp<int32_t>(T->hnodes), dp<int32_t>(tout), d_err,
+                           dp<unsigned long long>(tim));
         FEASM_HIP(hipGetLastError());
     }
-    FEASM_HIP(hipMemcpyAsync(hc, d_nout, sizeof(hc), hipMemcpyDeviceToHost, s));
+    FEASM_HIP(hipMemcpyAsync(&hc[0], d_err, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     FEASM_HIP(hipStreamSynchronize(s));
-    raise_tile_errors(hc[1]);
+    raise_tile_errors(hc[0]);
     clk.lap("tile arrays");
+    if (clk.on) {
+        unsigned long long tk[16];
+        FEASM_HIP(hipMemcpy(tk, tim->p, sizeof(tk), hipMemcpyDeviceToHost));
+        int khz = 0;
+        FEASM_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+        static const char *names[11] = {"eval",    "lists",  "rows",    "tasks",  "bucket sort", "schedule",
+                                        "records", "spread sets", "spread ranks", "spread greedy", "writes"};
+        for (int k = 0; k < 11; k++)
+            std::fprintf(stderr, "device tiles:   %-16s %10.3f ms summed over tiles\n", names[k],
+                         khz ? (double)tk[k] / khz : 0.0);
+    }
 
     // 4. packing
     DevPtr nbytes = dev_alloc((ntiles + 1) * 8), boff = dev_alloc((ntiles + 1) * 8);
@@ -1089,8 +1384,10 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
     if (ntiles > 0) {
         hipLaunchKernelGGL(k_tile_pack, dim3((unsigned)ntiles), dim3(256), 0, s, dp<int64_t>(ts), dp<int64_t>(info),
                            dp<int32_t>(tout), dp<int64_t>(x0), dp<int64_t>(slot_off), dp<int64_t>(boff),
-                           dp<unsigned char>(slots), dp<unsigned char>(T->blob), dp<TileHdr>(T->hdr),
-                           dp<TileStats>(tst));
+                           dp<unsigned char>(slots), dp<unsigned char>(T->blob), dp<TileHdr>(T->hdr));
+        FEASM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_tile_stats, dim3(grid_for(ntiles, 256, 1024)), dim3(256), 0, s, dp<TileHdr>(T->hdr),
+                           ntiles, dp<TileStats>(tst));
         FEASM_HIP(hipGetLastError());
     }
     TileStats st;
