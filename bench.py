@@ -149,18 +149,6 @@ def bench_vec(feasm, core, n=100, steps=10):
     its algorithmic bytes (int32 connectivity and FP64 coordinates read once,
     FP64 vector written once)."""
     ctx = feasm.Context(0)
-    # the process's first assembly loads the library's device code (every
-    # kernel of the path, once per process): timed on a 4^3 mesh, so that
-    # the C2 first assembly below is that of a new mesh in a running process
-    t0 = time.perf_counter()
-    c4, x4 = core.gen_h8block(1.0, 1.0, 1.0, 4, 4, 4)
-    m4 = core.DeviceMesh(ctx, feasm.H8, c4, x4)
-    m4.set_dofs(3)
-    m4.bilform(feasm.LIN_ELASTIC, feasm.RULE_GAUSS, 2, D).close()
-    ctx.synchronize()
-    t_load = time.perf_counter() - t0
-    m4.close()
-    log("process's first assembly (4^3 mesh, device code loaded): %.1f ms" % (1e3 * t_load))
     conn, xyz = core.gen_h8block(1.0, 1.0, 1.0, n, n, n)
     xyz = core.gen_perturb(xyz, [0, 0, 0], [1, 1, 1], 0.15 / n, 2)
     ne, nn = conn.shape[0], xyz.shape[0]

@@ -97,8 +97,10 @@ void prepare(feasm_batch *b) {
         feasm_mesh *m = it.mesh;
         if (!m->pattern || m->pattern->ndn != m->ndn) m->pattern = build_pattern(m, m->ndn);
     }
-    // tiles of the meshes that lack them, built concurrently (one host thread
-    // per mesh at a time)
+    // tiles of the meshes that lack them, built concurrently on the device
+    // (one host thread per mesh at a time, each with a stream of its own)
+    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto &it : b->items) FEASM_HIP(hipStreamSynchronize(it.mesh->ctx->stream));
     {
         std::vector<std::pair<feasm_mesh *, TileShape>> todo;
         for (auto &it : b->items) {
@@ -116,10 +118,12 @@ void prepare(feasm_batch *b) {
         for (int w = 0; w < nth; w++)
             pool.emplace_back([&]() {
                 const bool dev_ok = hipSetDevice(ctx->device) == hipSuccess;
+                hipStream_t st = nullptr;
+                const bool st_ok = dev_ok && hipStreamCreate(&st) == hipSuccess;
                 for (size_t i; (i = next++) < todo.size();) {
                     try {
-                        FEASM_REQUIRE(dev_ok, FEASM_EHIP, "hipSetDevice failed in a tile-building thread");
-                        built[i] = build_tiles(todo[i].first, todo[i].second, 1);
+                        FEASM_REQUIRE(st_ok, FEASM_EHIP, "no device stream in a tile-building thread");
+                        built[i] = build_tiles(todo[i].first, todo[i].second, st);
                     } catch (const Error &e) {
                         codes[i] = e.code;
                         errs[i] = e.what();
@@ -127,6 +131,10 @@ void prepare(feasm_batch *b) {
                         codes[i] = FEASM_ENOMEM;
                         errs[i] = e.what();
                     }
+                }
+                if (st_ok) {  // the builder leaves its stream idle (and holds no pool event on it)
+                    (void)hipStreamSynchronize(st);
+                    (void)hipStreamDestroy(st);
                 }
             });
         for (auto &th : pool) th.join();

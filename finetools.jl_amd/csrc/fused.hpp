@@ -135,7 +135,7 @@ struct HostTiles {
 };
 // max_threads bounds the host threads building the per-tile arrays
 void build_tiles_host(const TileHostMesh &h, const TileShape &shape, int max_threads, HostTiles &out);
-std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int max_threads = 16);
+std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, hipStream_t s = nullptr);
 // rewrites the coordinates in every tile's blob from the mesh's coordinates
 // (on the mesh's context stream)
 void refresh_tile_coords(feasm_mesh *m, TileSet &t);
@@ -212,8 +212,9 @@ __host__ __device__ inline int64_t box_split(const uint64_t *code, int64_t s0, i
 }
 
 // the device builder (csrc/tiles_gpu.hip): the same tiles as
-// build_tiles_host, built from the mesh's device arrays on its stream
-std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shape);
+// build_tiles_host, built from the mesh's device arrays on stream s (the
+// mesh's context stream when null; the mesh's arrays must be complete)
+std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shape, hipStream_t s = nullptr);
 // host copies of the arrays the host builder reads
 void tile_host_mesh(feasm_mesh *m, TileHostMesh &h);
 // builds the tiles with both builders and compares headers, blobs and halo

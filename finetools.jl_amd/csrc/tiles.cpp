@@ -664,9 +664,8 @@ int64_t compare_tile_builders(feasm_mesh *m, const TileShape &shape, int64_t *nt
     return -1;
 }
 
-std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, int max_threads) {
-    (void)max_threads;
-    return build_tiles_device(m, shape);
+std::shared_ptr<TileSet> build_tiles(feasm_mesh *m, const TileShape &shape, hipStream_t s) {
+    return build_tiles_device(m, shape, s);
 }
 
 }  // namespace feasm

@@ -1153,12 +1153,13 @@ __global__ __launch_bounds__(256) void k_tile_stats(const TileHdr *hdr, int64_t 
     block_max_to(mk, &st->max_k, sh);
 }
 
+// (the scan's scratch goes to hold: released by the caller once s is idle)
 template <class T>
-void scan_excl(hipStream_t s, const T *in, T *out, int64_t n) {
+void scan_excl(hipStream_t s, const T *in, T *out, int64_t n, std::vector<DevPtr> &hold) {
     size_t bytes = 0;
     FEASM_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, (T)0, (size_t)n, rocprim::plus<T>(), s));
-    DevPtr tmp = dev_alloc(std::max<size_t>(bytes, 16));
-    FEASM_HIP(rocprim::exclusive_scan(tmp->p, bytes, in, out, (T)0, (size_t)n, rocprim::plus<T>(), s));
+    hold.push_back(dev_alloc(std::max<size_t>(bytes, 16)));
+    FEASM_HIP(rocprim::exclusive_scan(hold.back()->p, bytes, in, out, (T)0, (size_t)n, rocprim::plus<T>(), s));
 }
 
 // FEASM_TILE_TIMING=1 prints the device builder's phase times to stderr
@@ -1194,9 +1195,9 @@ void raise_tile_errors(unsigned e) {
 
 }  // namespace
 
-std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shape) {
+std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shape, hipStream_t stream) {
     feasm_ctx *ctx = m->ctx;
-    const hipStream_t s = ctx->stream;
+    const hipStream_t s = stream ? stream : ctx->stream;
     const Pattern &pat = *m->pattern;
     const int64_t nn = m->nnodes;
     const int npe = m->npe;
@@ -1211,6 +1212,20 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
         T->hnodes = dev_alloc(16);
         return T;
     }
+    // Temporaries are released only after the stream is idle, also when an
+    // error leaves early: `hold` keeps every one of them and is destroyed
+    // after `idle`, whose destructor synchronises the stream (the stream
+    // need not be one the memory pool fences).
+    std::vector<DevPtr> hold;
+    struct Idle {
+        hipStream_t s;
+        ~Idle() { (void)hipStreamSynchronize(s); }
+    } idle{s};
+    auto tmp = [&](size_t bytes) {
+        DevPtr d = dev_alloc(bytes);
+        hold.push_back(d);
+        return d;
+    };
     Clock clk(s);
     clk.lap("stream work before");
     MeshView mv{dp<int32_t>(m->conn), dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e), dp<int64_t>(pat.adj_ptr),
@@ -1219,7 +1234,7 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
 
     // 1. Morton order
     const unsigned nbb = grid_for(nn, 256, 1024);
-    DevPtr bb = dev_alloc((size_t)(nbb + 1) * 6 * sizeof(double));
+    DevPtr bb = tmp((size_t)(nbb + 1) * 6 * sizeof(double));
     hipLaunchKernelGGL(k_bbox, dim3(nbb), dim3(256), 0, s, xyz, nn, dp<double>(bb));
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, s, dp<double>(bb), (int)nbb, dp<double>(bb) + 6 * nbb);
     FEASM_HIP(hipGetLastError());
@@ -1229,7 +1244,7 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
     const double lo[3] = {hb[0], hb[1], hb[2]}, hi[3] = {hb[3], hb[4], hb[5]};
     const double step = morton_step(lo, hi, nn);
     clk.lap("bounding box");
-    DevPtr code_a = dev_alloc(nn * 8), code_b = dev_alloc(nn * 8), id_a = dev_alloc(nn * 4), id_b = dev_alloc(nn * 4);
+    DevPtr code_a = tmp(nn * 8), code_b = tmp(nn * 8), id_a = tmp(nn * 4), id_b = tmp(nn * 4);
     hipLaunchKernelGGL(k_morton, dim3(grid_for(nn, 256, 65536)), dim3(256), 0, s, xyz, nn, lo[0], lo[1], lo[2], step,
                        dp<uint64_t>(code_a), dp<int32_t>(id_a));
     FEASM_HIP(hipGetLastError());
@@ -1241,18 +1256,18 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
         rocprim::double_buffer<int32_t> vb(dp<int32_t>(id_a), dp<int32_t>(id_b));
         size_t bytes = 0;
         FEASM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, kb, vb, (size_t)nn, 0u, 63u, s));
-        DevPtr tmp = dev_alloc(std::max<size_t>(bytes, 16));
-        FEASM_HIP(rocprim::radix_sort_pairs(tmp->p, bytes, kb, vb, (size_t)nn, 0u, 63u, s));
+        DevPtr sort_tmp = tmp(std::max<size_t>(bytes, 16));
+        FEASM_HIP(rocprim::radix_sort_pairs(sort_tmp->p, bytes, kb, vb, (size_t)nn, 0u, 63u, s));
         code = kb.current();
         order = vb.current();
     }
     clk.lap("sort");
 
     // 2. the box tree
-    DevPtr start = dev_alloc(nn), info = dev_alloc((size_t)nn * kInfo * 8);
-    DevPtr qa = dev_alloc((size_t)(nn + 2) * 16), qb = dev_alloc((size_t)(nn + 2) * 16);
+    DevPtr start = tmp(nn), info = tmp((size_t)nn * kInfo * 8);
+    DevPtr qa = tmp((size_t)(nn + 2) * 16), qb = tmp((size_t)(nn + 2) * 16);
     constexpr int kMaxLevels = 256;  // each level halves boxes at least once: depth < 64 + log2(nn)
-    DevPtr ctr = dev_alloc((kMaxLevels + 2) * sizeof(unsigned));  // [error bits, boxes per level...]
+    DevPtr ctr = tmp((kMaxLevels + 2) * sizeof(unsigned));  // [error bits, boxes per level...]
     FEASM_HIP(hipMemsetAsync(start->p, 0, nn, s));
     FEASM_HIP(hipMemsetAsync(ctr->p, 0, (kMaxLevels + 2) * sizeof(unsigned), s));
     const int64_t root[2] = {0, nn};
@@ -1282,35 +1297,35 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
     }
     qa.reset();
     qb.reset();
-    DevPtr inc = dev_alloc(nn * 4);
+    DevPtr inc = tmp(nn * 4);
     {
         size_t bytes = 0;
         FEASM_HIP(rocprim::inclusive_scan(nullptr, bytes, dp<uint8_t>(start), dp<int32_t>(inc), (size_t)nn,
                                           rocprim::plus<int32_t>(), s));
-        DevPtr tmp = dev_alloc(std::max<size_t>(bytes, 16));
-        FEASM_HIP(rocprim::inclusive_scan(tmp->p, bytes, dp<uint8_t>(start), dp<int32_t>(inc), (size_t)nn,
+        DevPtr scan_tmp = tmp(std::max<size_t>(bytes, 16));
+        FEASM_HIP(rocprim::inclusive_scan(scan_tmp->p, bytes, dp<uint8_t>(start), dp<int32_t>(inc), (size_t)nn,
                                           rocprim::plus<int32_t>(), s));
     }
     int32_t nt32 = 0;
     FEASM_HIP(hipMemcpyAsync(&nt32, dp<int32_t>(inc) + nn - 1, 4, hipMemcpyDeviceToHost, s));
     FEASM_HIP(hipStreamSynchronize(s));
     const int64_t ntiles = nt32;
-    DevPtr ts = dev_alloc((ntiles + 1) * 8);
+    DevPtr ts = tmp((ntiles + 1) * 8);
     hipLaunchKernelGGL(k_tile_starts, dim3(grid_for(nn, 256, 65536)), dim3(256), 0, s, dp<uint8_t>(start),
                        dp<int32_t>(inc), nn, dp<int64_t>(ts));
     FEASM_HIP(hipGetLastError());
     clk.lap("partition");
 
     // sizes, slots, LDS of the build kernel
-    DevPtr nh = dev_alloc((ntiles + 1) * 8), slot = dev_alloc((ntiles + 1) * 8);
-    DevPtr x0 = dev_alloc((ntiles + 1) * 8), slot_off = dev_alloc((ntiles + 1) * 8);
-    DevPtr pst = dev_alloc(sizeof(PrepStats));
+    DevPtr nh = tmp((ntiles + 1) * 8), slot = tmp((ntiles + 1) * 8);
+    DevPtr x0 = tmp((ntiles + 1) * 8), slot_off = tmp((ntiles + 1) * 8);
+    DevPtr pst = tmp(sizeof(PrepStats));
     FEASM_HIP(hipMemsetAsync(pst->p, 0, sizeof(PrepStats), s));
     hipLaunchKernelGGL(k_tile_prep, dim3(grid_for(ntiles + 1, 256, 1024)), dim3(256), 0, s, dp<int64_t>(ts), ntiles,
                        dp<int64_t>(info), shape, dp<int64_t>(nh), dp<int64_t>(slot), dp<PrepStats>(pst));
     FEASM_HIP(hipGetLastError());
-    scan_excl<int64_t>(s, dp<int64_t>(nh), dp<int64_t>(x0), ntiles + 1);
-    scan_excl<int64_t>(s, dp<int64_t>(slot), dp<int64_t>(slot_off), ntiles + 1);
+    scan_excl<int64_t>(s, dp<int64_t>(nh), dp<int64_t>(x0), ntiles + 1, hold);
+    scan_excl<int64_t>(s, dp<int64_t>(slot), dp<int64_t>(slot_off), ntiles + 1, hold);
     PrepStats ps;
     int64_t tot[2];
     FEASM_HIP(hipMemcpyAsync(&ps, pst->p, sizeof(ps), hipMemcpyDeviceToHost, s));
@@ -1331,14 +1346,14 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
     FEASM_REQUIRE(B.total + sizeof(EvalLds) + 64 <= (size_t)maxlds, FEASM_EUNSUPPORTED,
                   "tiles too large for the device tile builder");
     T->hnodes = dev_alloc(std::max<int64_t>(tot[0], 1) * 4);
-    DevPtr slots = dev_alloc(std::max<int64_t>(tot[1], 16));
-    DevPtr tout = dev_alloc(std::max<int64_t>(ntiles, 1) * 16);
+    DevPtr slots = tmp(std::max<int64_t>(tot[1], 16));
+    DevPtr tout = tmp(std::max<int64_t>(ntiles, 1) * 16);
     clk.lap("sizes");
 
     // 3. the tiles
     DevPtr tim;
     if (clk.on) {
-        tim = dev_alloc(16 * 8);
+        tim = tmp(16 * 8);
         FEASM_HIP(hipMemsetAsync(tim->p, 0, 16 * 8, s));
     }
     if (ntiles > 0) {
@@ -1368,18 +1383,18 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
     }
 
     // 4. packing
-    DevPtr nbytes = dev_alloc((ntiles + 1) * 8), boff = dev_alloc((ntiles + 1) * 8);
+    DevPtr nbytes = tmp((ntiles + 1) * 8), boff = tmp((ntiles + 1) * 8);
     hipLaunchKernelGGL(k_tile_sizes, dim3(grid_for(ntiles + 1, 256, 65536)), dim3(256), 0, s, dp<int32_t>(tout), ntiles,
                        dp<int64_t>(nbytes));
     FEASM_HIP(hipGetLastError());
-    scan_excl<int64_t>(s, dp<int64_t>(nbytes), dp<int64_t>(boff), ntiles + 1);
+    scan_excl<int64_t>(s, dp<int64_t>(nbytes), dp<int64_t>(boff), ntiles + 1, hold);
     int64_t nB = 0;
     FEASM_HIP(hipMemcpyAsync(&nB, dp<int64_t>(boff) + ntiles, 8, hipMemcpyDeviceToHost, s));
     FEASM_HIP(hipStreamSynchronize(s));
     T->blob = dev_alloc(std::max<int64_t>(nB, 16));
     if (nB < 16) FEASM_HIP(hipMemsetAsync(T->blob->p, 0, 16, s));
     T->hdr = dev_alloc(std::max<size_t>((size_t)ntiles * sizeof(TileHdr), 16));
-    DevPtr tst = dev_alloc(sizeof(TileStats));
+    DevPtr tst = tmp(sizeof(TileStats));
     FEASM_HIP(hipMemsetAsync(tst->p, 0, sizeof(TileStats), s));
     if (ntiles > 0) {
         hipLaunchKernelGGL(k_tile_pack, dim3((unsigned)ntiles), dim3(256), 0, s, dp<int64_t>(ts), dp<int64_t>(info),
