@@ -102,8 +102,7 @@ feasm_matrix *leading_block(feasm_ctx *ctx, const feasm_matrix *a, int64_t r, in
     FEASM_HIP(hipGetLastError());
     exclusive_scan_i32_to_i64(ctx, dp<int32_t>(cnt), dp<int64_t>(M->colptr), c + 1);
     int64_t nnz = 0;
-    FEASM_HIP(hipMemcpyAsync(&nnz, dp<int64_t>(M->colptr) + c, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    read_back(ctx->stream, &nnz, dp<int64_t>(M->colptr) + c, sizeof(int64_t));
     M->nnz = nnz;
     M->rowval = dev_alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t));
     M->nzval = dev_alloc(std::max<int64_t>(nnz, 1) * sizeof(double));
@@ -137,8 +136,7 @@ feasm_matrix *principal_submatrix(feasm_ctx *ctx, const feasm_matrix *a, const u
     M->colptr = dev_alloc((nf + 1) * sizeof(int64_t));
     exclusive_scan_i32_to_i64(ctx, dp<int32_t>(cnt), dp<int64_t>(M->colptr), nf + 1);
     int64_t nnz = 0;
-    FEASM_HIP(hipMemcpyAsync(&nnz, dp<int64_t>(M->colptr) + nf, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    read_back(ctx->stream, &nnz, dp<int64_t>(M->colptr) + nf, sizeof(int64_t));
     M->nnz = nnz;
     M->rowval = dev_alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t));
     M->nzval = dev_alloc(std::max<int64_t>(nnz, 1) * sizeof(double));

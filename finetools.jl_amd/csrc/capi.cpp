@@ -174,6 +174,7 @@ void feasm::numeric_into_ctx(feasm_ctx *ctx, feasm_mesh *m, const FormParams &fp
         }
     }
     if (tiles) {
+        if (m->pattern->bmult) m->pattern->bmult.reset();  // the staged path's contribution counts: not needed
         Timer tm(ctx);
         tm.mark(0);
         launch_fused(ctx, m, *m->pattern, *tiles, qt, fp, precision, nzval);

@@ -61,8 +61,7 @@ extern "C" int feasm_matrix_checksum(feasm_matrix *mat, uint64_t *digest) {
                            mat->nnz, dp<unsigned long long>(acc));
         FEASM_HIP(hipGetLastError());
         unsigned long long h = 0;
-        FEASM_HIP(hipMemcpyAsync(&h, acc->p, sizeof(h), hipMemcpyDeviceToHost, s));
-        FEASM_HIP(hipStreamSynchronize(s));
+        read_back(s, &h, acc->p, sizeof(h));
         *digest = (uint64_t)h;
         return FEASM_OK;
     } catch (const feasm::Error &e) {

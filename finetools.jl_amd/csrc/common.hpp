@@ -41,6 +41,13 @@ void pool_release(void *p, size_t bytes, int device);
 void pool_register_stream(int device, hipStream_t s);
 void pool_unregister_stream(int device, hipStream_t s);
 size_t pool_cached_bytes(int device);
+// the calling thread's buffers are used on stream s only (null: any
+// stream): blocks it releases are fenced on s alone
+void pool_set_thread_fence(int device, hipStream_t s);
+// copies bytes from device memory after the work queued on s, through a
+// pinned buffer of the calling thread (a copy into pageable memory goes
+// through the runtime's staging path, tens of microseconds a call)
+void read_back(hipStream_t s, void *host, const void *dev, size_t bytes);
 
 // Owning device allocation; shared so that a matrix can keep the pattern
 // arrays of its mesh alive.  bytes is the capacity (at least what was asked).
@@ -156,6 +163,7 @@ struct Pattern {
     feasm::DevPtr contrib;   // uint32 [ncontrib] (e << 7) | (pair << 1) | transposed
     feasm::DevPtr colptr;    // int64 [n+1]  DOF-level CSC column pointer (0-based)
     feasm::DevPtr rowval;    // int32 [nnz]  DOF-level CSC row index (0-based)
+    feasm::DevPtr bmult;     // uint16 [nblocks+1] contributions of each block (until the lists are built)
 };
 
 struct feasm_mesh {
@@ -247,6 +255,7 @@ feasm_matrix *sparse_device(feasm_ctx *ctx, int64_t ntrip, const int32_t *I, con
 
 // scan helpers (coo.hip)
 void exclusive_scan_i64(feasm_ctx *ctx, const int64_t *in, int64_t *out, int64_t n);
+void exclusive_scan_u16_to_i64(feasm_ctx *ctx, const uint16_t *in, int64_t *out, int64_t n);
 void exclusive_scan_i32_to_i64(feasm_ctx *ctx, const int32_t *in, int64_t *out, int64_t n);
 
 inline unsigned grid_for(int64_t n, int block, unsigned cap = 1u << 20) {

@@ -52,8 +52,7 @@ feasm_matrix *drop_zeros(feasm_ctx *ctx, const feasm_matrix *a) {
     M->colptr = dev_alloc((n + 1) * sizeof(int64_t));
     exclusive_scan_i32_to_i64(ctx, dp<int32_t>(cnt), dp<int64_t>(M->colptr), n + 1);
     int64_t nnz = 0;
-    FEASM_HIP(hipMemcpyAsync(&nnz, dp<int64_t>(M->colptr) + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    read_back(ctx->stream, &nnz, dp<int64_t>(M->colptr) + n, sizeof(int64_t));
     M->nnz = nnz;
     M->rowval = dev_alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t));
     M->nzval = dev_alloc(std::max<int64_t>(nnz, 1) * sizeof(double));

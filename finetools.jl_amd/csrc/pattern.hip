@@ -75,10 +75,108 @@ __global__ void k_n2e_sort(const int64_t *__restrict__ ptr, uint32_t *__restrict
 // adj[adj_ptr[c] ...].
 constexpr int kMaxCand = 2048;  // candidates of one node: incident elements x npe
 
+// Ascending bitonic sort of 64 E values held by one wavefront in
+// registers: lane l holds entries l + 64 m.  Partners less than 64 apart
+// are exchanged with lane shuffles, partners 64 or more apart sit in two
+// registers of the same lane.
+template <int E>
+__device__ __forceinline__ void wave_sort(int32_t (&x)[E], int lane) {
+    for (int k = 2; k <= 64 * E; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+                const int jj = j >> 6;
+#pragma unroll
+                for (int m = 0; m < E; m++) {
+                    const int mp1 = m ^ 1, mp2 = m ^ 2;
+                    const int mp = jj == 1 ? mp1 : mp2;
+                    if (mp > m) {
+                        const bool up = ((lane + 64 * m) & k) == 0;
+                        // both partners named with compile-time indices
+                        int32_t &a = x[m];
+                        int32_t &b = (jj == 1) ? x[mp1 < E ? mp1 : m] : x[mp2 < E ? mp2 : m];
+                        const int32_t lo = min(a, b), hi = max(a, b);
+                        a = up ? lo : hi;
+                        b = up ? hi : lo;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < E; m++) {
+                    const int i = lane + 64 * m;
+                    const int32_t y = __shfl_xor(x[m], j, 64);
+                    const bool take_min = ((i & k) == 0) == ((i & j) == 0);
+                    x[m] = take_min ? min(x[m], y) : max(x[m], y);
+                }
+            }
+        }
+}
+
+// The distinct values of a wave-sorted register list of L values (the pads
+// after them): their number; each value written to out[pos] and its number
+// of occurrences to mult[pos], pos its rank among the distinct values.
+template <int E>
+__device__ __forceinline__ int wave_unique(const int32_t (&x)[E], int lane, int L, int32_t *out, uint16_t *mult) {
+    uint64_t fm[E];
+    bool first[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        int32_t prev = __shfl_up(x[m], 1, 64);
+        if (m > 0) {
+            const int32_t last = __shfl(x[m > 0 ? m - 1 : 0], 63, 64);
+            if (lane == 0) prev = last;
+        }
+        const int i = lane + 64 * m;
+        first[m] = i < L && (i == 0 || x[m] != prev);
+        fm[m] = __ballot(first[m]);
+    }
+    int base = 0;
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        if (first[m]) {
+            const int i = lane + 64 * m;
+            // the next distinct value's first index (L after the last)
+            int nxt = L;
+            const uint64_t above = fm[m] & ~((2ull << lane) - 1ull);
+            if (above) {
+                nxt = 64 * m + __ffsll((unsigned long long)above) - 1;
+            } else {
+#pragma unroll
+                for (int q = E - 1; q > m; q--)
+                    if (fm[q]) nxt = 64 * q + __ffsll((unsigned long long)fm[q]) - 1;
+            }
+            const int pos = base + __popcll(fm[m] & ((1ull << lane) - 1ull));
+            out[pos] = x[m];
+            mult[pos] = (uint16_t)(nxt - i);
+        }
+        base += __popcll(fm[m]);
+    }
+    return base;
+}
+
+template <int E>
+__device__ __forceinline__ int adj_registers(const uint32_t *__restrict__ n2e, const int32_t *__restrict__ conn,
+                                             int npe, int64_t s0, int L, int lane, int32_t *out, uint16_t *mult) {
+    constexpr int32_t kPad = 0x7fffffff;
+    int32_t x[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        const int i = lane + 64 * m;
+        x[m] = i < L ? conn[(int64_t)(n2e[s0 + i / npe] >> 4) * npe + i % npe] : kPad;
+    }
+    wave_sort<E>(x, lane);
+    return wave_unique<E>(x, lane, L, out, mult);
+}
+
+// Node adjacency, one wavefront per node: the nodes of the node's elements,
+// sorted, each distinct one with its number of occurrences (one per
+// (local index of the node, local index of the neighbour) of every element
+// the two share: the block's contributions).  The list of
+// node c goes to tmp / tmult at npe * n2e_ptr[c] (it has at most that many
+// candidates' room), its length to deg[c]; k_adj_compact packs the lists.
 __global__ __launch_bounds__(64) void k_adj(const int64_t *__restrict__ n2e_ptr, const uint32_t *__restrict__ n2e,
                                             const int32_t *__restrict__ conn, int npe, int64_t nnodes,
-                                            const int64_t *__restrict__ adj_ptr, int32_t *__restrict__ deg,
-                                            int32_t *__restrict__ adj, int32_t *__restrict__ overflow) {
+                                            int32_t *__restrict__ deg, int32_t *__restrict__ tmp,
+                                            uint16_t *__restrict__ tmult, int32_t *__restrict__ overflow) {
     __shared__ int32_t cv[kMaxCand];
     __shared__ int32_t part[64];
     const int lane = threadIdx.x;
@@ -86,8 +184,20 @@ __global__ __launch_bounds__(64) void k_adj(const int64_t *__restrict__ n2e_ptr,
     for (int64_t c = blockIdx.x; c < nnodes; c += gridDim.x) {
         const int64_t s0 = n2e_ptr[c];
         int L = (int)((n2e_ptr[c + 1] - s0) * npe);
+        int32_t *out = tmp + (int64_t)npe * s0;
+        uint16_t *mout = tmult + (int64_t)npe * s0;
+        if (L <= 256) {  // the candidates fit in the wavefront's registers
+            const int tot = L <= 64    ? adj_registers<1>(n2e, conn, npe, s0, L, lane, out, mout)
+                            : L <= 128 ? adj_registers<2>(n2e, conn, npe, s0, L, lane, out, mout)
+                                       : adj_registers<4>(n2e, conn, npe, s0, L, lane, out, mout);
+            if (lane == 0) {
+                if (tot > kCap) atomicOr(overflow, 1);
+                deg[c] = tot > kCap ? 0 : tot;
+            }
+            continue;
+        }
         if (L > kMaxCand) {
-            if (lane == 0) *overflow = 1;
+            if (lane == 0) atomicOr(overflow, 1);
             L = 0;
         }
         int P = 1;
@@ -128,18 +238,40 @@ __global__ __launch_bounds__(64) void k_adj(const int64_t *__restrict__ n2e_ptr,
             tot += pq;
         }
         if (tot > kCap) {
-            if (lane == 0) *overflow = 1;
-        } else if (adj_ptr == nullptr) {
-            if (lane == 0) deg[c] = tot;
+            if (lane == 0) {
+                atomicOr(overflow, 1);
+                deg[c] = 0;
+            }
         } else {
-            const int64_t o = adj_ptr[c] + off;
-            int w = 0;
+            int w = off;
             for (int i = beg; i < end; i++) {
                 const int32_t v = cv[i];
-                if (v != kPad && (i == 0 || cv[i - 1] != v)) adj[o + w++] = v;
+                if (v != kPad && (i == 0 || cv[i - 1] != v)) {
+                    int r = i + 1;
+                    while (r < L && cv[r] == v) r++;
+                    out[w] = v;
+                    mout[w] = (uint16_t)(r - i);
+                    w++;
+                }
             }
+            if (lane == 0) deg[c] = tot;
         }
         __syncthreads();  // cv and part are refilled for the next node
+    }
+}
+
+// packs the lists of k_adj: node c's at adj_ptr[c]
+__global__ __launch_bounds__(64) void k_adj_compact(const int64_t *__restrict__ n2e_ptr, int npe, int64_t nnodes,
+                                                    const int64_t *__restrict__ adj_ptr, const int32_t *__restrict__ tmp,
+                                                    const uint16_t *__restrict__ tmult, int32_t *__restrict__ adj,
+                                                    uint16_t *__restrict__ bmult) {
+    for (int64_t c = blockIdx.x; c < nnodes; c += gridDim.x) {
+        const int64_t src = (int64_t)npe * n2e_ptr[c], dst = adj_ptr[c];
+        const int d = (int)(adj_ptr[c + 1] - dst);
+        for (int i = threadIdx.x; i < d; i += 64) {
+            adj[dst + i] = tmp[src + i];
+            bmult[dst + i] = tmult[src + i];
+        }
     }
 }
 
@@ -166,20 +298,23 @@ __global__ __launch_bounds__(256) void k_contrib(const int32_t *__restrict__ bco
         int64_t w = cptr ? cptr[t] : 0;
         int64_t j = n2e_ptr[m];
         const int64_t je = n2e_ptr[m + 1];
+        // every (local index of c, local index of m) of every shared
+        // element: an element that lists a node twice contributes each pair
         for (int64_t s = n2e_ptr[c]; s < n2e_ptr[c + 1]; s++) {
             const uint32_t u = n2e[s];
             const uint32_t e = u >> 4;
             while (j < je && (n2e[j] >> 4) < e) j++;
             if (j == je) break;
-            const uint32_t v = n2e[j];
-            if ((v >> 4) != e) continue;
-            if (cptr) {
-                const int ac = (int)(u & 15u), am = (int)(v & 15u);
-                const int lo = am < ac ? am : ac, hi = am < ac ? ac : am;
-                const uint32_t p = (uint32_t)(hi * (hi + 1) / 2 + lo);
-                contrib[w++] = (e << 7) | (p << 1) | (am > ac ? 1u : 0u);
+            for (int64_t q = j; q < je && (n2e[q] >> 4) == e; q++) {
+                if (cptr) {
+                    const uint32_t v = n2e[q];
+                    const int ac = (int)(u & 15u), am = (int)(v & 15u);
+                    const int lo = am < ac ? am : ac, hi = am < ac ? ac : am;
+                    const uint32_t p = (uint32_t)(hi * (hi + 1) / 2 + lo);
+                    contrib[w++] = (e << 7) | (p << 1) | (am > ac ? 1u : 0u);
+                }
+                cnt++;
             }
-            cnt++;
         }
         if (!cptr) ccount[t] = cnt;
     }
@@ -208,14 +343,22 @@ __global__ __launch_bounds__(256) void k_dof_pattern(const int64_t *__restrict__
 
 int64_t read_i64(feasm_ctx *ctx, const int64_t *d) {
     int64_t v = 0;
-    FEASM_HIP(hipMemcpyAsync(&v, d, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
-    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    read_back(ctx->stream, &v, d, sizeof(v));
     return v;
 }
 
 }  // namespace
 
 void exclusive_scan_i32_to_i64(feasm_ctx *ctx, const int32_t *in, int64_t *out, int64_t n) {
+    size_t bytes = 0;
+    FEASM_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(),
+                                      ctx->stream));
+    void *tmp = ctx->scratch_get(bytes);
+    FEASM_HIP(
+        rocprim::exclusive_scan(tmp, bytes, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), ctx->stream));
+}
+
+void exclusive_scan_u16_to_i64(feasm_ctx *ctx, const uint16_t *in, int64_t *out, int64_t n) {
     size_t bytes = 0;
     FEASM_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(),
                                       ctx->stream));
@@ -241,8 +384,7 @@ bool conn_in_range(feasm_ctx *ctx, const int32_t *d_conn, int64_t n, int64_t nno
                        dp<unsigned int>(flag));
     FEASM_HIP(hipGetLastError());
     unsigned int bad = 0;
-    FEASM_HIP(hipMemcpyAsync(&bad, flag->p, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
-    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    read_back(ctx->stream, &bad, flag->p, sizeof(bad));
     return bad == 0;
 }
 
@@ -272,14 +414,20 @@ void build_contributions(feasm_mesh *m, Pattern &P_) {
     feasm_ctx *ctx = m->ctx;
     Pattern *P = &P_;
     if (P->contrib) return;
-    DevPtr ccount = dev_alloc((P->nblocks + 1) * sizeof(int32_t));
-    FEASM_HIP(hipMemsetAsync(ccount->p, 0, (P->nblocks + 1) * sizeof(int32_t), ctx->stream));
-    hipLaunchKernelGGL(k_contrib, dim3(grid_for(P->nblocks, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(P->bcol),
-                       dp<int32_t>(P->adj), P->nblocks, dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e),
-                       (const int64_t *)nullptr, dp<int32_t>(ccount), (uint32_t *)nullptr);
-    FEASM_HIP(hipGetLastError());
     P->cptr = dev_alloc((P->nblocks + 1) * sizeof(int64_t));
-    exclusive_scan_i32_to_i64(ctx, dp<int32_t>(ccount), dp<int64_t>(P->cptr), P->nblocks + 1);
+    if (P->bmult) {
+        // a block's contributions are its row node's occurrences among the
+        // column node's elements' nodes, counted by the adjacency pass
+        exclusive_scan_u16_to_i64(ctx, dp<uint16_t>(P->bmult), dp<int64_t>(P->cptr), P->nblocks + 1);
+    } else {
+        DevPtr ccount = dev_alloc((P->nblocks + 1) * sizeof(int32_t));
+        FEASM_HIP(hipMemsetAsync(ccount->p, 0, (P->nblocks + 1) * sizeof(int32_t), ctx->stream));
+        hipLaunchKernelGGL(k_contrib, dim3(grid_for(P->nblocks, 256)), dim3(256), 0, ctx->stream,
+                           dp<int32_t>(P->bcol), dp<int32_t>(P->adj), P->nblocks, dp<int64_t>(m->n2e_ptr),
+                           dp<uint32_t>(m->n2e), (const int64_t *)nullptr, dp<int32_t>(ccount), (uint32_t *)nullptr);
+        FEASM_HIP(hipGetLastError());
+        exclusive_scan_i32_to_i64(ctx, dp<int32_t>(ccount), dp<int64_t>(P->cptr), P->nblocks + 1);
+    }
     P->ncontrib = read_i64(ctx, dp<int64_t>(P->cptr) + P->nblocks);
     DevPtr contrib = dev_alloc(std::max<size_t>(P->ncontrib * sizeof(uint32_t), 4));
     hipLaunchKernelGGL(k_contrib, dim3(grid_for(P->nblocks, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(P->bcol),
@@ -288,6 +436,7 @@ void build_contributions(feasm_mesh *m, Pattern &P_) {
     FEASM_HIP(hipGetLastError());
     FEASM_HIP(hipStreamSynchronize(ctx->stream));
     P->contrib = contrib;
+    P->bmult.reset();
 }
 
 std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn) {
@@ -297,27 +446,34 @@ std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn) {
     P->ndn = ndn;
     P->nnodes = m->nnodes;
     const int64_t nn = m->nnodes;
-    // 2. adjacency
+    // 2. adjacency: one pass into per-node room, then packed
+    const int npe = m->npe;
+    const int64_t nent = m->nelem * npe;  // node -> element entries
     DevPtr deg = dev_alloc((nn + 1) * sizeof(int32_t));
-    DevPtr overflow = dev_alloc(sizeof(int32_t));
-    FEASM_HIP(hipMemsetAsync(overflow->p, 0, sizeof(int32_t), ctx->stream));
-    FEASM_HIP(hipMemsetAsync(deg->p, 0, (nn + 1) * sizeof(int32_t), ctx->stream));
+    DevPtr flags = dev_alloc(sizeof(int32_t));  // too many neighbours
+    DevPtr tmp = dev_alloc(std::max<size_t>((size_t)npe * nent * sizeof(int32_t), 4));
+    DevPtr tmult = dev_alloc(std::max<size_t>((size_t)npe * nent * sizeof(uint16_t), 4));
+    FEASM_HIP(hipMemsetAsync(flags->p, 0, sizeof(int32_t), ctx->stream));
+    FEASM_HIP(hipMemsetAsync(dp<int32_t>(deg) + nn, 0, sizeof(int32_t), ctx->stream));
     const unsigned gadj = grid_for(nn, 1, 1 << 17);
-    hipLaunchKernelGGL(k_adj, dim3(gadj), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr),
-                       dp<uint32_t>(m->n2e), dp<int32_t>(m->conn), m->npe, nn, (const int64_t *)nullptr,
-                       dp<int32_t>(deg), (int32_t *)nullptr, dp<int32_t>(overflow));
+    hipLaunchKernelGGL(k_adj, dim3(gadj), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e),
+                       dp<int32_t>(m->conn), npe, nn, dp<int32_t>(deg), dp<int32_t>(tmp), dp<uint16_t>(tmult),
+                       dp<int32_t>(flags));
     FEASM_HIP(hipGetLastError());
     P->adj_ptr = dev_alloc((nn + 1) * sizeof(int64_t));
     exclusive_scan_i32_to_i64(ctx, dp<int32_t>(deg), dp<int64_t>(P->adj_ptr), nn + 1);
     P->nblocks = read_i64(ctx, dp<int64_t>(P->adj_ptr) + nn);
-    int32_t ovf = 0;
-    FEASM_HIP(hipMemcpy(&ovf, overflow->p, sizeof(int32_t), hipMemcpyDeviceToHost));
-    FEASM_REQUIRE(ovf == 0, FEASM_EUNSUPPORTED,
+    int32_t fl = 0;
+    read_back(ctx->stream, &fl, flags->p, sizeof(int32_t));
+    FEASM_REQUIRE(!(fl & 1), FEASM_EUNSUPPORTED,
                   "a node has more than 256 distinct neighbours or more than 2048 (element, node) incidences");
-    P->adj = dev_alloc(P->nblocks * sizeof(int32_t));
-    hipLaunchKernelGGL(k_adj, dim3(gadj), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr),
-                       dp<uint32_t>(m->n2e), dp<int32_t>(m->conn), m->npe, nn, dp<int64_t>(P->adj_ptr),
-                       (int32_t *)nullptr, dp<int32_t>(P->adj), dp<int32_t>(overflow));
+    P->adj = dev_alloc(std::max<size_t>(P->nblocks * sizeof(int32_t), 4));
+    P->bmult = dev_alloc((P->nblocks + 1) * sizeof(uint16_t));
+    FEASM_HIP(hipMemsetAsync(dp<uint16_t>(P->bmult) + P->nblocks, 0, sizeof(uint16_t), ctx->stream));
+    hipLaunchKernelGGL(k_adj_compact, dim3(gadj), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr), npe, nn,
+                       dp<int64_t>(P->adj_ptr), dp<int32_t>(tmp), dp<uint16_t>(tmult), dp<int32_t>(P->adj),
+                       dp<uint16_t>(P->bmult));
+    FEASM_HIP(hipGetLastError());
     P->bcol = dev_alloc(P->nblocks * sizeof(int32_t));
     hipLaunchKernelGGL(k_bcol, dim3(grid_for(nn, 256)), dim3(256), 0, ctx->stream, dp<int64_t>(P->adj_ptr), nn,
                        dp<int32_t>(P->bcol));

@@ -9,8 +9,11 @@
 // Work on other streams that touches library buffers must be complete before
 // the buffer's owner is destroyed (the library's own entry points synchronise
 // or stay on their context's stream).
+#include <algorithm>
+#include <cstring>
 #include <map>
 #include <mutex>
+#include <shared_mutex>
 
 #include "common.hpp"
 
@@ -25,8 +28,14 @@ struct Block {
     bool unfenced = false;  // some stream could not be fenced: freed only after a device synchronisation
 };
 
+thread_local hipStream_t t_fence_stream = nullptr;
+thread_local int t_fence_device = -1;
+hipStream_t thread_fence_stream() { return t_fence_stream; }
+int thread_fence_device() { return t_fence_device; }
+
 struct Pool {
-    std::mutex mu;
+    std::mutex mu;           // the cache and the spare events
+    std::shared_mutex smu;   // the stream list: shared while fencing, exclusive to change it
     std::multimap<size_t, Block> cached;  // by capacity
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> spare;
@@ -137,43 +146,63 @@ void pool_release(void *p, size_t capacity, int device) {
     if (!p) return;
     DeviceScope scope(device);
     Pool &P = pool(device);
-    std::lock_guard<std::mutex> lk(P.mu);
     Block b;
     b.p = p;
     b.bytes = capacity;
-    for (hipStream_t s : P.streams) {
-        // a stream being captured into a graph cannot take the fence
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
-            (void)hipGetLastError();
-            b.unfenced = true;
-            continue;
+    {
+        // the fences are recorded outside the cache lock (other threads keep
+        // allocating meanwhile); the shared lock keeps every stream
+        // registered, and so alive, while it is fenced
+        std::shared_lock<std::shared_mutex> sl(P.smu);
+        // a thread that declared the one stream its buffers are used on fences
+        // that stream only (pool_set_thread_fence); others fence every stream
+        const hipStream_t one[1] = {thread_fence_stream()};
+        const bool only = one[0] != nullptr && thread_fence_device() == device;
+        const hipStream_t *fs = only ? one : P.streams.data();
+        const size_t nfs = only ? 1 : P.streams.size();
+        for (size_t k = 0; k < nfs; k++) {
+            const hipStream_t s = fs[k];
+            // a stream being captured into a graph cannot take the fence
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+                (void)hipGetLastError();
+                b.unfenced = true;
+                continue;
+            }
+            hipEvent_t e = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(P.mu);
+                if (!P.spare.empty()) {
+                    e = P.spare.back();
+                    P.spare.pop_back();
+                }
+            }
+            if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                (void)hipGetLastError();
+                (void)hipDeviceSynchronize();  // cannot fence: free it outright
+                (void)hipFree(p);
+                std::lock_guard<std::mutex> lk(P.mu);
+                recycle_events(P, b);
+                return;
+            }
+            if (hipEventRecord(e, s) != hipSuccess) {
+                (void)hipGetLastError();
+                b.unfenced = true;
+                std::lock_guard<std::mutex> lk(P.mu);
+                P.spare.push_back(e);
+                continue;
+            }
+            b.evs.push_back(e);
         }
-        hipEvent_t e;
-        if (!P.spare.empty()) {
-            e = P.spare.back();
-            P.spare.pop_back();
-        } else if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-            (void)hipGetLastError();
-            (void)hipDeviceSynchronize();  // cannot fence: free it outright
-            (void)hipFree(p);
-            recycle_events(P, b);
-            return;
-        }
-        if (hipEventRecord(e, s) != hipSuccess) {
-            (void)hipGetLastError();
-            b.unfenced = true;
-            P.spare.push_back(e);
-            continue;
-        }
-        b.evs.push_back(e);
     }
+    std::lock_guard<std::mutex> lk(P.mu);
     P.cached.emplace(capacity, std::move(b));
     P.cached_bytes += capacity;
 }
 
 void pool_register_stream(int device, hipStream_t s) {
     Pool &P = pool(device);
+    std::unique_lock<std::shared_mutex> sl(P.smu);
     std::lock_guard<std::mutex> lk(P.mu);
     P.streams.push_back(s);
 }
@@ -181,6 +210,7 @@ void pool_register_stream(int device, hipStream_t s) {
 void pool_unregister_stream(int device, hipStream_t s) {
     DeviceScope scope(device);
     Pool &P = pool(device);
+    std::unique_lock<std::shared_mutex> sl(P.smu);
     std::lock_guard<std::mutex> lk(P.mu);
     // the stream is idle (its owner synchronised it): its fences are complete
     for (size_t i = 0; i < P.streams.size(); i++)
@@ -189,6 +219,57 @@ void pool_unregister_stream(int device, hipStream_t s) {
             break;
         }
     if (P.streams.empty()) trim_locked(P, false);
+}
+
+void pool_set_thread_fence(int device, hipStream_t s) {
+    t_fence_stream = s;
+    t_fence_device = s ? device : -1;
+}
+
+namespace {
+// pinned host buffers of read_back, one per host thread; a thread's buffer
+// goes back to a free list when the thread ends (never freed: the HIP runtime
+// may be gone when the last threads end)
+struct Pinned {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_pinned_mu;
+std::vector<Pinned> g_pinned_free;
+struct PinnedHolder {
+    Pinned b;
+    ~PinnedHolder() {
+        if (!b.p) return;
+        std::lock_guard<std::mutex> lk(g_pinned_mu);
+        g_pinned_free.push_back(b);
+    }
+};
+thread_local PinnedHolder t_pinned;
+}  // namespace
+
+void read_back(hipStream_t s, void *host, const void *dev, size_t bytes) {
+    Pinned &b = t_pinned.b;
+    if (b.bytes < bytes) {
+        Pinned nb;
+        {
+            std::lock_guard<std::mutex> lk(g_pinned_mu);
+            for (size_t i = 0; i < g_pinned_free.size(); i++)
+                if (g_pinned_free[i].bytes >= bytes) {
+                    nb = g_pinned_free[i];
+                    g_pinned_free.erase(g_pinned_free.begin() + i);
+                    break;
+                }
+            if (b.p) g_pinned_free.push_back(b);  // too small for this thread now; another may use it
+        }
+        if (!nb.p) {
+            nb.bytes = std::max<size_t>(bytes, 4096);
+            FEASM_HIP(hipHostMalloc(&nb.p, nb.bytes, hipHostMallocDefault));
+        }
+        b = nb;
+    }
+    FEASM_HIP(hipMemcpyAsync(b.p, dev, bytes, hipMemcpyDeviceToHost, s));
+    FEASM_HIP(hipStreamSynchronize(s));
+    std::memcpy(host, b.p, bytes);
 }
 
 size_t pool_cached_bytes(int device) {

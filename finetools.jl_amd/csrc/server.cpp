@@ -15,6 +15,7 @@
 // Results are deterministic: every kernel sums in a fixed order, so a request
 // gives bitwise the same matrix alone or in any mix.
 #include <algorithm>
+#include <cstring>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -116,9 +117,34 @@ struct feasm_server {
         meshes.clear();
     }
 
+    // pinned host memory for a worker's mesh uploads (a copy from pageable
+    // memory goes through the runtime's staging path)
+    struct Stage {
+        void *p = nullptr;
+        size_t bytes = 0;
+        void *get(size_t n) {  // the previous request's copies have completed
+            if (n > bytes) {
+                if (p) (void)hipHostFree(p);
+                p = nullptr;
+                bytes = 0;
+                FEASM_HIP(hipHostMalloc(&p, n, hipHostMallocDefault));
+                bytes = n;
+            }
+            return p;
+        }
+        ~Stage() {
+            if (p) (void)hipHostFree(p);
+        }
+    };
+
     void work(int w) {
         feasm_ctx *ctx = ctxs[w];
         (void)hipSetDevice(device);
+        // every buffer this worker releases was last used on its own stream
+        // (a resident mesh's requests synchronise before they finish), so the
+        // memory pool fences that stream alone
+        pool_set_thread_fence(device, ctx->stream);
+        Stage stage;
         for (;;) {
             Request *r = nullptr;
             std::vector<void *> evicted;
@@ -143,7 +169,7 @@ struct feasm_server {
             release_meshes(evicted);
             if (quit) return;
             if (!r) continue;
-            run_one(ctx, r);  // publishes the outcome and the bookkeeping under one lock
+            run_one(ctx, r, stage);  // publishes the outcome and the bookkeeping under one lock
             cv_work.notify_all();
             cv_done.notify_all();
         }
@@ -172,7 +198,7 @@ struct feasm_server {
         r->xyz.shrink_to_fit();
     }
 
-    void run_one(feasm_ctx *ctx, Request *r) {
+    void run_one(feasm_ctx *ctx, Request *r, Stage &stage) {
         std::unique_ptr<feasm_mesh> owned;
         try {
             feasm_mesh *m = nullptr;
@@ -192,23 +218,29 @@ struct feasm_server {
                 m->npe = r->npe;
                 m->nelem = r->nelem;
                 m->nnodes = r->nnodes;
-                if (r->bconn) {  // borrowed arrays: converted and checked here, in the worker
-                    r->conn0.resize(r->nelem * r->npe);
+                // the arrays go through the worker's pinned staging memory;
+                // borrowed ones are converted and checked here, in the worker
+                const size_t cb = (size_t)(r->nelem * r->npe) * 4, xb = (size_t)r->nnodes * 24;
+                unsigned char *st = static_cast<unsigned char *>(stage.get(((cb + 15) & ~size_t(15)) + xb + 16));
+                int32_t *sc = reinterpret_cast<int32_t *>(st);
+                double *sx = reinterpret_cast<double *>(st + ((cb + 15) & ~size_t(15)));
+                if (r->bconn) {
                     for (int64_t i = 0; i < r->nelem * r->npe; i++) {
                         FEASM_REQUIRE(r->bconn[i] >= 1 && r->bconn[i] <= r->nnodes, FEASM_EINVAL,
                                       "connectivity refers to a missing node");
-                        r->conn0[i] = (int32_t)(r->bconn[i] - 1);
+                        sc[i] = (int32_t)(r->bconn[i] - 1);
                     }
-                    r->xyz.assign(r->bxyz, r->bxyz + 3 * r->nnodes);
+                    if (xb) std::memcpy(sx, r->bxyz, xb);
                     r->bconn = nullptr;
                     r->bxyz = nullptr;
+                } else {
+                    if (cb) std::memcpy(sc, r->conn0.data(), cb);
+                    if (xb) std::memcpy(sx, r->xyz.data(), xb);
                 }
-                m->conn = dev_alloc(std::max<size_t>(r->conn0.size() * 4, 4));
-                m->xyz = dev_alloc(std::max<size_t>(r->xyz.size() * 8, 8));
-                FEASM_HIP(hipMemcpyAsync(m->conn->p, r->conn0.data(), r->conn0.size() * 4, hipMemcpyHostToDevice,
-                                         ctx->stream));
-                FEASM_HIP(hipMemcpyAsync(m->xyz->p, r->xyz.data(), r->xyz.size() * 8, hipMemcpyHostToDevice,
-                                         ctx->stream));
+                m->conn = dev_alloc(std::max<size_t>(cb, 4));
+                m->xyz = dev_alloc(std::max<size_t>(xb, 8));
+                if (cb) FEASM_HIP(hipMemcpyAsync(m->conn->p, sc, cb, hipMemcpyHostToDevice, ctx->stream));
+                if (xb) FEASM_HIP(hipMemcpyAsync(m->xyz->p, sx, xb, hipMemcpyHostToDevice, ctx->stream));
                 m->ndn = r->ndn;
                 m->nalldofs = r->nnodes * r->ndn;
                 m->nodemajor = true;

@@ -1239,8 +1239,7 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, s, dp<double>(bb), (int)nbb, dp<double>(bb) + 6 * nbb);
     FEASM_HIP(hipGetLastError());
     double hb[6];
-    FEASM_HIP(hipMemcpyAsync(hb, dp<double>(bb) + 6 * nbb, sizeof(hb), hipMemcpyDeviceToHost, s));
-    FEASM_HIP(hipStreamSynchronize(s));
+    read_back(s, hb, dp<double>(bb) + 6 * nbb, sizeof(hb));
     const double lo[3] = {hb[0], hb[1], hb[2]}, hi[3] = {hb[3], hb[4], hb[5]};
     const double step = morton_step(lo, hi, nn);
     clk.lap("bounding box");
@@ -1289,9 +1288,8 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
             FEASM_HIP(hipGetLastError());
             std::swap(qa, qb);
         }
-        FEASM_HIP(hipMemcpyAsync(&hc[0], d_err, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        FEASM_HIP(hipMemcpyAsync(&hc[1], d_count + levels, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        FEASM_HIP(hipStreamSynchronize(s));
+        read_back(s, &hc[0], d_err, sizeof(unsigned));
+        read_back(s, &hc[1], d_count + levels, sizeof(unsigned));
         raise_tile_errors(hc[0]);
         more = hc[1] > 0;
     }
@@ -1307,8 +1305,7 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
                                           rocprim::plus<int32_t>(), s));
     }
     int32_t nt32 = 0;
-    FEASM_HIP(hipMemcpyAsync(&nt32, dp<int32_t>(inc) + nn - 1, 4, hipMemcpyDeviceToHost, s));
-    FEASM_HIP(hipStreamSynchronize(s));
+    read_back(s, &nt32, dp<int32_t>(inc) + nn - 1, 4);
     const int64_t ntiles = nt32;
     DevPtr ts = tmp((ntiles + 1) * 8);
     hipLaunchKernelGGL(k_tile_starts, dim3(grid_for(nn, 256, 65536)), dim3(256), 0, s, dp<uint8_t>(start),
@@ -1328,10 +1325,9 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
     scan_excl<int64_t>(s, dp<int64_t>(slot), dp<int64_t>(slot_off), ntiles + 1, hold);
     PrepStats ps;
     int64_t tot[2];
-    FEASM_HIP(hipMemcpyAsync(&ps, pst->p, sizeof(ps), hipMemcpyDeviceToHost, s));
-    FEASM_HIP(hipMemcpyAsync(&tot[0], dp<int64_t>(x0) + ntiles, 8, hipMemcpyDeviceToHost, s));
-    FEASM_HIP(hipMemcpyAsync(&tot[1], dp<int64_t>(slot_off) + ntiles, 8, hipMemcpyDeviceToHost, s));
-    FEASM_HIP(hipStreamSynchronize(s));
+    read_back(s, &ps, pst->p, sizeof(ps));
+    read_back(s, &tot[0], dp<int64_t>(x0) + ntiles, 8);
+    read_back(s, &tot[1], dp<int64_t>(slot_off) + ntiles, 8);
     FEASM_REQUIRE(tot[0] < INT32_MAX, FEASM_EUNSUPPORTED, "mesh too large for 32-bit tile offsets");
     BuildCaps caps;
     caps.ntu = (int)ps.ntu_max;
@@ -1366,8 +1362,7 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
                            dp<unsigned long long>(tim));
         FEASM_HIP(hipGetLastError());
     }
-    FEASM_HIP(hipMemcpyAsync(&hc[0], d_err, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    FEASM_HIP(hipStreamSynchronize(s));
+    read_back(s, &hc[0], d_err, sizeof(unsigned));
     raise_tile_errors(hc[0]);
     clk.lap("tile arrays");
     if (clk.on) {
@@ -1389,8 +1384,7 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
     FEASM_HIP(hipGetLastError());
     scan_excl<int64_t>(s, dp<int64_t>(nbytes), dp<int64_t>(boff), ntiles + 1, hold);
     int64_t nB = 0;
-    FEASM_HIP(hipMemcpyAsync(&nB, dp<int64_t>(boff) + ntiles, 8, hipMemcpyDeviceToHost, s));
-    FEASM_HIP(hipStreamSynchronize(s));
+    read_back(s, &nB, dp<int64_t>(boff) + ntiles, 8);
     T->blob = dev_alloc(std::max<int64_t>(nB, 16));
     if (nB < 16) FEASM_HIP(hipMemsetAsync(T->blob->p, 0, 16, s));
     T->hdr = dev_alloc(std::max<size_t>((size_t)ntiles * sizeof(TileHdr), 16));
@@ -1406,8 +1400,7 @@ std::shared_ptr<TileSet> build_tiles_device(feasm_mesh *m, const TileShape &shap
         FEASM_HIP(hipGetLastError());
     }
     TileStats st;
-    FEASM_HIP(hipMemcpyAsync(&st, tst->p, sizeof(st), hipMemcpyDeviceToHost, s));
-    FEASM_HIP(hipStreamSynchronize(s));
+    read_back(s, &st, tst->p, sizeof(st));
     clk.lap("packing");
     T->ntiles = ntiles;
     T->nhalo = (int64_t)st.nhalo;
