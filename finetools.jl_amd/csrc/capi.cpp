@@ -546,6 +546,7 @@ extern "C" int feasm_pattern_build(feasm_mesh *m) {
     FEASM_REQUIRE(m->nodemajor, FEASM_EUNSUPPORTED, "resident pattern needs the node-major numbering");
     m->pattern.reset();
     ensure_pattern(m);
+    FEASM_HIP(hipStreamSynchronize(m->ctx->stream));  // the pattern is complete when the call returns
     API_CATCH(m ? m->ctx : nullptr)
 }
 

@@ -407,7 +407,8 @@ void build_node_to_element(feasm_mesh *m) {
     hipLaunchKernelGGL(k_n2e_sort, dim3(grid_for(m->nnodes, 256)), dim3(256), 0, ctx->stream,
                        dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e), m->nnodes);
     FEASM_HIP(hipGetLastError());
-    FEASM_HIP(hipStreamSynchronize(ctx->stream));
+    // no wait: the arrays are read by later work on the same stream (the
+    // temporaries above are released behind a fence on it)
 }
 
 void build_contributions(feasm_mesh *m, Pattern &P_) {
@@ -428,13 +429,13 @@ void build_contributions(feasm_mesh *m, Pattern &P_) {
         FEASM_HIP(hipGetLastError());
         exclusive_scan_i32_to_i64(ctx, dp<int32_t>(ccount), dp<int64_t>(P->cptr), P->nblocks + 1);
     }
-    P->ncontrib = read_i64(ctx, dp<int64_t>(P->cptr) + P->nblocks);
+    // every (element, local column, local row) is one contribution
+    P->ncontrib = m->nelem * m->npe * m->npe;
     DevPtr contrib = dev_alloc(std::max<size_t>(P->ncontrib * sizeof(uint32_t), 4));
     hipLaunchKernelGGL(k_contrib, dim3(grid_for(P->nblocks, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(P->bcol),
                        dp<int32_t>(P->adj), P->nblocks, dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e),
                        dp<int64_t>(P->cptr), (int32_t *)nullptr, dp<uint32_t>(contrib));
     FEASM_HIP(hipGetLastError());
-    FEASM_HIP(hipStreamSynchronize(ctx->stream));
     P->contrib = contrib;
     P->bmult.reset();
 }
@@ -488,7 +489,6 @@ std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn) {
     hipLaunchKernelGGL(k_dof_pattern, dim3(grid_for((nn + 3) / 4, 1, 65536)), dim3(256), 0, ctx->stream, dp<int64_t>(P->adj_ptr),
                        dp<int32_t>(P->adj), nn, ndn, dp<int64_t>(P->colptr), dp<int32_t>(P->rowval));
     FEASM_HIP(hipGetLastError());
-    FEASM_HIP(hipStreamSynchronize(ctx->stream));
     return P;
 }
 

@@ -244,8 +244,7 @@ struct feasm_server {
                 m->ndn = r->ndn;
                 m->nalldofs = r->nnodes * r->ndn;
                 m->nodemajor = true;
-                FEASM_HIP(hipStreamSynchronize(ctx->stream));
-                m->pattern = build_pattern(m, m->ndn);
+                m->pattern = build_pattern(m, m->ndn);  // its read-back waits for the uploads
             }
             std::unique_ptr<feasm_matrix> M(new feasm_matrix);
             M->ctx = ctx;
