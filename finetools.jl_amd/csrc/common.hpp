@@ -228,7 +228,9 @@ feasm_matrix *symm_plus_transpose(feasm_ctx *ctx, const feasm_matrix *S);
 
 // pattern.hip
 void build_node_to_element(feasm_mesh *m);
-std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn);
+// with_contrib: the staged path will run first, so the adjacency pass also
+// writes the contribution lists (build_contributions then only scans)
+std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn, bool with_contrib = false);
 // every entry of the device connectivity d_conn[0..n) lies in [0, nnodes)
 bool conn_in_range(feasm_ctx *ctx, const int32_t *d_conn, int64_t n, int64_t nnodes);
 void build_contributions(feasm_mesh *m, Pattern &p);

@@ -111,10 +111,11 @@ __device__ __forceinline__ void wave_sort(int32_t (&x)[E], int lane) {
         }
 }
 
-// The distinct values of a wave-sorted register list of L values (the pads
-// after them): their number; each value written to out[pos] and its number
-// of occurrences to mult[pos], pos its rank among the distinct values.
-template <int E>
+// The distinct values v = x >> S of a wave-sorted register list of L keys
+// x (the pads after them): their number; each value written to out[pos] and
+// its number of occurrences to mult[pos], pos its rank among the distinct
+// values.
+template <int E, int S>
 __device__ __forceinline__ int wave_unique(const int32_t (&x)[E], int lane, int L, int32_t *out, uint16_t *mult) {
     uint64_t fm[E];
     bool first[E];
@@ -126,7 +127,7 @@ __device__ __forceinline__ int wave_unique(const int32_t (&x)[E], int lane, int 
             if (lane == 0) prev = last;
         }
         const int i = lane + 64 * m;
-        first[m] = i < L && (i == 0 || x[m] != prev);
+        first[m] = i < L && (i == 0 || (x[m] >> S) != (prev >> S));
         fm[m] = __ballot(first[m]);
     }
     int base = 0;
@@ -145,7 +146,7 @@ __device__ __forceinline__ int wave_unique(const int32_t (&x)[E], int lane, int 
                     if (fm[q]) nxt = 64 * q + __ffsll((unsigned long long)fm[q]) - 1;
             }
             const int pos = base + __popcll(fm[m] & ((1ull << lane) - 1ull));
-            out[pos] = x[m];
+            out[pos] = x[m] >> S;
             mult[pos] = (uint16_t)(nxt - i);
         }
         base += __popcll(fm[m]);
@@ -153,18 +154,41 @@ __device__ __forceinline__ int wave_unique(const int32_t (&x)[E], int lane, int 
     return base;
 }
 
-template <int E>
+// Candidate i of a node (i < L = npe x its elements) is local node i % npe of
+// the node's (i / npe)-th element entry.  Keyed (S = 8, L <= 256): the sort
+// key is (candidate node << 8) | i, so the candidates of one neighbour come
+// out in ascending (element entry, local index) -- the order in which the
+// staged path sums the block's contributions -- and, when contrib is given,
+// each candidate's contribution record is written at its sorted position
+// (the column node's contributions start at npe x its first element entry).
+template <int E, int S>
 __device__ __forceinline__ int adj_registers(const uint32_t *__restrict__ n2e, const int32_t *__restrict__ conn,
-                                             int npe, int64_t s0, int L, int lane, int32_t *out, uint16_t *mult) {
+                                             int npe, int64_t s0, int L, int lane, int32_t *out, uint16_t *mult,
+                                             uint32_t *__restrict__ contrib) {
     constexpr int32_t kPad = 0x7fffffff;
     int32_t x[E];
 #pragma unroll
     for (int m = 0; m < E; m++) {
         const int i = lane + 64 * m;
-        x[m] = i < L ? conn[(int64_t)(n2e[s0 + i / npe] >> 4) * npe + i % npe] : kPad;
+        x[m] = i < L ? (conn[(int64_t)(n2e[s0 + i / npe] >> 4) * npe + i % npe] << S) | (S ? i : 0) : kPad;
     }
     wave_sort<E>(x, lane);
-    return wave_unique<E>(x, lane, L, out, mult);
+    const int tot = wave_unique<E, S>(x, lane, L, out, mult);
+    if (S && contrib) {
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const int i = lane + 64 * m;
+            if (i < L) {
+                const int ci = x[m] & ((1 << S) - 1);
+                const uint32_t u = n2e[s0 + ci / npe];
+                const int ac = (int)(u & 15u), am = ci % npe;
+                const int lo = am < ac ? am : ac, hi = am < ac ? ac : am;
+                const uint32_t p = (uint32_t)(hi * (hi + 1) / 2 + lo);
+                contrib[(int64_t)npe * s0 + i] = ((u >> 4) << 7) | (p << 1) | (am > ac ? 1u : 0u);
+            }
+        }
+    }
+    return tot;
 }
 
 // Node adjacency, one wavefront per node: the nodes of the node's elements,
@@ -173,10 +197,15 @@ __device__ __forceinline__ int adj_registers(const uint32_t *__restrict__ n2e, c
 // the two share: the block's contributions).  The list of
 // node c goes to tmp / tmult at npe * n2e_ptr[c] (it has at most that many
 // candidates' room), its length to deg[c]; k_adj_compact packs the lists.
+// S = 8 (node numbers below 2^23 - 1) sorts keyed candidates, and with
+// contrib also writes the contribution lists; a node with more than 256
+// candidates then sets bit 2 of overflow (its lists are left to k_contrib).
+template <int S>
 __global__ __launch_bounds__(64) void k_adj(const int64_t *__restrict__ n2e_ptr, const uint32_t *__restrict__ n2e,
                                             const int32_t *__restrict__ conn, int npe, int64_t nnodes,
                                             int32_t *__restrict__ deg, int32_t *__restrict__ tmp,
-                                            uint16_t *__restrict__ tmult, int32_t *__restrict__ overflow) {
+                                            uint16_t *__restrict__ tmult, int32_t *__restrict__ overflow,
+                                            uint32_t *__restrict__ contrib) {
     __shared__ int32_t cv[kMaxCand];
     __shared__ int32_t part[64];
     const int lane = threadIdx.x;
@@ -187,9 +216,9 @@ __global__ __launch_bounds__(64) void k_adj(const int64_t *__restrict__ n2e_ptr,
         int32_t *out = tmp + (int64_t)npe * s0;
         uint16_t *mout = tmult + (int64_t)npe * s0;
         if (L <= 256) {  // the candidates fit in the wavefront's registers
-            const int tot = L <= 64    ? adj_registers<1>(n2e, conn, npe, s0, L, lane, out, mout)
-                            : L <= 128 ? adj_registers<2>(n2e, conn, npe, s0, L, lane, out, mout)
-                                       : adj_registers<4>(n2e, conn, npe, s0, L, lane, out, mout);
+            const int tot = L <= 64    ? adj_registers<1, S>(n2e, conn, npe, s0, L, lane, out, mout, contrib)
+                            : L <= 128 ? adj_registers<2, S>(n2e, conn, npe, s0, L, lane, out, mout, contrib)
+                                       : adj_registers<4, S>(n2e, conn, npe, s0, L, lane, out, mout, contrib);
             if (lane == 0) {
                 if (tot > kCap) atomicOr(overflow, 1);
                 deg[c] = tot > kCap ? 0 : tot;
@@ -200,6 +229,7 @@ __global__ __launch_bounds__(64) void k_adj(const int64_t *__restrict__ n2e_ptr,
             if (lane == 0) atomicOr(overflow, 1);
             L = 0;
         }
+        if (contrib && lane == 0) atomicOr(overflow, 2);
         int P = 1;
         while (P < L) P <<= 1;
         for (int i = lane; i < P; i += 64) {
@@ -320,25 +350,49 @@ __global__ __launch_bounds__(256) void k_contrib(const int32_t *__restrict__ bco
     }
 }
 
-// One wavefront per node column: its ndn DOF columns are ndn^2 deg
-// consecutive rowval entries, written by consecutive lanes.
-__global__ __launch_bounds__(256) void k_dof_pattern(const int64_t *__restrict__ adj_ptr,
-                                                     const int32_t *__restrict__ adj, int64_t nnodes, int ndn,
-                                                     int64_t *__restrict__ colptr, int32_t *__restrict__ rowval) {
-    const int lane = threadIdx.x % 64;
-    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / 64);
-    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; c < nnodes; c += nw) {
+// One thread per node block t = (row node m, column node c), the k-th block
+// of its column: in each of c's NDN DOF columns (NDN deg(c) rows each) the
+// block's NDN rows m NDN + j sit at k NDN + j.
+template <int NDN>
+__global__ __launch_bounds__(256) void k_dof_rows(const int64_t *__restrict__ adj_ptr,
+                                                  const int32_t *__restrict__ adj, const int32_t *__restrict__ bcol,
+                                                  int64_t nblocks, int32_t *__restrict__ rowval) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nblocks;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t c = bcol[t], m = adj[t];
         const int64_t b = adj_ptr[c];
-        const int deg = (int)(adj_ptr[c + 1] - b);
-        const int64_t base = (int64_t)ndn * ndn * b;
-        if (lane < ndn) colptr[c * ndn + lane] = base + (int64_t)lane * ndn * deg;
-        if (c == nnodes - 1 && lane == 0) colptr[nnodes * ndn] = base + (int64_t)ndn * ndn * deg;
-        const int per_col = ndn * deg, total = ndn * per_col;
-        for (int idx = lane; idx < total; idx += 64) {
-            const int rem = idx % per_col;
-            rowval[base + idx] = adj[b + rem / ndn] * ndn + rem % ndn;
-        }
+        const int64_t deg = adj_ptr[c + 1] - b, k = t - b;
+        const int64_t base = (int64_t)NDN * NDN * b;
+#pragma unroll
+        for (int i = 0; i < NDN; i++)
+#pragma unroll
+            for (int j = 0; j < NDN; j++) rowval[base + i * NDN * deg + k * NDN + j] = m * NDN + j;
     }
+}
+
+// One thread per node: the column pointers of its NDN DOF columns (a node
+// of no element has empty columns), the last thread also the end.
+template <int NDN>
+__global__ __launch_bounds__(256) void k_dof_colptr(const int64_t *__restrict__ adj_ptr, int64_t nnodes,
+                                                    int64_t *__restrict__ colptr) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nnodes;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = adj_ptr[c], deg = adj_ptr[c + 1] - b;
+#pragma unroll
+        for (int i = 0; i < NDN; i++) colptr[c * NDN + i] = (int64_t)NDN * NDN * b + i * NDN * deg;
+        if (c == nnodes - 1) colptr[nnodes * NDN] = (int64_t)NDN * NDN * (b + deg);
+    }
+}
+
+template <int NDN>
+void launch_dof_pattern(feasm_ctx *ctx, const Pattern &P) {
+    if (P.nblocks > 0)
+        hipLaunchKernelGGL(k_dof_rows<NDN>, dim3(grid_for(P.nblocks, 256)), dim3(256), 0, ctx->stream,
+                           dp<int64_t>(P.adj_ptr), dp<int32_t>(P.adj), dp<int32_t>(P.bcol), P.nblocks,
+                           dp<int32_t>(P.rowval));
+    hipLaunchKernelGGL(k_dof_colptr<NDN>, dim3(grid_for(P.nnodes, 256)), dim3(256), 0, ctx->stream,
+                       dp<int64_t>(P.adj_ptr), P.nnodes, dp<int64_t>(P.colptr));
+    FEASM_HIP(hipGetLastError());
 }
 
 int64_t read_i64(feasm_ctx *ctx, const int64_t *d) {
@@ -414,7 +468,7 @@ void build_node_to_element(feasm_mesh *m) {
 void build_contributions(feasm_mesh *m, Pattern &P_) {
     feasm_ctx *ctx = m->ctx;
     Pattern *P = &P_;
-    if (P->contrib) return;
+    if (P->contrib && P->cptr) return;
     P->cptr = dev_alloc((P->nblocks + 1) * sizeof(int64_t));
     if (P->bmult) {
         // a block's contributions are its row node's occurrences among the
@@ -431,6 +485,10 @@ void build_contributions(feasm_mesh *m, Pattern &P_) {
     }
     // every (element, local column, local row) is one contribution
     P->ncontrib = m->nelem * m->npe * m->npe;
+    if (P->contrib) {  // written by the adjacency pass
+        P->bmult.reset();
+        return;
+    }
     DevPtr contrib = dev_alloc(std::max<size_t>(P->ncontrib * sizeof(uint32_t), 4));
     hipLaunchKernelGGL(k_contrib, dim3(grid_for(P->nblocks, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(P->bcol),
                        dp<int32_t>(P->adj), P->nblocks, dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e),
@@ -440,7 +498,7 @@ void build_contributions(feasm_mesh *m, Pattern &P_) {
     P->bmult.reset();
 }
 
-std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn) {
+std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn, bool with_contrib) {
     feasm_ctx *ctx = m->ctx;
     if (!m->n2e_ptr) build_node_to_element(m);
     auto P = std::make_shared<Pattern>();
@@ -457,9 +515,22 @@ std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn) {
     FEASM_HIP(hipMemsetAsync(flags->p, 0, sizeof(int32_t), ctx->stream));
     FEASM_HIP(hipMemsetAsync(dp<int32_t>(deg) + nn, 0, sizeof(int32_t), ctx->stream));
     const unsigned gadj = grid_for(nn, 1, 1 << 17);
-    hipLaunchKernelGGL(k_adj, dim3(gadj), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e),
-                       dp<int32_t>(m->conn), npe, nn, dp<int32_t>(deg), dp<int32_t>(tmp), dp<uint16_t>(tmult),
-                       dp<int32_t>(flags));
+    // the staged path's contribution lists come out of the adjacency sort
+    // when the node numbers leave room for the candidate index in the keys
+    const bool keyed = with_contrib && nn < (1ll << 23) - 1;
+    if (keyed) {
+        P->ncontrib = m->nelem * npe * npe;
+        P->contrib = dev_alloc(std::max<size_t>(P->ncontrib * sizeof(uint32_t), 4));
+    }
+    uint32_t *cout = keyed ? dp<uint32_t>(P->contrib) : nullptr;
+    if (keyed)
+        hipLaunchKernelGGL(k_adj<8>, dim3(gadj), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr),
+                           dp<uint32_t>(m->n2e), dp<int32_t>(m->conn), npe, nn, dp<int32_t>(deg), dp<int32_t>(tmp),
+                           dp<uint16_t>(tmult), dp<int32_t>(flags), cout);
+    else
+        hipLaunchKernelGGL(k_adj<0>, dim3(gadj), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr),
+                           dp<uint32_t>(m->n2e), dp<int32_t>(m->conn), npe, nn, dp<int32_t>(deg), dp<int32_t>(tmp),
+                           dp<uint16_t>(tmult), dp<int32_t>(flags), cout);
     FEASM_HIP(hipGetLastError());
     P->adj_ptr = dev_alloc((nn + 1) * sizeof(int64_t));
     exclusive_scan_i32_to_i64(ctx, dp<int32_t>(deg), dp<int64_t>(P->adj_ptr), nn + 1);
@@ -468,6 +539,7 @@ std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn) {
     read_back(ctx->stream, &fl, flags->p, sizeof(int32_t));
     FEASM_REQUIRE(!(fl & 1), FEASM_EUNSUPPORTED,
                   "a node has more than 256 distinct neighbours or more than 2048 (element, node) incidences");
+    if (fl & 2) P->contrib.reset();  // some node's lists are left to k_contrib
     P->adj = dev_alloc(std::max<size_t>(P->nblocks * sizeof(int32_t), 4));
     P->bmult = dev_alloc((P->nblocks + 1) * sizeof(uint16_t));
     FEASM_HIP(hipMemsetAsync(dp<uint16_t>(P->bmult) + P->nblocks, 0, sizeof(uint16_t), ctx->stream));
@@ -480,15 +552,21 @@ std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn) {
                        dp<int32_t>(P->bcol));
     FEASM_HIP(hipGetLastError());
     // 4. DOF-level CSC
+    FEASM_REQUIRE(ndn >= 1 && ndn <= 3, FEASM_EUNSUPPORTED, "1 to 3 DOFs per node");
     P->n = nn * ndn;
     P->nnz = P->nblocks * ndn * ndn;
     FEASM_REQUIRE(P->n < (1ll << 31), FEASM_EUNSUPPORTED, "more than 2^31 DOFs");
     P->colptr = dev_alloc((P->n + 1) * sizeof(int64_t));
     P->rowval = dev_alloc(std::max<size_t>(P->nnz * sizeof(int32_t), 4));
-    FEASM_HIP(hipMemsetAsync(P->colptr->p, 0, (P->n + 1) * sizeof(int64_t), ctx->stream));
-    hipLaunchKernelGGL(k_dof_pattern, dim3(grid_for((nn + 3) / 4, 1, 65536)), dim3(256), 0, ctx->stream, dp<int64_t>(P->adj_ptr),
-                       dp<int32_t>(P->adj), nn, ndn, dp<int64_t>(P->colptr), dp<int32_t>(P->rowval));
-    FEASM_HIP(hipGetLastError());
+    if (nn == 0) {
+        FEASM_HIP(hipMemsetAsync(P->colptr->p, 0, sizeof(int64_t), ctx->stream));
+    } else if (ndn == 1) {
+        launch_dof_pattern<1>(ctx, *P);
+    } else if (ndn == 2) {
+        launch_dof_pattern<2>(ctx, *P);
+    } else {
+        launch_dof_pattern<3>(ctx, *P);
+    }
     return P;
 }
 

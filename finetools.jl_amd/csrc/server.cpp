@@ -244,7 +244,10 @@ struct feasm_server {
                 m->ndn = r->ndn;
                 m->nalldofs = r->nnodes * r->ndn;
                 m->nodemajor = true;
-                m->pattern = build_pattern(m, m->ndn);  // its read-back waits for the uploads
+                // the first assembly takes the staged path: the adjacency pass
+                // writes its contribution lists too; its read-back waits for
+                // the uploads
+                m->pattern = build_pattern(m, m->ndn, true);
             }
             std::unique_ptr<feasm_matrix> M(new feasm_matrix);
             M->ctx = ctx;
