@@ -29,31 +29,37 @@ __global__ __launch_bounds__(256) void k_gather(const int64_t *__restrict__ adj_
         const int64_t deg = adj_ptr[c + 1] - b0;
         const int64_t r = t - b0;
         double acc[SW];
+#pragma unroll
+        for (int k = 0; k < SW; k++) acc[k] = 0.0;
         const int64_t i0 = cptr[t], i1 = cptr[t + 1];
-        {
-            const uint32_t u = contrib[i0];
-            const S *s = stage + ((int64_t)(u >> 7) * npairs + ((u >> 1) & 63u)) * SW;
-            if (SW == 9 && (u & 1u)) {
+        // contributions U at a time: their list entries, then their staged
+        // blocks, are loaded before any is added, so the loads of one group
+        // overlap; the sums still run in list order (the first value is
+        // taken as it is, the left fold of sparse())
+        constexpr int U = 4;
+        for (int64_t i = i0; i < i1; i += U) {
+            uint32_t u[U];
 #pragma unroll
-                for (int k = 0; k < 3; k++)
+            for (int j = 0; j < U; j++) u[j] = i + j < i1 ? contrib[i + j] : 0u;
+            S v[U][SW];
 #pragma unroll
-                    for (int l = 0; l < 3; l++) acc[3 * k + l] = (double)s[3 * l + k];
-            } else {
+            for (int j = 0; j < U; j++) {
+                const S *s = stage + ((int64_t)(u[j] >> 7) * npairs + ((u[j] >> 1) & 63u)) * SW;
+                if (i + j < i1) {
 #pragma unroll
-                for (int k = 0; k < SW; k++) acc[k] = (double)s[k];
+                    for (int k = 0; k < SW; k++) v[j][k] = s[k];
+                }
             }
-        }
-        for (int64_t i = i0 + 1; i < i1; i++) {
-            const uint32_t u = contrib[i];
-            const S *s = stage + ((int64_t)(u >> 7) * npairs + ((u >> 1) & 63u)) * SW;
-            if (SW == 9 && (u & 1u)) {
 #pragma unroll
-                for (int k = 0; k < 3; k++)
+            for (int j = 0; j < U; j++) {
+                if (i + j >= i1) break;
+                const bool tr = SW == 9 && (u[j] & 1u);
 #pragma unroll
-                    for (int l = 0; l < 3; l++) acc[3 * k + l] = acc[3 * k + l] + (double)s[3 * l + k];
-            } else {
-#pragma unroll
-                for (int k = 0; k < SW; k++) acc[k] = acc[k] + (double)s[k];
+                for (int k = 0; k < SW; k++) {
+                    // a transposed pair's block is read as its transpose
+                    const double x = (double)(tr ? v[j][3 * (k % 3) + k / 3] : v[j][k]);
+                    acc[k] = (i + j == i0) ? x : acc[k] + x;
+                }
             }
         }
         // block (m, c): entry (k, l) sits in DOF column c*NDN + l at row slot r*NDN + k

@@ -49,11 +49,56 @@ __global__ void k_n2e_fill(const int32_t *__restrict__ conn, int64_t nelem, int 
     }
 }
 
-// the atomic fill leaves each node's list in arbitrary order; sort it
+// Ascending bitonic sort of N (a power of two) values in one thread's
+// registers (every index a compile-time constant once unrolled).
+template <int N>
+__device__ __forceinline__ void thread_sort(uint32_t (&x)[N]) {
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t a = x[i], b = x[l];
+                    const bool up = (i & k) == 0;
+                    x[i] = up ? min(a, b) : max(a, b);
+                    x[l] = up ? max(a, b) : min(a, b);
+                }
+            }
+}
+
+template <int N>
+__device__ __forceinline__ void sort_list(uint32_t *__restrict__ v, int len) {
+    uint32_t x[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) x[i] = i < len ? v[i] : 0xffffffffu;
+    thread_sort<N>(x);
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        if (i < len) v[i] = x[i];
+}
+
+// the atomic fill leaves each node's list in arbitrary order; sort it (in
+// registers up to 32 entries)
 __global__ void k_n2e_sort(const int64_t *__restrict__ ptr, uint32_t *__restrict__ n2e, int64_t nnodes) {
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nnodes;
          c += (int64_t)gridDim.x * blockDim.x) {
         const int64_t b = ptr[c], e = ptr[c + 1];
+        const int len = (int)(e - b);
+        if (len <= 8) {
+            if (len > 1) sort_list<8>(n2e + b, len);
+            continue;
+        }
+        if (len <= 16) {
+            sort_list<16>(n2e + b, len);
+            continue;
+        }
+        if (len <= 32) {
+            sort_list<32>(n2e + b, len);
+            continue;
+        }
         for (int64_t i = b + 1; i < e; i++) {
             const uint32_t v = n2e[i];
             int64_t j = i - 1;
@@ -290,25 +335,21 @@ __global__ __launch_bounds__(64) void k_adj(const int64_t *__restrict__ n2e_ptr,
     }
 }
 
-// packs the lists of k_adj: node c's at adj_ptr[c]
+// packs the lists of k_adj: node c's at adj_ptr[c], with the column node of
+// each block
 __global__ __launch_bounds__(64) void k_adj_compact(const int64_t *__restrict__ n2e_ptr, int npe, int64_t nnodes,
                                                     const int64_t *__restrict__ adj_ptr, const int32_t *__restrict__ tmp,
                                                     const uint16_t *__restrict__ tmult, int32_t *__restrict__ adj,
-                                                    uint16_t *__restrict__ bmult) {
+                                                    uint16_t *__restrict__ bmult, int32_t *__restrict__ bcol) {
     for (int64_t c = blockIdx.x; c < nnodes; c += gridDim.x) {
         const int64_t src = (int64_t)npe * n2e_ptr[c], dst = adj_ptr[c];
         const int d = (int)(adj_ptr[c + 1] - dst);
         for (int i = threadIdx.x; i < d; i += 64) {
             adj[dst + i] = tmp[src + i];
             bmult[dst + i] = tmult[src + i];
+            bcol[dst + i] = (int32_t)c;
         }
     }
-}
-
-__global__ void k_bcol(const int64_t *__restrict__ adj_ptr, int64_t nnodes, int32_t *__restrict__ bcol) {
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nnodes;
-         c += (int64_t)gridDim.x * blockDim.x)
-        for (int64_t t = adj_ptr[c]; t < adj_ptr[c + 1]; t++) bcol[t] = (int32_t)c;
 }
 
 // Contributions of block t = (row node m, column node c): the elements
@@ -543,13 +584,10 @@ std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn, bool with_contrib
     P->adj = dev_alloc(std::max<size_t>(P->nblocks * sizeof(int32_t), 4));
     P->bmult = dev_alloc((P->nblocks + 1) * sizeof(uint16_t));
     FEASM_HIP(hipMemsetAsync(dp<uint16_t>(P->bmult) + P->nblocks, 0, sizeof(uint16_t), ctx->stream));
+    P->bcol = dev_alloc(std::max<size_t>(P->nblocks * sizeof(int32_t), 4));
     hipLaunchKernelGGL(k_adj_compact, dim3(gadj), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr), npe, nn,
                        dp<int64_t>(P->adj_ptr), dp<int32_t>(tmp), dp<uint16_t>(tmult), dp<int32_t>(P->adj),
-                       dp<uint16_t>(P->bmult));
-    FEASM_HIP(hipGetLastError());
-    P->bcol = dev_alloc(P->nblocks * sizeof(int32_t));
-    hipLaunchKernelGGL(k_bcol, dim3(grid_for(nn, 256)), dim3(256), 0, ctx->stream, dp<int64_t>(P->adj_ptr), nn,
-                       dp<int32_t>(P->bcol));
+                       dp<uint16_t>(P->bmult), dp<int32_t>(P->bcol));
     FEASM_HIP(hipGetLastError());
     // 4. DOF-level CSC
     FEASM_REQUIRE(ndn >= 1 && ndn <= 3, FEASM_EUNSUPPORTED, "1 to 3 DOFs per node");
