@@ -249,6 +249,16 @@ extern "C" int feasm_ctx_create(int device, feasm_ctx **out) {
     API_CATCH(nullptr)
 }
 
+extern "C" int feasm_pool_trim(int device, int64_t *freed_bytes) {
+    API_TRY
+    int ndev = 0;
+    FEASM_HIP(hipGetDeviceCount(&ndev));
+    FEASM_REQUIRE(device >= 0 && device < ndev, FEASM_EINVAL, "no such HIP device");
+    const size_t f = pool_trim(device);
+    if (freed_bytes) *freed_bytes = (int64_t)f;
+    API_CATCH(nullptr)
+}
+
 void feasm_ctx_release(feasm_ctx *ctx) {
     if (--ctx->refs > 0) return;
     (void)hipStreamSynchronize(ctx->stream);

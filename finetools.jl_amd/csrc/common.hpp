@@ -41,13 +41,29 @@ void pool_release(void *p, size_t bytes, int device);
 void pool_register_stream(int device, hipStream_t s);
 void pool_unregister_stream(int device, hipStream_t s);
 size_t pool_cached_bytes(int device);
+// frees every cached block after a device synchronisation; the bytes freed
+size_t pool_trim(int device);
 // the calling thread's buffers are used on stream s only (null: any
 // stream): blocks it releases are fenced on s alone
 void pool_set_thread_fence(int device, hipStream_t s);
+// pool_set_thread_fence for a scope (the previous setting comes back)
+struct ScopedThreadFence {
+    hipStream_t prev_s = nullptr;
+    int prev_d = -1;
+    ScopedThreadFence(int device, hipStream_t s);
+    ~ScopedThreadFence();
+};
 // copies bytes from device memory after the work queued on s, through a
 // pinned buffer of the calling thread (a copy into pageable memory goes
 // through the runtime's staging path, tens of microseconds a call)
 void read_back(hipStream_t s, void *host, const void *dev, size_t bytes);
+// several pieces behind one wait
+struct ReadBack {
+    void *host;
+    const void *dev;
+    size_t bytes;
+};
+void read_back(hipStream_t s, const ReadBack *parts, int n);
 
 // Owning device allocation; shared so that a matrix can keep the pattern
 // arrays of its mesh alive.  bytes is the capacity (at least what was asked).

@@ -436,12 +436,6 @@ void launch_dof_pattern(feasm_ctx *ctx, const Pattern &P) {
     FEASM_HIP(hipGetLastError());
 }
 
-int64_t read_i64(feasm_ctx *ctx, const int64_t *d) {
-    int64_t v = 0;
-    read_back(ctx->stream, &v, d, sizeof(v));
-    return v;
-}
-
 }  // namespace
 
 void exclusive_scan_i32_to_i64(feasm_ctx *ctx, const int32_t *in, int64_t *out, int64_t n) {
@@ -575,9 +569,10 @@ std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn, bool with_contrib
     FEASM_HIP(hipGetLastError());
     P->adj_ptr = dev_alloc((nn + 1) * sizeof(int64_t));
     exclusive_scan_i32_to_i64(ctx, dp<int32_t>(deg), dp<int64_t>(P->adj_ptr), nn + 1);
-    P->nblocks = read_i64(ctx, dp<int64_t>(P->adj_ptr) + nn);
     int32_t fl = 0;
-    read_back(ctx->stream, &fl, flags->p, sizeof(int32_t));
+    const ReadBack rb[2] = {{&P->nblocks, dp<int64_t>(P->adj_ptr) + nn, sizeof(int64_t)},
+                            {&fl, flags->p, sizeof(int32_t)}};
+    read_back(ctx->stream, rb, 2);
     FEASM_REQUIRE(!(fl & 1), FEASM_EUNSUPPORTED,
                   "a node has more than 256 distinct neighbours or more than 2048 (element, node) incidences");
     if (fl & 2) P->contrib.reset();  // some node's lists are left to k_contrib

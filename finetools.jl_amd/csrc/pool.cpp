@@ -2,7 +2,10 @@
 //
 // hipFree waits for the whole device, so a request loop whose requests
 // allocate and free their own buffers serialises every worker on it.  Blocks
-// released here are kept and handed out again instead.  A released block is
+// released here are kept and handed out again instead, also to contexts
+// created later, until an allocation fails (everything cached is then freed
+// after a device synchronisation and the allocation retried) or the caller
+// trims the pool (feasm_pool_trim).  A released block is
 // reused only once every library stream has finished the work enqueued before
 // its release: the release records an event on each registered stream (every
 // context's stream) and the block waits until all of them have completed.
@@ -102,7 +105,8 @@ void *pool_alloc(size_t n, size_t *capacity, int *device) {
     Pool &P = pool(*device);
     {
         std::lock_guard<std::mutex> lk(P.mu);
-        for (auto it = P.cached.lower_bound(cap); it != P.cached.end() && it->first <= cap + cap / 4; ++it) {
+        // the smallest ready block of at least the size class, up to twice it
+        for (auto it = P.cached.lower_bound(cap); it != P.cached.end() && it->first <= 2 * cap; ++it) {
             if (!ready(it->second)) continue;
             Block b = it->second;
             P.cached.erase(it);
@@ -218,12 +222,31 @@ void pool_unregister_stream(int device, hipStream_t s) {
             P.streams.erase(P.streams.begin() + i);
             break;
         }
-    if (P.streams.empty()) trim_locked(P, false);
+    // cached blocks stay for the next contexts (feasm_pool_trim frees them)
+}
+
+size_t pool_trim(int device) {
+    DeviceScope scope(device);
+    Pool &P = pool(device);
+    std::unique_lock<std::shared_mutex> sl(P.smu);
+    std::lock_guard<std::mutex> lk(P.mu);
+    const size_t before = P.cached_bytes;
+    trim_locked(P, true);
+    return before - P.cached_bytes;
 }
 
 void pool_set_thread_fence(int device, hipStream_t s) {
     t_fence_stream = s;
     t_fence_device = s ? device : -1;
+}
+
+ScopedThreadFence::ScopedThreadFence(int device, hipStream_t s) : prev_s(t_fence_stream), prev_d(t_fence_device) {
+    pool_set_thread_fence(device, s);
+}
+
+ScopedThreadFence::~ScopedThreadFence() {
+    t_fence_stream = prev_s;
+    t_fence_device = prev_d;
 }
 
 namespace {
@@ -248,6 +271,13 @@ thread_local PinnedHolder t_pinned;
 }  // namespace
 
 void read_back(hipStream_t s, void *host, const void *dev, size_t bytes) {
+    const ReadBack one{host, dev, bytes};
+    read_back(s, &one, 1);
+}
+
+void read_back(hipStream_t s, const ReadBack *parts, int n) {
+    size_t bytes = 0;
+    for (int i = 0; i < n; i++) bytes += (parts[i].bytes + 15) & ~size_t(15);
     Pinned &b = t_pinned.b;
     if (b.bytes < bytes) {
         Pinned nb;
@@ -267,9 +297,17 @@ void read_back(hipStream_t s, void *host, const void *dev, size_t bytes) {
         }
         b = nb;
     }
-    FEASM_HIP(hipMemcpyAsync(b.p, dev, bytes, hipMemcpyDeviceToHost, s));
+    unsigned char *q = static_cast<unsigned char *>(b.p);
+    for (int i = 0; i < n; i++) {
+        if (parts[i].bytes) FEASM_HIP(hipMemcpyAsync(q, parts[i].dev, parts[i].bytes, hipMemcpyDeviceToHost, s));
+        q += (parts[i].bytes + 15) & ~size_t(15);
+    }
     FEASM_HIP(hipStreamSynchronize(s));
-    std::memcpy(host, b.p, bytes);
+    q = static_cast<unsigned char *>(b.p);
+    for (int i = 0; i < n; i++) {
+        if (parts[i].bytes) std::memcpy(parts[i].host, q, parts[i].bytes);
+        q += (parts[i].bytes + 15) & ~size_t(15);
+    }
 }
 
 size_t pool_cached_bytes(int device) {

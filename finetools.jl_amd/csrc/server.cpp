@@ -15,6 +15,8 @@
 // Results are deterministic: every kernel sums in a fixed order, so a request
 // gives bitwise the same matrix alone or in any mix.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <atomic>
 #include <chrono>
@@ -108,6 +110,15 @@ struct feasm_server {
     bool stop = false;
     int64_t completed = 0;
     std::string last_error;
+    // per worker: seconds waiting for an admissible request, converting and
+    // staging new meshes' arrays, waiting for a request's device work at its
+    // end, in run_one; requests run
+    // (FEASM_SERVER_TIMING=1 prints them to stderr when the server ends)
+    struct WorkerTimes {
+        double idle = 0, convert = 0, sync = 0, run = 0;
+        int64_t n = 0;
+    };
+    std::vector<WorkerTimes> times;
 
     // evicted meshes are released outside the lock: no request uses them
     // (every request synchronises its stream before it finishes) and the
@@ -149,6 +160,7 @@ struct feasm_server {
             Request *r = nullptr;
             std::vector<void *> evicted;
             bool quit = false;
+            const double t_wait = now_s();
             {
                 std::unique_lock<std::mutex> lk(mu);
                 for (;;) {
@@ -166,10 +178,14 @@ struct feasm_server {
                     adm.start(r);
                 }
             }
+            const double t_picked = now_s();
+            times[w].idle += t_picked - t_wait;
             release_meshes(evicted);
             if (quit) return;
             if (!r) continue;
-            run_one(ctx, r, stage);  // publishes the outcome and the bookkeeping under one lock
+            run_one(ctx, r, stage, times[w]);  // publishes the outcome and the bookkeeping under one lock
+            times[w].run += now_s() - t_picked;
+            times[w].n++;
             cv_work.notify_all();
             cv_done.notify_all();
         }
@@ -198,7 +214,7 @@ struct feasm_server {
         r->xyz.shrink_to_fit();
     }
 
-    void run_one(feasm_ctx *ctx, Request *r, Stage &stage) {
+    void run_one(feasm_ctx *ctx, Request *r, Stage &stage, WorkerTimes &wt) {
         std::unique_ptr<feasm_mesh> owned;
         try {
             feasm_mesh *m = nullptr;
@@ -220,16 +236,22 @@ struct feasm_server {
                 m->nnodes = r->nnodes;
                 // the arrays go through the worker's pinned staging memory;
                 // borrowed ones are converted and checked here, in the worker
+                const double t_conv = now_s();
                 const size_t cb = (size_t)(r->nelem * r->npe) * 4, xb = (size_t)r->nnodes * 24;
                 unsigned char *st = static_cast<unsigned char *>(stage.get(((cb + 15) & ~size_t(15)) + xb + 16));
                 int32_t *sc = reinterpret_cast<int32_t *>(st);
                 double *sx = reinterpret_cast<double *>(st + ((cb + 15) & ~size_t(15)));
                 if (r->bconn) {
-                    for (int64_t i = 0; i < r->nelem * r->npe; i++) {
-                        FEASM_REQUIRE(r->bconn[i] >= 1 && r->bconn[i] <= r->nnodes, FEASM_EINVAL,
-                                      "connectivity refers to a missing node");
-                        sc[i] = (int32_t)(r->bconn[i] - 1);
+                    // converted and range-checked in one branch-free pass
+                    const int64_t nc = r->nelem * r->npe, nn = r->nnodes;
+                    const int64_t *bc = r->bconn;
+                    int64_t bad = 0;
+                    for (int64_t i = 0; i < nc; i++) {
+                        const int64_t v = bc[i];
+                        bad |= (v < 1) | (v > nn);
+                        sc[i] = (int32_t)(v - 1);
                     }
+                    FEASM_REQUIRE(!bad, FEASM_EINVAL, "connectivity refers to a missing node");
                     if (xb) std::memcpy(sx, r->bxyz, xb);
                     r->bconn = nullptr;
                     r->bxyz = nullptr;
@@ -237,6 +259,7 @@ struct feasm_server {
                     if (cb) std::memcpy(sc, r->conn0.data(), cb);
                     if (xb) std::memcpy(sx, r->xyz.data(), xb);
                 }
+                wt.convert += now_s() - t_conv;
                 m->conn = dev_alloc(std::max<size_t>(cb, 4));
                 m->xyz = dev_alloc(std::max<size_t>(xb, 8));
                 if (cb) FEASM_HIP(hipMemcpyAsync(m->conn->p, sc, cb, hipMemcpyHostToDevice, ctx->stream));
@@ -271,7 +294,9 @@ struct feasm_server {
             }
             m->assemblies++;
             numeric_into_ctx(ctx, m, r->fp, r->rule, r->order, r->precision, dp<double>(M->nzval));
+            const double t_sync = now_s();
             FEASM_HIP(hipStreamSynchronize(ctx->stream));
+            wt.sync += now_s() - t_sync;
             std::lock_guard<std::mutex> lk(mu);
             publish_locked(r, 2, std::string(), std::move(M), r->keep ? owned.release() : nullptr);
         } catch (const feasm::Error &e) {
@@ -336,6 +361,7 @@ extern "C" int feasm_server_create(int device, int workers, double hbm_fraction,
         s->ctxs.push_back(c);
     }
     feasm_server *sp = s.get();
+    s->times.resize(workers);
     for (int w = 0; w < workers; w++) s->workers.emplace_back([sp, w] { sp->work(w); });
     *out = s.release();
     S_CATCH(nullptr)
@@ -476,11 +502,22 @@ extern "C" int feasm_server_result(feasm_server *s, int64_t request_id, feasm_ma
 extern "C" int feasm_server_release(feasm_server *s, int64_t request_id) {
     S_TRY
     FEASM_REQUIRE(s, FEASM_EINVAL, "null server");
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->requests.find(request_id);
-    FEASM_REQUIRE(it != s->requests.end(), FEASM_EINVAL, "unknown request");
-    FEASM_REQUIRE(it->second->status == 2 || it->second->status < 0, FEASM_ESTATE, "request still pending");
-    s->requests.erase(it);
+    std::unique_ptr<Request> dead;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->requests.find(request_id);
+        FEASM_REQUIRE(it != s->requests.end(), FEASM_EINVAL, "unknown request");
+        FEASM_REQUIRE(it->second->status == 2 || it->second->status < 0, FEASM_ESTATE, "request still pending");
+        dead = std::move(it->second);
+        s->requests.erase(it);
+    }
+    // freed outside the lock; the result's buffers were last used on its
+    // worker's stream (the request synchronised it, and operations on the
+    // matrix run on its context's stream), so only that stream is fenced
+    if (dead->result && dead->result->ctx) {
+        ScopedThreadFence fence(s->device, dead->result->ctx->stream);
+        dead.reset();
+    }
     S_CATCH(s)
 }
 
@@ -538,6 +575,13 @@ extern "C" int feasm_server_destroy(feasm_server *s) {
     }
     s->cv_work.notify_all();
     for (auto &t : s->workers) t.join();
+    if (std::getenv("FEASM_SERVER_TIMING"))
+        for (size_t w = 0; w < s->times.size(); w++)
+            std::fprintf(stderr,
+                         "feasm server worker %zu: %lld requests, idle %.1f ms, run %.1f ms (convert %.1f ms, "
+                         "final wait %.1f ms)\n",
+                         w, (long long)s->times[w].n, 1e3 * s->times[w].idle, 1e3 * s->times[w].run,
+                         1e3 * s->times[w].convert, 1e3 * s->times[w].sync);
     s->requests.clear();
     for (auto &kv : s->adm.residents) delete static_cast<feasm_mesh *>(kv.second.mesh);
     s->adm.residents.clear();

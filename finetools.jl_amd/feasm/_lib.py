@@ -29,6 +29,7 @@ SIGNATURES = {
     "feasm_ctx_stream": (P, [P]),
     "feasm_ctx_synchronize": (I32, [P]),
     "feasm_mem_info": (I32, [P, PI64, PI64]),
+    "feasm_pool_trim": (I32, [I32, PI64]),
     "feasm_ctx_set_timing": (I32, [P, I32]),
     "feasm_timings": (I32, [P, P, I32, ctypes.POINTER(ctypes.c_int)]),
     "feasm_timings_reset": (I32, [P]),

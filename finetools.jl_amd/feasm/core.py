@@ -58,6 +58,13 @@ class Context:
         check(lib().feasm_timings_reset(self.h), self.h)
 
 
+def pool_trim(device=0):
+    """Free the device memory the library keeps for reuse; the bytes freed."""
+    f = ctypes.c_int64()
+    check(lib().feasm_pool_trim(device, ctypes.byref(f)))
+    return f.value
+
+
 def default_context():
     global _default_ctx
     if _default_ctx is None:

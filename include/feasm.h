@@ -84,6 +84,12 @@ void *feasm_ctx_stream(feasm_ctx *ctx);
 int feasm_ctx_synchronize(feasm_ctx *ctx);
 /* Free and total device memory in bytes (admission control). */
 int feasm_mem_info(feasm_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
+/* Device memory pool (no reference counterpart: Julia's GC has no
+ * equivalent): buffers the library releases are kept for reuse by later
+ * calls and contexts on the same device -- a hipFree would wait for the whole
+ * device -- until an allocation fails or this call frees them (after a
+ * device synchronisation).  *freed_bytes (may be null): the bytes freed. */
+int feasm_pool_trim(int device, int64_t *freed_bytes);
 /* Per-kernel timing with HIP events on the context stream.  slots > 0 turns
  * it on with a ring of `slots` numeric assemblies (each records an event
  * before the element kernel, between the kernels and after the gather);
