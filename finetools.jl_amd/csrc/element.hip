@@ -39,6 +39,14 @@ namespace {
 constexpr int kWaves = 4;  // wavefronts per workgroup
 constexpr int kBlock = 64 * kWaves;
 
+// Orders one wavefront's LDS writes before its later reads: every wavefront
+// works on its own LDS slices, so no workgroup barrier is needed.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Voigt index of the symmetric pair (i, k) in the strain order xx yy zz xy xz yz
 __device__ constexpr int voigt(int i, int k) {
     return (i == k) ? i : ((i + k == 1) ? 3 : ((i + k == 2) ? 4 : 5));
@@ -82,19 +90,15 @@ __global__ __launch_bounds__(kBlock) void k_element(const int32_t *__restrict__ 
     const int ndn = kp.fp.ndn;
     const int kd = ndn * NPE;
 
-    // Every wavefront of the workgroup runs the same number of iterations so
-    // that the workgroup barriers stay uniform; a wavefront past the end of
-    // the element range idles through its iteration.
-    for (int64_t g = blockIdx.x; g * kWaves < nelem; g += gridDim.x) {
-        const int64_t e = g * kWaves + wv;
-        const bool live = e < nelem;
-        const int32_t *ce = conn + (live ? e : 0) * NPE;
+    // Each wavefront walks its own elements (grid-stride) and synchronises
+    // with itself only.
+    for (int64_t e = (int64_t)blockIdx.x * kWaves + wv; e < nelem; e += (int64_t)gridDim.x * kWaves) {
+        const int32_t *ce = conn + e * NPE;
         // A: coordinates
-        if (live)
-            for (int i = lane; i < NPE * 3; i += 64) sX[wv][i] = xyz[(int64_t)ce[i / 3] * 3 + (i % 3)];
-        __syncthreads();
+        for (int i = lane; i < NPE * 3; i += 64) sX[wv][i] = xyz[(int64_t)ce[i / 3] * 3 + (i % 3)];
+        wave_sync();
         // B: Jacobian entries J[q][r][c] = sum_a X[a][r] dN[q][a][c]
-        for (int i = live ? lane : NQP * 9; i < NQP * 9; i += 64) {
+        for (int i = lane; i < NQP * 9; i += 64) {
             const int q = i / 9, r = (i % 9) / 3, c = i % 3;
             const double *dq = kp.dN + q * NPE * 3;
             double s = 0.0;
@@ -102,9 +106,9 @@ __global__ __launch_bounds__(kBlock) void k_element(const int32_t *__restrict__ 
             for (int a = 0; a < NPE; a++) s += sX[wv][3 * a + r] * dq[3 * a + c];
             sJ[wv][i] = s;
         }
-        __syncthreads();
+        wave_sync();
         // C: determinant and inverse, FP64
-        for (int q = live ? lane : NQP; q < NQP; q += 64) {
+        for (int q = lane; q < NQP; q += 64) {
             const double *Jq = &sJ[wv][q * 9];
 #define RJ(i, j) Jq[3 * ((i)-1) + ((j)-1)]
             const double det = RJ(1, 1) * (RJ(2, 2) * RJ(3, 3) - RJ(3, 2) * RJ(2, 3)) -
@@ -131,10 +135,10 @@ __global__ __launch_bounds__(kBlock) void k_element(const int32_t *__restrict__ 
                 wj = det * kp.w[q];
             sW[wv][q] = (R)wj;
         }
-        __syncthreads();
+        wave_sync();
         // D: spatial gradients g[q][a][c] = sum_k dN[q][a][k] inv[k][c]
         if (GRAD) {
-            for (int i = live ? lane : NQP * NPE; i < NQP * NPE; i += 64) {
+            for (int i = lane; i < NQP * NPE; i += 64) {
                 const int q = i / NPE, a = i % NPE;
                 const double *d = kp.dN + (q * NPE + a) * 3;
                 const R d0 = (R)d[0], d1 = (R)d[1], d2 = (R)d[2];
@@ -144,10 +148,10 @@ __global__ __launch_bounds__(kBlock) void k_element(const int32_t *__restrict__ 
                 g[1] = d0 * iv[1] + d1 * iv[4] + d2 * iv[7];
                 g[2] = d0 * iv[2] + d1 * iv[5] + d2 * iv[8];
             }
-            __syncthreads();
+            wave_sync();
         }
         // E: node-pair blocks
-        for (int p = live ? lane : NP; p < NP; p += 64) {
+        for (int p = lane; p < NP; p += 64) {
             int a, b;
             pair_of(p, a, b);
             R out[SW];
@@ -257,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_element(const int32_t *__restrict__ 
                     }
             }
         }
-        __syncthreads();
+        wave_sync();
     }
 }
 

@@ -336,19 +336,39 @@ __global__ __launch_bounds__(64) void k_adj(const int64_t *__restrict__ n2e_ptr,
 }
 
 // packs the lists of k_adj: node c's at adj_ptr[c], with the column node of
-// each block
+// each block, and either the multiplicities (bmult) or the contribution list
+// offsets (cptr: node c's contributions start at npe * n2e_ptr[c], each
+// block's after the ones of the blocks before it in the column)
 __global__ __launch_bounds__(64) void k_adj_compact(const int64_t *__restrict__ n2e_ptr, int npe, int64_t nnodes,
                                                     const int64_t *__restrict__ adj_ptr, const int32_t *__restrict__ tmp,
                                                     const uint16_t *__restrict__ tmult, int32_t *__restrict__ adj,
-                                                    uint16_t *__restrict__ bmult, int32_t *__restrict__ bcol) {
+                                                    uint16_t *__restrict__ bmult, int32_t *__restrict__ bcol,
+                                                    int64_t *__restrict__ cptr) {
+    const int lane = threadIdx.x;
     for (int64_t c = blockIdx.x; c < nnodes; c += gridDim.x) {
         const int64_t src = (int64_t)npe * n2e_ptr[c], dst = adj_ptr[c];
         const int d = (int)(adj_ptr[c + 1] - dst);
-        for (int i = threadIdx.x; i < d; i += 64) {
-            adj[dst + i] = tmp[src + i];
-            bmult[dst + i] = tmult[src + i];
-            bcol[dst + i] = (int32_t)c;
+        int64_t carry = src;
+        for (int i0 = 0; i0 < d; i0 += 64) {  // uniform trip count: the shuffles see every lane
+            const int i = i0 + lane;
+            const int mlt = i < d ? (int)tmult[src + i] : 0;
+            if (i < d) {
+                adj[dst + i] = tmp[src + i];
+                bcol[dst + i] = (int32_t)c;
+                if (bmult) bmult[dst + i] = (uint16_t)mlt;
+            }
+            if (cptr) {
+                int incl = mlt;  // inclusive prefix over the lanes
+#pragma unroll
+                for (int k = 1; k < 64; k <<= 1) {
+                    const int y = __shfl_up(incl, k, 64);
+                    if (lane >= k) incl += y;
+                }
+                if (i < d) cptr[dst + i] = carry + incl - mlt;
+                carry += __shfl(incl, 63, 64);
+            }
         }
+        if (cptr && c == nnodes - 1 && lane == 0) cptr[dst + d] = (int64_t)npe * n2e_ptr[nnodes];
     }
 }
 
@@ -504,12 +524,15 @@ void build_contributions(feasm_mesh *m, Pattern &P_) {
     feasm_ctx *ctx = m->ctx;
     Pattern *P = &P_;
     if (P->contrib && P->cptr) return;
-    P->cptr = dev_alloc((P->nblocks + 1) * sizeof(int64_t));
-    if (P->bmult) {
+    if (P->cptr) {
+        // offsets written with the pattern
+    } else if (P->bmult) {
+        P->cptr = dev_alloc((P->nblocks + 1) * sizeof(int64_t));
         // a block's contributions are its row node's occurrences among the
         // column node's elements' nodes, counted by the adjacency pass
         exclusive_scan_u16_to_i64(ctx, dp<uint16_t>(P->bmult), dp<int64_t>(P->cptr), P->nblocks + 1);
     } else {
+        P->cptr = dev_alloc((P->nblocks + 1) * sizeof(int64_t));
         DevPtr ccount = dev_alloc((P->nblocks + 1) * sizeof(int32_t));
         FEASM_HIP(hipMemsetAsync(ccount->p, 0, (P->nblocks + 1) * sizeof(int32_t), ctx->stream));
         hipLaunchKernelGGL(k_contrib, dim3(grid_for(P->nblocks, 256)), dim3(256), 0, ctx->stream,
@@ -577,12 +600,19 @@ std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn, bool with_contrib
                   "a node has more than 256 distinct neighbours or more than 2048 (element, node) incidences");
     if (fl & 2) P->contrib.reset();  // some node's lists are left to k_contrib
     P->adj = dev_alloc(std::max<size_t>(P->nblocks * sizeof(int32_t), 4));
-    P->bmult = dev_alloc((P->nblocks + 1) * sizeof(uint16_t));
-    FEASM_HIP(hipMemsetAsync(dp<uint16_t>(P->bmult) + P->nblocks, 0, sizeof(uint16_t), ctx->stream));
+    if (with_contrib) {
+        // the staged path's list offsets, straight from the multiplicities
+        P->cptr = dev_alloc((P->nblocks + 1) * sizeof(int64_t));
+        if (nn == 0) FEASM_HIP(hipMemsetAsync(P->cptr->p, 0, sizeof(int64_t), ctx->stream));
+    } else {
+        P->bmult = dev_alloc((P->nblocks + 1) * sizeof(uint16_t));
+        FEASM_HIP(hipMemsetAsync(dp<uint16_t>(P->bmult) + P->nblocks, 0, sizeof(uint16_t), ctx->stream));
+    }
     P->bcol = dev_alloc(std::max<size_t>(P->nblocks * sizeof(int32_t), 4));
     hipLaunchKernelGGL(k_adj_compact, dim3(gadj), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr), npe, nn,
                        dp<int64_t>(P->adj_ptr), dp<int32_t>(tmp), dp<uint16_t>(tmult), dp<int32_t>(P->adj),
-                       dp<uint16_t>(P->bmult), dp<int32_t>(P->bcol));
+                       P->bmult ? dp<uint16_t>(P->bmult) : nullptr, dp<int32_t>(P->bcol),
+                       P->cptr ? dp<int64_t>(P->cptr) : nullptr);
     FEASM_HIP(hipGetLastError());
     // 4. DOF-level CSC
     FEASM_REQUIRE(ndn >= 1 && ndn <= 3, FEASM_EUNSUPPORTED, "1 to 3 DOFs per node");
