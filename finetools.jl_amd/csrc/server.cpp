@@ -128,24 +128,53 @@ struct feasm_server {
         meshes.clear();
     }
 
-    // pinned host memory for a worker's mesh uploads (a copy from pageable
-    // memory goes through the runtime's staging path)
+    // Pinned host memory for the mesh uploads (a copy from pageable memory
+    // goes through the runtime's staging path).  The buffers are shared by
+    // every worker of every server in the process: a request takes the
+    // smallest free one that fits and gives it back when its copies have
+    // completed, so a large mesh on any worker reuses the buffer of an
+    // earlier large one (pinning tens of megabytes takes milliseconds).  New
+    // buffers are sized to a power of two; they are kept until the process
+    // ends (the HIP runtime may be gone by then, so they are never freed).
+    struct StagePool {
+        std::mutex mu;
+        std::multimap<size_t, void *> free;  // by capacity
+    };
+    static StagePool &stage_pool() {
+        static StagePool *p = new StagePool;
+        return *p;
+    }
     struct Stage {
         void *p = nullptr;
         size_t bytes = 0;
-        void *get(size_t n) {  // the previous request's copies have completed
-            if (n > bytes) {
-                if (p) (void)hipHostFree(p);
-                p = nullptr;
-                bytes = 0;
-                FEASM_HIP(hipHostMalloc(&p, n, hipHostMallocDefault));
-                bytes = n;
+        void *get(size_t n) {
+            put();
+            StagePool &sp = stage_pool();
+            {
+                std::lock_guard<std::mutex> lk(sp.mu);
+                auto it = sp.free.lower_bound(n);
+                if (it != sp.free.end()) {
+                    bytes = it->first;
+                    p = it->second;
+                    sp.free.erase(it);
+                    return p;
+                }
             }
+            size_t cap = 1 << 20;
+            while (cap < n) cap <<= 1;
+            FEASM_HIP(hipHostMalloc(&p, cap, hipHostMallocDefault));
+            bytes = cap;
             return p;
         }
-        ~Stage() {
-            if (p) (void)hipHostFree(p);
+        void put() {  // the copies from the buffer have completed
+            if (!p) return;
+            StagePool &sp = stage_pool();
+            std::lock_guard<std::mutex> lk(sp.mu);
+            sp.free.emplace(bytes, p);
+            p = nullptr;
+            bytes = 0;
         }
+        ~Stage() { put(); }
     };
 
     void work(int w) {
@@ -184,6 +213,7 @@ struct feasm_server {
             if (quit) return;
             if (!r) continue;
             run_one(ctx, r, stage, times[w]);  // publishes the outcome and the bookkeeping under one lock
+            stage.put();                       // its uploads have completed
             times[w].run += now_s() - t_picked;
             times[w].n++;
             cv_work.notify_all();
@@ -300,9 +330,11 @@ struct feasm_server {
             std::lock_guard<std::mutex> lk(mu);
             publish_locked(r, 2, std::string(), std::move(M), r->keep ? owned.release() : nullptr);
         } catch (const feasm::Error &e) {
+            (void)hipStreamSynchronize(ctx->stream);  // uploads from the staging buffer may still run
             std::lock_guard<std::mutex> lk(mu);
             publish_locked(r, e.code, e.what(), nullptr, nullptr);
         } catch (const std::exception &e) {
+            (void)hipStreamSynchronize(ctx->stream);
             std::lock_guard<std::mutex> lk(mu);
             publish_locked(r, FEASM_EINVAL, e.what(), nullptr, nullptr);
         }
