@@ -217,6 +217,10 @@ struct feasm_matrix {
 
 namespace feasm {
 
+// fused.hip: D (6 x 6, row-major) has cubic symmetry -- D00 = D11 = D22,
+// D01 = D02 = D12, D33 = D44 = D55, every other entry zero
+bool cubic_symmetry(const double *D);
+
 // tables.cpp
 const DevQTable &device_qtable(feasm_ctx *ctx, int etype, int rule, int order);
 

@@ -62,6 +62,11 @@ __device__ __forceinline__ void pair_of(int p, int &a, int &b) {
     a = p - bb * (bb + 1) / 2;
 }
 
+// Linear elasticity whose D has cubic symmetry (isotropic materials among
+// them: D00 = D11 = D22, D01 = D02 = D12, D33 = D44 = D55, the rest zero):
+// K = D01 M + D33 M^T off the diagonal, K_ii = D00 M_ii + D33 (tr M - M_ii).
+constexpr int kElasticCubic = 16;
+
 struct KParams {
     FormParams fp;
     const double *N;   // [nqp][npe]
@@ -78,7 +83,8 @@ __global__ __launch_bounds__(kBlock) void k_element(const int32_t *__restrict__ 
                                                     int32_t *__restrict__ J, double *__restrict__ V) {
     constexpr int NP = NPE * (NPE + 1) / 2;
     constexpr bool GRAD = (FORM != FEASM_FORM_DOT);
-    constexpr int SW = (FORM == FEASM_FORM_LIN_ELASTIC) ? 9 : 1;
+    constexpr bool ELASTIC = (FORM == FEASM_FORM_LIN_ELASTIC || FORM == kElasticCubic);
+    constexpr int SW = ELASTIC ? 9 : 1;
     __shared__ double sX[kWaves][NPE * 3];
     __shared__ double sJ[kWaves][NQP * 9];
     __shared__ R sInv[kWaves][NQP * 9];
@@ -155,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_element(const int32_t *__restrict__ 
             int a, b;
             pair_of(p, a, b);
             R out[SW];
-            if (FORM == FEASM_FORM_LIN_ELASTIC || FORM == FEASM_FORM_DIFFUSION_MAT) {
+            if (ELASTIC || FORM == FEASM_FORM_DIFFUSION_MAT) {
                 R M[9];
 #pragma unroll
                 for (int i = 0; i < 9; i++) M[i] = (R)0;
@@ -172,7 +178,21 @@ __global__ __launch_bounds__(kBlock) void k_element(const int32_t *__restrict__ 
                             M[3 * k + l] += wq * prod;
                         }
                 }
-                if (FORM == FEASM_FORM_LIN_ELASTIC) {
+                if (FORM == kElasticCubic) {
+                    const R d00 = (R)kp.fp.coef[0], d01 = (R)kp.fp.coef[1], d33 = (R)kp.fp.coef[21];
+                    const R tr = (M[0] + M[4]) + M[8];
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+#pragma unroll
+                        for (int j = 0; j < 3; j++)
+                            out[3 * i + j] = (i == j) ? d00 * M[4 * i] + d33 * (tr - M[4 * i])
+                                                      : d01 * M[3 * i + j] + d33 * M[3 * j + i];
+                    if (a == b) {  // exact symmetry of the diagonal block
+                        out[3] = out[1];
+                        out[6] = out[2];
+                        out[7] = out[5];
+                    }
+                } else if (FORM == FEASM_FORM_LIN_ELASTIC) {
 #pragma unroll
                     for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -229,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_element(const int32_t *__restrict__ 
                 for (int k = 0; k < ndn; k++)
                     for (int l = 0; l < ndn; l++) {
                         double v;
-                        if (FORM == FEASM_FORM_LIN_ELASTIC)
+                        if (ELASTIC)
                             v = (double)out[3 * k + l];
                         else if (FORM == FEASM_FORM_DOT)
                             v = (double)out[0] * kp.fp.coef[ndn * k + l];
@@ -285,6 +305,8 @@ void by_form(feasm_ctx *ctx, const feasm_mesh *m, const KParams &kp, void *stage
         case FEASM_FORM_DIFFUSION_MAT:
             return launch_t<NPE, NQP, FEASM_FORM_DIFFUSION_MAT, R, OUT>(ctx, m, kp, stage, dofs, I, J, V);
         case FEASM_FORM_LIN_ELASTIC:
+            if (cubic_symmetry(kp.fp.coef))
+                return launch_t<NPE, NQP, kElasticCubic, R, OUT>(ctx, m, kp, stage, dofs, I, J, V);
             return launch_t<NPE, NQP, FEASM_FORM_LIN_ELASTIC, R, OUT>(ctx, m, kp, stage, dofs, I, J, V);
         case FEASM_FORM_DOT:
             return launch_t<NPE, NQP, FEASM_FORM_DOT, R, OUT>(ctx, m, kp, stage, dofs, I, J, V);

@@ -42,6 +42,8 @@ __device__ constexpr int voigt(int i, int k) {
 // K_ii = D00 M_ii + D33 (tr M - M_ii), three coefficients instead of 36.
 constexpr int kFormElasticCubic = 16;
 
+}  // namespace
+
 bool cubic_symmetry(const double *D) {
     for (int i = 0; i < 6; i++)
         for (int j = 0; j < 6; j++) {
@@ -56,6 +58,8 @@ bool cubic_symmetry(const double *D) {
         }
     return true;
 }
+
+namespace {
 
 struct FusedParams {
     FormParams fp;
