@@ -38,13 +38,16 @@ __global__ void k_n2e_count(const int32_t *__restrict__ conn, int64_t n, int32_t
         atomicAdd(&cnt[conn[i]], 1);
 }
 
+// each entry takes a slot of its node's list by counting the node's count
+// down (the list is sorted afterwards)
 __global__ void k_n2e_fill(const int32_t *__restrict__ conn, int64_t nelem, int npe,
-                           unsigned long long *__restrict__ cursor, uint32_t *__restrict__ n2e) {
+                           const int64_t *__restrict__ ptr, int32_t *__restrict__ cnt, uint32_t *__restrict__ n2e) {
     const int64_t n = nelem * npe;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t e = i / npe;
         const int a = (int)(i % npe);
-        const unsigned long long pos = atomicAdd(&cursor[conn[i]], 1ull);
+        const int32_t c = conn[i];
+        const int64_t pos = ptr[c] + (int64_t)atomicSub(&cnt[c], 1) - 1;
         n2e[pos] = ((uint32_t)e << 4) | (uint32_t)a;
     }
 }
@@ -507,12 +510,9 @@ void build_node_to_element(feasm_mesh *m) {
                        nent, dp<int32_t>(cnt));
     m->n2e_ptr = dev_alloc((m->nnodes + 1) * sizeof(int64_t));
     exclusive_scan_i32_to_i64(ctx, dp<int32_t>(cnt), dp<int64_t>(m->n2e_ptr), m->nnodes + 1);
-    DevPtr cursor = dev_alloc((m->nnodes + 1) * sizeof(int64_t));
-    FEASM_HIP(hipMemcpyAsync(cursor->p, m->n2e_ptr->p, (m->nnodes + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice,
-                             ctx->stream));
     m->n2e = dev_alloc(nent * sizeof(uint32_t));
     hipLaunchKernelGGL(k_n2e_fill, dim3(grid_for(nent, 256)), dim3(256), 0, ctx->stream, dp<int32_t>(m->conn),
-                       m->nelem, m->npe, dp<unsigned long long>(cursor), dp<uint32_t>(m->n2e));
+                       m->nelem, m->npe, dp<int64_t>(m->n2e_ptr), dp<int32_t>(cnt), dp<uint32_t>(m->n2e));
     hipLaunchKernelGGL(k_n2e_sort, dim3(grid_for(m->nnodes, 256)), dim3(256), 0, ctx->stream,
                        dp<int64_t>(m->n2e_ptr), dp<uint32_t>(m->n2e), m->nnodes);
     FEASM_HIP(hipGetLastError());
@@ -566,12 +566,13 @@ std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn, bool with_contrib
     // 2. adjacency: one pass into per-node room, then packed
     const int npe = m->npe;
     const int64_t nent = m->nelem * npe;  // node -> element entries
-    DevPtr deg = dev_alloc((nn + 1) * sizeof(int32_t));
-    DevPtr flags = dev_alloc(sizeof(int32_t));  // too many neighbours
+    // degrees [nn + 1] (the last one zero, for the scan) and the flags word
+    // after them (too many neighbours; lists left to k_contrib): one clear
+    DevPtr deg = dev_alloc((nn + 2) * sizeof(int32_t));
+    int32_t *flags = dp<int32_t>(deg) + nn + 1;
     DevPtr tmp = dev_alloc(std::max<size_t>((size_t)npe * nent * sizeof(int32_t), 4));
     DevPtr tmult = dev_alloc(std::max<size_t>((size_t)npe * nent * sizeof(uint16_t), 4));
-    FEASM_HIP(hipMemsetAsync(flags->p, 0, sizeof(int32_t), ctx->stream));
-    FEASM_HIP(hipMemsetAsync(dp<int32_t>(deg) + nn, 0, sizeof(int32_t), ctx->stream));
+    FEASM_HIP(hipMemsetAsync(dp<int32_t>(deg) + nn, 0, 2 * sizeof(int32_t), ctx->stream));
     const unsigned gadj = grid_for(nn, 1, 1 << 17);
     // the staged path's contribution lists come out of the adjacency sort
     // when the node numbers leave room for the candidate index in the keys
@@ -584,17 +585,17 @@ std::shared_ptr<Pattern> build_pattern(feasm_mesh *m, int ndn, bool with_contrib
     if (keyed)
         hipLaunchKernelGGL(k_adj<8>, dim3(gadj), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr),
                            dp<uint32_t>(m->n2e), dp<int32_t>(m->conn), npe, nn, dp<int32_t>(deg), dp<int32_t>(tmp),
-                           dp<uint16_t>(tmult), dp<int32_t>(flags), cout);
+                           dp<uint16_t>(tmult), flags, cout);
     else
         hipLaunchKernelGGL(k_adj<0>, dim3(gadj), dim3(64), 0, ctx->stream, dp<int64_t>(m->n2e_ptr),
                            dp<uint32_t>(m->n2e), dp<int32_t>(m->conn), npe, nn, dp<int32_t>(deg), dp<int32_t>(tmp),
-                           dp<uint16_t>(tmult), dp<int32_t>(flags), cout);
+                           dp<uint16_t>(tmult), flags, cout);
     FEASM_HIP(hipGetLastError());
     P->adj_ptr = dev_alloc((nn + 1) * sizeof(int64_t));
     exclusive_scan_i32_to_i64(ctx, dp<int32_t>(deg), dp<int64_t>(P->adj_ptr), nn + 1);
     int32_t fl = 0;
     const ReadBack rb[2] = {{&P->nblocks, dp<int64_t>(P->adj_ptr) + nn, sizeof(int64_t)},
-                            {&fl, flags->p, sizeof(int32_t)}};
+                            {&fl, flags, sizeof(int32_t)}};
     read_back(ctx->stream, rb, 2);
     FEASM_REQUIRE(!(fl & 1), FEASM_EUNSUPPORTED,
                   "a node has more than 256 distinct neighbours or more than 2048 (element, node) incidences");
