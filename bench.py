@@ -333,6 +333,50 @@ def bench_batched(feasm, core, steps=5, tune_db=None):
     return results
 
 
+def bench_coo(feasm, core, D, n, reps=3):
+    """The COO path on C2: the element kernel writes every element's 24 x 24
+    triples in assemble!'s order, then the device sparse() (radix sort of the
+    64-bit keys, segment sums in input order, column pointers) builds the CSC
+    matrix -- the path general DOF numberings and the assembler protocol take.
+    Wall time of whole feasm_bilform_coo calls (new matrix each), and the
+    result against the resident-pattern (fused) assembly: same pattern, values
+    within rounding."""
+    ctx = feasm.Context(0)
+    conn, xyz = core.gen_h8block(1.0, 1.0, 1.0, n, n, n)
+    xyz = core.gen_perturb(xyz, [0, 0, 0], [1, 1, 1], 0.15 / n, 2)
+    ne = conn.shape[0]
+    mesh = core.DeviceMesh(ctx, feasm.H8, conn, xyz)
+    del conn
+    mesh.set_dofs(3)
+    mesh.bilform(feasm.LIN_ELASTIC, feasm.RULE_GAUSS, 2, D, path="coo").close()  # warm-up
+    ts = []
+    for _ in range(reps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        Mc = mesh.bilform(feasm.LIN_ELASTIC, feasm.RULE_GAUSS, 2, D, path="coo")
+        ctx.synchronize()
+        ts.append(1e3 * (time.perf_counter() - t0))
+        if len(ts) < reps:
+            Mc.close()
+    Mf = mesh.bilform(feasm.LIN_ELASTIC, feasm.RULE_GAUSS, 2, D)
+    cp_c, rv_c, nz_c = Mc.csc_arrays(0)
+    cp_f, rv_f, nz_f = Mf.csc_arrays(0)
+    same_pattern = bool(np.array_equal(cp_c, cp_f) and np.array_equal(rv_c, rv_f))
+    del cp_c, rv_c, cp_f, rv_f
+    rel = float(np.linalg.norm(nz_c - nz_f) / np.linalg.norm(nz_f))
+    triples = ne * 24 * 24
+    ms = float(np.median(ts))
+    out = {"workload": "H8 %d^3 linear elasticity FP64 through the COO path (element triples + device sparse())" % n,
+           "ms": ms, "runs_ms": ts, "elements_per_s": ne / (ms * 1e-3), "triples": triples,
+           "triples_per_s": triples / (ms * 1e-3), "same_pattern_as_resident": same_pattern,
+           "rel_fro_vs_resident": rel}
+    Mc.close()
+    Mf.close()
+    mesh.close()
+    ctx.close()
+    return out
+
+
 def bench_served(feasm, core, rate=200.0, duration=3.0, workers=8):
     """C5: Poisson arrivals (rate/s) of H8/T10 (50/50) block meshes with log-normal
     element counts (median 5e4, sigma 1.0, clipped to [1e3, 2e6]), diffusion or
@@ -409,8 +453,8 @@ def main():
     ap.add_argument("--cells", type=int, default=100, help="cells per direction of the H8 block")
     ap.add_argument("--cpu-cells", type=int, default=50, help="cells per direction of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extras", default="mixed,vec,t10,batch,served,gram",
-                    help="comma list of extra workloads: mixed,vec,t10,batch,served,gram (or 'none')")
+    ap.add_argument("--extras", default="mixed,vec,t10,batch,served,gram,coo",
+                    help="comma list of extra workloads: mixed,vec,t10,batch,served,gram,coo (or 'none')")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB streaming copy (3 launches) to scale traffic counters")
     a = ap.parse_args()
@@ -551,6 +595,10 @@ def main():
     if "gram" in extras:
         log("element Gram, vector vs matrix cores ...")
         out["gram_probe"] = bench_gram(feasm)
+    if "coo" in extras:
+        log("COO path ...")
+        out["coo_path"] = bench_coo(feasm, core, D, n)
+        out["coo_path"]["ms_over_resident_fused"] = out["coo_path"]["ms"] / el_ms
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(a.cpu_cells, 2, D)
