@@ -153,6 +153,7 @@ TileShape feasm::mesh_tile_shape(feasm_mesh *m, int nqp, int form, int precision
 void feasm::numeric_into_ctx(feasm_ctx *ctx, feasm_mesh *m, const FormParams &fp, int rule, int order,
                              int precision, double *nzval) {
     const DevQTable &qt = device_qtable(ctx, m->etype, rule, order);
+    if (m->nelem == 0 || m->pattern->nnz == 0) return;  // no element: an empty matrix, nothing to integrate
     const bool fused = m->numeric_path != 1 && fused_supported(m->npe, qt.nqp);
     FEASM_REQUIRE(fused || m->numeric_path != 2, FEASM_EUNSUPPORTED,
                   "no fused kernel for this element type and rule");

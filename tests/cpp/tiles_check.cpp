@@ -86,6 +86,27 @@ static TileHostMesh h8(int nx, int ny, int nz, double thick, double amp, uint64_
     return h;
 }
 
+// an H8 block of which only the elements with centroid x < 0.45 are kept,
+// over all the block's nodes: the others are nodes of no element (a region
+// of a larger mesh assembled on the global node set)
+static TileHostMesh h8_region(int nx, int ny, int nz) {
+    TileHostMesh h;
+    h.npe = 8;
+    const int64_t ne = (int64_t)nx * ny * nz;
+    h.nn = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+    std::vector<int64_t> c(ne * 8), kept;
+    h.xyz.resize(h.nn * 3);
+    gen_h8block(1.0, 1.0, 1.0, nx, ny, nz, c.data(), h.xyz.data());
+    for (int64_t e = 0; e < ne; e++) {
+        double cx = 0;
+        for (int a = 0; a < 8; a++) cx += h.xyz[3 * (c[e * 8 + a] - 1)] / 8;
+        if (cx < 0.45) kept.insert(kept.end(), c.begin() + e * 8, c.begin() + e * 8 + 8);
+    }
+    h.ne = (int64_t)kept.size() / 8;
+    finish_mesh(h, kept);
+    return h;
+}
+
 static TileHostMesh t10(int n, uint64_t seed) {
     TileHostMesh h;
     h.npe = 10;
@@ -308,6 +329,9 @@ int main() {
     check_mesh("h8 11x9x7 fp64 128 nodes", a, shape_of(8, 8, true, 128, 160));
     check_mesh("h8 slab", slab, shape_of(8, 8, true, 64, 160));
     check_mesh("h8 one element", one, shape_of(8, 8, true, 64, 160));
+    const TileHostMesh region = h8_region(9, 7, 5);
+    check_mesh("h8 region, nodes of no element", region, shape_of(8, 8, true, 64, 160));
+    check_mesh("h8 region mixed", region, shape_of(8, 4, true, 64, 160));
     check_mesh("t10 6^3 fp64", b, shape_of(10, 8, true, 64, 160));
     check_mesh("t10 6^3 mixed", b, shape_of(10, 4, true, 64, 160));
     check_mesh("t10 6^3 dot", b, shape_of(10, 8, false, 32, 160));
