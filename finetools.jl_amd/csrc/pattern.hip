@@ -10,7 +10,9 @@
 //      (m, c) is a block of ndn x ndn entries;
 //   3. per block, the list of element contributions (element, upper node
 //      pair, transposed?) in ascending element order, so that the numeric
-//      phase sums duplicates in exactly the reference's order;
+//      phase sums duplicates in exactly the reference's order -- written by
+//      the adjacency sort itself when the staged path will run first (keyed
+//      candidates), else merged per block when the staged path first needs it;
 //   4. the DOF-level CSC colptr/rowval for the node-major numbering
 //      dof(node, k) = node*ndn + k (numberdofs! with no EBCs,
 //      src/FieldModule.jl:356-377), rows sorted within each column.

@@ -110,10 +110,10 @@ struct feasm_server {
     bool stop = false;
     int64_t completed = 0;
     std::string last_error;
-    // per worker: seconds waiting for an admissible request, converting and
-    // staging new meshes' arrays, waiting for a request's device work at its
-    // end, in run_one; requests run
-    // (FEASM_SERVER_TIMING=1 prints them to stderr when the server ends)
+    // per worker, in seconds: idle (waiting for an admissible request),
+    // converting and staging new meshes' arrays, waiting at the end of a
+    // request for its device work, and in run_one overall; and the requests
+    // run (FEASM_SERVER_TIMING=1 prints them to stderr when the server ends)
     struct WorkerTimes {
         double idle = 0, convert = 0, sync = 0, run = 0;
         int64_t n = 0;
