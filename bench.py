@@ -575,33 +575,38 @@ def main():
     mesh.close()
     ctx.close()
     extras = [x for x in a.extras.split(",") if x and x != "none"]
+
+    def extra(key, name, fn):
+        # an extra workload that fails is reported as such; the headline
+        # line above is printed either way
+        log("%s ..." % name)
+        try:
+            out[key] = fn()
+        except Exception as e:  # noqa: BLE001
+            log("%s failed: %r" % (name, e))
+            out[key] = {"error": repr(e)}
+
     if "mixed" in extras:
-        log("C2 mixed ...")
-        out["c2_mixed"] = bench_c2_mixed(feasm, core, D, n)
-        out["c2_mixed"]["kernel_ratio_vs_fp64"] = el_ms / out["c2_mixed"]["ms"]
+        extra("c2_mixed", "C2 mixed", lambda: bench_c2_mixed(feasm, core, D, n))
+        if "ms" in out["c2_mixed"]:
+            out["c2_mixed"]["kernel_ratio_vs_fp64"] = el_ms / out["c2_mixed"]["ms"]
     if "vec" in extras:
-        log("load vector ...")
-        out["vector"] = bench_vec(feasm, core)
+        extra("vector", "load vector", lambda: bench_vec(feasm, core))
     if "t10" in extras:
-        log("T10 ...")
-        out["t10"] = bench_t10(feasm, core)
+        extra("t10", "T10", lambda: bench_t10(feasm, core))
     if "batch" in extras:
-        log("batched ...")
-        out["batched"] = bench_batched(feasm, core, tune_db=os.path.join(ROOT, "finetools.jl_amd", "feasm",
-                                                                          "autotune.txt"))
+        extra("batched", "batched", lambda: bench_batched(
+            feasm, core, tune_db=os.path.join(ROOT, "finetools.jl_amd", "feasm", "autotune.txt")))
     if "served" in extras:
-        log("served ...")
-        out["served"] = bench_served(feasm, core)
+        extra("served", "served", lambda: bench_served(feasm, core))
     if "gram" in extras:
-        log("element Gram, vector vs matrix cores ...")
-        out["gram_probe"] = bench_gram(feasm)
+        extra("gram_probe", "element Gram, vector vs matrix cores", lambda: bench_gram(feasm))
     if "coo" in extras:
-        log("COO path ...")
-        out["coo_path"] = bench_coo(feasm, core, D, n)
-        out["coo_path"]["ms_over_resident_fused"] = out["coo_path"]["ms"] / el_ms
+        extra("coo_path", "COO path", lambda: bench_coo(feasm, core, D, n))
+        if "ms" in out["coo_path"]:
+            out["coo_path"]["ms_over_resident_fused"] = out["coo_path"]["ms"] / el_ms
     if not a.no_cpu:
-        log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(a.cpu_cells, 2, D)
+        extra("cpu_baseline", "cpu baseline", lambda: cpu_baseline(a.cpu_cells, 2, D))
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
