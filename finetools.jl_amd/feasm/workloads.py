@@ -100,12 +100,18 @@ def run_served(stream, meshes, precision, workers, paced=True, sequential=False,
 
     def collect(k, rid, ne):
         st, lat, _ = srv.wait(rid)
-        rec = {"index": k, "elements": ne, "status": st, "latency_ms": lat, "digest": None, "csc": None}
+        rec = {"index": k, "elements": ne, "status": st, "latency_ms": lat, "digest": None, "csc": None,
+               "error": None}
         if st == 2:
             if digests:
                 rec["digest"] = srv.result(rid).checksum()
             if keep_results is not None and k in keep_results:
                 rec["csc"] = srv.result(rid).csc_arrays(1)
+        else:
+            try:
+                srv.result(rid)
+            except L.FeasmError as e:  # the request's own error text
+                rec["error"] = str(e)
         srv.release(rid)
         records.append(rec)
 
@@ -116,9 +122,15 @@ def run_served(stream, meshes, precision, workers, paced=True, sequential=False,
             et, conn, xyz, rule = meshes[pi]
             ndn = 1 if form == L.DIFFUSION_ISO else 3
             key = (pi, ndn)
+            rid = None
             if re and key in resident:
-                rid = srv.submit_reassembly(resident[key], form, rule[0], rule[1], coeff, precision)
-            else:
+                try:
+                    rid = srv.submit_reassembly(resident[key], form, rule[0], rule[1], coeff, precision)
+                except L.FeasmError:
+                    # its mesh is not resident (its first request failed, or it
+                    # was evicted): the client sends the mesh again
+                    del resident[key]
+            if rid is None:
                 rid, mid = srv.submit_mesh(et, conn, xyz, ndn, form, rule[0], rule[1], coeff, precision,
                                            keep=key not in resident, borrow=True)
                 if key not in resident:
@@ -139,6 +151,10 @@ def run_served(stream, meshes, precision, workers, paced=True, sequential=False,
 def served_summary(records, wall):
     lats = np.array([r["latency_ms"] for r in records if r["status"] == 2])
     ne = sum(r["elements"] for r in records if r["status"] == 2)
-    return {"requests": len(records), "completed": int(len(lats)), "wall_s": wall, "elements_per_s": ne / wall,
-            "p50_ms": float(np.percentile(lats, 50)), "p99_ms": float(np.percentile(lats, 99)),
-            "mean_ms": float(lats.mean())}
+    out = {"requests": len(records), "completed": int(len(lats)), "wall_s": wall, "elements_per_s": ne / wall,
+           "p50_ms": float(np.percentile(lats, 50)), "p99_ms": float(np.percentile(lats, 99)),
+           "mean_ms": float(lats.mean())}
+    errors = sorted({r["error"] or "status %d" % r["status"] for r in records if r["status"] != 2})
+    if errors:
+        out["errors"] = errors[:5]
+    return out

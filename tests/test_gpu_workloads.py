@@ -174,7 +174,7 @@ def test_c5_served_stream_same_alone_and_in_the_mix():
     say("C5 one at a time: %.2f s" % wall1)
     assert len(mix) == len(alone) == len(stream)
     for a, b in zip(mix, alone):
-        assert a["status"] == 2 and b["status"] == 2, (a["index"], a["status"], b["status"])
+        assert a["status"] == 2 and b["status"] == 2, (a["index"], a["status"], a["error"], b["status"], b["error"])
         assert a["digest"] == b["digest"], a["index"]
     for k in sample:
         a, b = mix[k], alone[k]
