@@ -129,25 +129,6 @@ constexpr int kMaxCand = 2048;  // candidates of one node: incident elements x n
 // registers: lane l holds entries l + 64 m.  Partners less than 64 apart
 // are exchanged with lane shuffles, partners 64 or more apart sit in two
 // registers of the same lane.
-// The value of lane (lane ^ j) of a wavefront, j a power of two below 64:
-// exchanges within rows of 16 lanes as DPP moves (quad permutations, row
-// rotations: a rotation by r gives lane i the value of lane i - r of its
-// row), across 16 lanes as a swizzle, across the halves as a permute.
-__device__ __forceinline__ int32_t xor_lane(int32_t x, int j, int lane) {
-    switch (j) {
-        case 1: return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
-        case 2: return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
-        case 4: {
-            const int32_t from_below = __builtin_amdgcn_mov_dpp(x, 0x124, 0xF, 0xF, false);  // row_ror 4: lane i - 4
-            const int32_t from_above = __builtin_amdgcn_mov_dpp(x, 0x12C, 0xF, 0xF, false);  // row_ror 12: lane i + 4
-            return (lane & 4) ? from_below : from_above;
-        }
-        case 8: return __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);  // row_ror 8
-        case 16: return __builtin_amdgcn_ds_swizzle(x, 0x401F);              // xor 16 within 32 lanes
-        default: return __shfl_xor(x, j, 64);
-    }
-}
-
 template <int E>
 __device__ __forceinline__ void wave_sort(int32_t (&x)[E], int lane) {
 #pragma unroll
@@ -174,7 +155,7 @@ __device__ __forceinline__ void wave_sort(int32_t (&x)[E], int lane) {
 #pragma unroll
                 for (int m = 0; m < E; m++) {
                     const int i = lane + 64 * m;
-                    const int32_t y = xor_lane(x[m], j, lane);
+                    const int32_t y = lane_xor(x[m], j, lane);
                     const bool take_min = ((i & k) == 0) == ((i & j) == 0);
                     x[m] = take_min ? min(x[m], y) : max(x[m], y);
                 }
