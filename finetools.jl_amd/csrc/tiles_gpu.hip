@@ -144,13 +144,19 @@ __device__ __forceinline__ void sort_reg_stage(uint32_t (&x)[E], int t, int k) {
 }
 
 template <int E>
-__device__ void block_sort_e(uint32_t *a, int n) {
+__device__ void block_sort_e(uint32_t *a) {
+    // n = E kTB entries: every stage's distance is a compile-time constant
+    // once the loops are unrolled, so the lane exchanges are DPP moves
+    constexpr int N = E * kTB;
     const int t = threadIdx.x;
+    const int lane = t & 63;
     uint32_t x[E];
 #pragma unroll
     for (int m = 0; m < E; m++) x[m] = a[t + kTB * m];
-    for (int k = 2; k <= n; k <<= 1) {
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
         int j = k >> 1;
+#pragma unroll
         for (; j >= kTB; j >>= 1) {
             const int jj = j / kTB;
             if (jj == 1) sort_reg_stage<E, 1>(x, t, k);
@@ -163,6 +169,7 @@ __device__ void block_sort_e(uint32_t *a, int n) {
 #pragma unroll
             for (int m = 0; m < E; m++) a[t + kTB * m] = x[m];
             __syncthreads();
+#pragma unroll
             for (; j >= 64; j >>= 1) {
 #pragma unroll
                 for (int m = 0; m < E; m++) {
@@ -180,11 +187,12 @@ __device__ void block_sort_e(uint32_t *a, int n) {
 #pragma unroll
             for (int m = 0; m < E; m++) x[m] = a[t + kTB * m];
         }
+#pragma unroll
         for (; j > 0; j >>= 1) {
 #pragma unroll
             for (int m = 0; m < E; m++) {
                 const int i = t + kTB * m;
-                const uint32_t y = (uint32_t)__shfl_xor((int)x[m], j, 64);
+                const uint32_t y = (uint32_t)lane_xor((int)x[m], j, lane);
                 const bool take_min = ((i & k) == 0) == ((i & j) == 0);
                 x[m] = take_min ? min(x[m], y) : max(x[m], y);
             }
@@ -205,9 +213,9 @@ __device__ void block_sort(uint32_t *a, int n) {
         n = kTB;
     }
     switch (n / kTB) {
-        case 1: block_sort_e<1>(a, n); return;
-        case 2: block_sort_e<2>(a, n); return;
-        case 4: block_sort_e<4>(a, n); return;
+        case 1: block_sort_e<1>(a); return;
+        case 2: block_sort_e<2>(a); return;
+        case 4: block_sort_e<4>(a); return;
         default: break;
     }
     // larger sorts stay in LDS (more entries per thread would cost registers)
