@@ -70,11 +70,44 @@ size_t size_class(size_t n) {
     return (n + step - 1) / step * step;
 }
 
+// a query that does not return success leaves its status as the calling
+// thread's last error, where the next launch check (ours or a library's)
+// would take it for its own: cleared here
+void clear_query_status(hipError_t q) {
+    if (q != hipSuccess && hipPeekAtLastError() == q) (void)hipGetLastError();
+}
+
 bool ready(Block &b) {
     if (b.unfenced) return false;
-    for (hipEvent_t e : b.evs)
-        if (hipEventQuery(e) != hipSuccess) return false;
+    for (hipEvent_t e : b.evs) {
+        const hipError_t q = hipEventQuery(e);
+        if (q != hipSuccess) {
+            clear_query_status(q);
+            return false;
+        }
+    }
     return true;
+}
+
+// Drops every completed fence of the cached blocks.  Called when a stream is
+// unregistered (it is idle then, so every fence recorded on it is complete):
+// no cached block keeps an event of a stream that is about to be destroyed,
+// since querying an event whose stream is gone is not safe.
+void drop_completed_fences_locked(Pool &P) {
+    for (auto &kv : P.cached) {
+        Block &b = kv.second;
+        for (size_t i = 0; i < b.evs.size();) {
+            const hipError_t q = hipEventQuery(b.evs[i]);
+            if (q == hipSuccess) {
+                P.spare.push_back(b.evs[i]);
+                b.evs[i] = b.evs.back();
+                b.evs.pop_back();
+            } else {
+                clear_query_status(q);
+                i++;
+            }
+        }
+    }
 }
 
 void recycle_events(Pool &P, Block &b) {
@@ -222,7 +255,9 @@ void pool_unregister_stream(int device, hipStream_t s) {
             P.streams.erase(P.streams.begin() + i);
             break;
         }
-    // cached blocks stay for the next contexts (feasm_pool_trim frees them)
+    // cached blocks stay for the next contexts (feasm_pool_trim frees them),
+    // without the fences recorded on this stream
+    drop_completed_fences_locked(P);
 }
 
 size_t pool_trim(int device) {
