@@ -397,14 +397,18 @@ def bench_served(feasm, core, rate=200.0, duration=3.0, workers=8):
         rec, wall = W.run_served(stream, meshes, prec, w, paced=paced)
         out[key] = W.served_summary(rec, wall)
         log("%s: %s" % (key, out[key]))
-    # at saturation (all requests at once): two interleaved rounds of each,
-    # summarised by their mean throughput (each round's throughput kept) and
-    # the last round's latencies
+    # at saturation (all requests at once): one unmeasured round of each (the
+    # memory pool and the pinned staging buffers reach the sizes this
+    # concurrency needs, as in a server that has been running), then two
+    # interleaved rounds of each, summarised by their mean throughput (each
+    # round's throughput kept) and the last round's latencies
     sat = {"saturated_served_mixed": (feasm.MIXED, workers), "saturated_plain_fp64_sequential": (feasm.FP64, 1)}
     rounds = {k: [] for k in sat}
-    for _ in range(2):
+    for r in range(3):
         for key, (prec, w) in sat.items():
             rec, wall = W.run_served(stream, meshes, prec, w, paced=False)
+            if r == 0:
+                continue
             rounds[key].append(W.served_summary(rec, wall))
             log("%s: %s" % (key, rounds[key][-1]))
     for key, rs in rounds.items():
