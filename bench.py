@@ -392,23 +392,24 @@ def bench_served(feasm, core, rate=200.0, duration=3.0, workers=8):
     log("served: pool of %d meshes generated in %.1f s" % (len(meshes), time.perf_counter() - t0))
     out = {"workload": "Poisson %.0f req/s for %.1f s, H8/T10 50/50, median 5e4 elements (sigma 1.0), 30%% "
                        "re-assembly of resident meshes" % (rate, duration)}
+    # one unmeasured round of each path at saturation first: the memory pool
+    # and the pinned staging buffers reach the sizes the stream needs, as in
+    # a server that has been running
+    sat = {"saturated_served_mixed": (feasm.MIXED, workers), "saturated_plain_fp64_sequential": (feasm.FP64, 1)}
+    for key, (prec, w) in sat.items():
+        W.run_served(stream, meshes, prec, w, paced=False)
     for key, prec, w, paced in (("served_mixed", feasm.MIXED, workers, True),
                                 ("plain_fp64_sequential", feasm.FP64, 1, True)):
         rec, wall = W.run_served(stream, meshes, prec, w, paced=paced)
         out[key] = W.served_summary(rec, wall)
         log("%s: %s" % (key, out[key]))
-    # at saturation (all requests at once): one unmeasured round of each (the
-    # memory pool and the pinned staging buffers reach the sizes this
-    # concurrency needs, as in a server that has been running), then two
-    # interleaved rounds of each, summarised by their mean throughput (each
-    # round's throughput kept) and the last round's latencies
-    sat = {"saturated_served_mixed": (feasm.MIXED, workers), "saturated_plain_fp64_sequential": (feasm.FP64, 1)}
+    # at saturation (all requests at once): two interleaved rounds of each,
+    # summarised by their mean throughput (each round's throughput kept) and
+    # the last round's latencies
     rounds = {k: [] for k in sat}
-    for r in range(3):
+    for r in range(2):
         for key, (prec, w) in sat.items():
             rec, wall = W.run_served(stream, meshes, prec, w, paced=False)
-            if r == 0:
-                continue
             rounds[key].append(W.served_summary(rec, wall))
             log("%s: %s" % (key, rounds[key][-1]))
     for key, rs in rounds.items():
