@@ -1023,6 +1023,10 @@ __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *x
                 // the next element's reads (independent of this one's choice)
                 const int en = idx + 1 < H ? eorder[idx + 1] : 0;
                 const int n0 = (int)occcnt[en], n1 = (int)occcnt[en + 1];
+                // this element's reads for the load update below, loaded ahead
+                // of the choice (they do not depend on it)
+                const uint32_t pre0 = o0 + lane < o1 ? occ[o0 + lane] : 0u;
+                const uint32_t pre1 = o0 + lane + 64 < o1 ? occ[o0 + lane + 64] : 0u;
                 uint32_t part = 0;
                 for (int i0 = o0 + sub; i0 < o1; i0 += 32) {  // eight reads at a time, no branches
                     uint32_t x[8];
@@ -1042,18 +1046,24 @@ __global__ __launch_bounds__(kTB) void k_tile_build(MeshView mv, const double *x
                                        : 0xffffffffu;
                 key = min(key, (uint32_t)__builtin_amdgcn_mov_dpp((int)key, 0x141, 0xf, 0xf, false));  // row_half_mirror
                 key = min(key, (uint32_t)__builtin_amdgcn_mov_dpp((int)key, 0x140, 0xf, 0xf, false));  // row_mirror
-                key = min(key, (uint32_t)__shfl_xor((int)key, 16, 64));
-                key = min(key, (uint32_t)__shfl_xor((int)key, 32, 64));
-                const int best = (int)((uint32_t)__builtin_amdgcn_readfirstlane((int)key) & 15u);
+                // every lane of a row of 16 holds the row's minimum: the four rows
+                // are combined in scalar registers
+                const uint32_t k0 = (uint32_t)__builtin_amdgcn_readlane((int)key, 0);
+                const uint32_t k1 = (uint32_t)__builtin_amdgcn_readlane((int)key, 16);
+                const uint32_t k2 = (uint32_t)__builtin_amdgcn_readlane((int)key, 32);
+                const uint32_t k3 = (uint32_t)__builtin_amdgcn_readlane((int)key, 48);
+                const int best = (int)(min(min(k0, k1), min(k2, k3)) & 15u);
                 if (c == best) {
                     if (sub == 0) perm[e] = slotl[16 + cstart + used];
                     used++;
                 }
-                for (int i = o0 + lane; i < o1; i += 64) {
-                    const uint32_t xx = occ[i];
+                auto bump = [&](uint32_t xx) {
                     uint8_t &ld = load[(xx >> 4) * 16 + (((xx & 15u) + (uint32_t)best) & 15u)];
                     if (ld < 255) ld = (uint8_t)(ld + 1);
-                }
+                };
+                if (o0 + lane < o1) bump(pre0);
+                if (o0 + lane + 64 < o1) bump(pre1);
+                for (int i = o0 + lane + 128; i < o1; i += 64) bump(occ[i]);
                 e = en;
                 o0 = n0;
                 o1 = n1;
