@@ -462,6 +462,9 @@ def main():
                     help="comma list of extra workloads: mixed,vec,t10,batch,served,gram,coo (or 'none')")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB streaming copy (3 launches) to scale traffic counters")
+    ap.add_argument("--path", default="auto", choices=["auto", "staged"],
+                    help="numeric path of the C2 workload: auto (the fused kernel) or staged (element + gather "
+                         "kernels, the fallback), for profiling the latter")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X; "
@@ -501,6 +504,8 @@ def main():
     mesh = core.DeviceMesh(ctx, feasm.H8, conn, xyz)
     del conn
     mesh.set_dofs(3)
+    if a.path != "auto":
+        mesh.set_numeric_path(a.path)
     ctx.synchronize()
     t0 = time.perf_counter()
     mesh.build_pattern()
