@@ -171,6 +171,38 @@ def bench_vec(feasm, core, n=100, steps=10):
     return out
 
 
+def bench_lumped(feasm, core, n=100, steps=10):
+    """HRZ-lumped mass (bilform_dot into SysmatAssemblerSparseHRZLumpingSymm +
+    makematrix!) of rho I3 on the C2 mesh (H8 100^3, 3 DOF/node, 2x2x2
+    Gauss): stream time per assembly (element kernel, node gather, scan,
+    compaction and the nnz read-back), and the total mass, which must equal
+    ndn * rho * volume."""
+    ctx = feasm.Context(0)
+    conn, xyz = core.gen_h8block(1.0, 1.0, 1.0, n, n, n)
+    xyz = core.gen_perturb(xyz, [0, 0, 0], [1, 1, 1], 0.15 / n, 2)
+    ne, nn = conn.shape[0], xyz.shape[0]
+    mesh = core.DeviceMesh(ctx, feasm.H8, conn, xyz)
+    del conn, xyz
+    mesh.set_dofs(3)
+    rho = 7.8e-9
+    c = (rho * np.eye(3)).ravel()
+    M = mesh.bilform_dot_lumped(feasm.RULE_GAUSS, 2, c)
+    total = float(M.csc_arrays(1)[2].sum())
+    M.close()
+    ctx.set_timing(steps)
+    for _ in range(steps):
+        mesh.bilform_dot_lumped(feasm.RULE_GAUSS, 2, c).close()
+    ctx.synchronize()
+    ms = float(np.mean(ctx.timings()[:, 0]))
+    ctx.set_timing(0)
+    out = {"workload": "H8 %d^3 HRZ-lumped mass rho I3, 3 DOF/node, 2x2x2 Gauss" % n, "ms": ms,
+           "elements_per_s": ne / (ms * 1e-3), "total_mass_rel_err": abs(total - 3 * rho) / (3 * rho),
+           "nodes": nn}
+    mesh.close()
+    ctx.close()
+    return out
+
+
 def bench_c2_mixed(feasm, core, D, n=100, steps=10):
     """C2 in mixed precision (FP32 element arithmetic with FP64 Jacobians,
     FP64 sums): device time per re-assembly and the relative Frobenius error
@@ -458,8 +490,8 @@ def main():
     ap.add_argument("--cells", type=int, default=100, help="cells per direction of the H8 block")
     ap.add_argument("--cpu-cells", type=int, default=50, help="cells per direction of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extras", default="mixed,vec,t10,batch,served,gram,coo",
-                    help="comma list of extra workloads: mixed,vec,t10,batch,served,gram,coo (or 'none')")
+    ap.add_argument("--extras", default="mixed,vec,lumped,t10,batch,served,gram,coo",
+                    help="comma list of extra workloads: mixed,vec,lumped,t10,batch,served,gram,coo (or 'none')")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB streaming copy (3 launches) to scale traffic counters")
     ap.add_argument("--path", default="auto", choices=["auto", "staged"],
@@ -602,6 +634,8 @@ def main():
             out["c2_mixed"]["kernel_ratio_vs_fp64"] = el_ms / out["c2_mixed"]["ms"]
     if "vec" in extras:
         extra("vector", "load vector", lambda: bench_vec(feasm, core))
+    if "lumped" in extras:
+        extra("lumped", "lumped mass", lambda: bench_lumped(feasm, core))
     if "t10" in extras:
         extra("t10", "T10", lambda: bench_t10(feasm, core))
     if "batch" in extras:
