@@ -58,6 +58,34 @@ def test_lumped_mass_matches_oracle(ctx, case):
     m.close()
 
 
+RULE_CASES = [
+    ("H8 Gauss 1", O.H8, (3, 3, 2), 3, (L.RULE_GAUSS, 1)),
+    ("H8 Gauss 3", O.H8, (4, 3, 3), 3, (L.RULE_GAUSS, 3)),   # 27 points: 64-thread blocks
+    ("H8 Gauss 4", O.H8, (3, 3, 3), 2, (L.RULE_GAUSS, 4)),   # 64 points, the largest rule
+    ("T10 Tet 1", O.T10, (2, 2, 2), 3, (L.RULE_TET, 1)),
+    ("T10 Tet 5", O.T10, (2, 3, 2), 1, (L.RULE_TET, 5)),     # a negative weight
+]
+
+
+@pytest.mark.parametrize("case", RULE_CASES, ids=[c[0] for c in RULE_CASES])
+def test_lumped_mass_other_rules(ctx, case):
+    name, et, n, ndn, rule = case
+    say("case", name)
+    conn, xyz, _ = mesh_for(et, n, 0.1, 5)
+    c = np.diag(np.arange(1.0, ndn + 1.0)) + 0.25 * (np.ones((ndn, ndn)) - np.eye(ndn))
+    m = core.DeviceMesh(ctx, et, conn, xyz)
+    m.set_dofs(ndn)
+    M = m.bilform_dot_lumped(rule[0], rule[1], c)
+    cp, rv, nz = M.csc_arrays(1)
+    ocp, orv, onz = O.hrz_lumped_csc(et, conn, xyz, ndn, c.ravel(), rule=rule)
+    assert np.array_equal(cp, ocp) and np.array_equal(rv, orv)
+    err = np.linalg.norm(nz - onz) / np.linalg.norm(onz)
+    say(name, "rel err", err)
+    assert err <= 1e-12
+    M.close()
+    m.close()
+
+
 def test_general_numbering_and_untouched_dofs(ctx):
     conn, xyz, rule = mesh_for(O.H8, (3, 2, 2), 0.1, 3)
     nn = xyz.shape[0]
