@@ -126,8 +126,11 @@ void check_dofs_like_assemble(const feasm_mesh *m, int ndn, int64_t n, const std
     }
 }
 
-void ensure_pattern(feasm_mesh *m) {
-    if (!m->pattern || m->pattern->ndn != m->ndn) m->pattern = build_pattern(m, m->ndn);
+// staged: the staged path will integrate (the mesh asks for it, or the rule
+// has no fused kernel), so the adjacency sort also writes its contribution
+// lists (a fused-path mesh does not need them)
+void ensure_pattern(feasm_mesh *m, bool staged = false) {
+    if (!m->pattern || m->pattern->ndn != m->ndn) m->pattern = build_pattern(m, m->ndn, staged);
 }
 
 void *ensure_stage(feasm_mesh *m, int form, int precision) {
@@ -628,7 +631,8 @@ extern "C" int feasm_bilform(feasm_mesh *m, int form, int rule, int order, const
         *out = coo_path(m, fp, rule, order, precision);
         return FEASM_OK;
     }
-    ensure_pattern(m);
+    ensure_pattern(m, m->numeric_path == 1 ||
+                          !fused_supported(m->npe, device_qtable(m->ctx, m->etype, rule, order).nqp));
     std::unique_ptr<feasm_matrix> M(new feasm_matrix);
     M->ctx = m->ctx;
     M->m = M->n = m->pattern->n;
