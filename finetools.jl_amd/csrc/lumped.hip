@@ -65,41 +65,60 @@ __global__ __launch_bounds__(256) void k_hrz_elem(const int32_t *__restrict__ co
     }
     // em2: the column (b, cq) sums of Ke[(a, p), (b, cq)] = sum_q (N_a N_b J w) c[p][cq], added up
     double em2 = 0.0;
-    double diag[NPE][NDN];
+    // the row nodes in groups of AC (fewer live accumulators); the column
+    // sums still add the rows in ascending (a, p) order
+    constexpr int AC = NDN == 3 ? NPE / 2 : NPE;
     for (int b = 0; b < NPE; b++) {
-        double ke[NPE][NDN][NDN];
-#pragma unroll
-        for (int a = 0; a < NPE; a++)
-#pragma unroll
-            for (int p = 0; p < NDN; p++)
-#pragma unroll
-                for (int k = 0; k < NDN; k++) ke[a][p][k] = 0.0;
-        for (int q = 0; q < nqp; q++) {
-            const double nb = N[q * NPE + b], jq = jt[q * nt], wq = w[q];
-#pragma unroll
-            for (int a = 0; a < NPE; a++) {
-                const double factor = N[q * NPE + a] * nb * jq * wq;
-#pragma unroll
-                for (int p = 0; p < NDN; p++)
-#pragma unroll
-                    for (int k = 0; k < NDN; k++) ke[a][p][k] += factor * c[p][k];
-            }
-        }
         double cs[NDN];
 #pragma unroll
         for (int k = 0; k < NDN; k++) cs[k] = 0.0;
 #pragma unroll
-        for (int a = 0; a < NPE; a++) {
+        for (int a0 = 0; a0 < NPE; a0 += AC) {
+            double ke[AC][NDN][NDN];
 #pragma unroll
-            for (int p = 0; p < NDN; p++)
+            for (int i = 0; i < AC; i++)
 #pragma unroll
-                for (int k = 0; k < NDN; k++) cs[k] += ke[a][p][k];
-            if (a == b)
+                for (int p = 0; p < NDN; p++)
 #pragma unroll
-                for (int p = 0; p < NDN; p++) diag[a][p] = ke[a][p][p];
+                    for (int k = 0; k < NDN; k++) ke[i][p][k] = 0.0;
+            for (int q = 0; q < nqp; q++) {
+                const double nb = N[q * NPE + b], jq = jt[q * nt], wq = w[q];
+#pragma unroll
+                for (int i = 0; i < AC; i++) {
+                    const double factor = N[q * NPE + a0 + i] * nb * jq * wq;
+#pragma unroll
+                    for (int p = 0; p < NDN; p++)
+#pragma unroll
+                        for (int k = 0; k < NDN; k++) ke[i][p][k] += factor * c[p][k];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < AC; i++) {
+#pragma unroll
+                for (int p = 0; p < NDN; p++)
+#pragma unroll
+                    for (int k = 0; k < NDN; k++) cs[k] += ke[i][p][k];
+            }
         }
 #pragma unroll
         for (int k = 0; k < NDN; k++) em2 += cs[k];
+    }
+    // the diagonal entries Ke[(a, p), (a, p)], formed as the loop above
+    // forms them for b = a (the same operations, so the same values)
+    double diag[NPE][NDN];
+#pragma unroll
+    for (int a = 0; a < NPE; a++)
+#pragma unroll
+        for (int p = 0; p < NDN; p++) diag[a][p] = 0.0;
+    for (int q = 0; q < nqp; q++) {
+        const double jq = jt[q * nt], wq = w[q];
+#pragma unroll
+        for (int a = 0; a < NPE; a++) {
+            const double na = N[q * NPE + a];
+            const double factor = na * na * jq * wq;
+#pragma unroll
+            for (int p = 0; p < NDN; p++) diag[a][p] += factor * c[p][p];
+        }
     }
     double dem2 = 0.0;
 #pragma unroll
