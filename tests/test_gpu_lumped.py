@@ -117,6 +117,24 @@ def test_c2_size_total_mass(ctx):
     m.close()
 
 
+def test_t10_full_size_total_mass(ctx):
+    """T10 60^3 x 6 (1.3 M elements): every DOF gets a positive mass and the
+    total is ndn * rho * volume (HRZ keeps each element's total, and the
+    4-point rule integrates det J of these straight-sided tetrahedra exactly)."""
+    n = 60
+    conn, xyz = core.gen_t10block(1.0, 1.0, 1.0, n, n, n, 0.0, 0.0, 1)
+    m = core.DeviceMesh(ctx, O.T10, conn, xyz)
+    m.set_dofs(3)
+    rho = 2.7e-9
+    M = m.bilform_dot_lumped(L.RULE_TET, 4, rho * np.eye(3))
+    cp, rv, nz = M.csc_arrays(1)
+    assert len(nz) == xyz.shape[0] * 3 and np.all(nz > 0)
+    assert np.array_equal(rv, np.arange(1, len(nz) + 1))
+    np.testing.assert_allclose(nz.sum(), 3 * rho, rtol=1e-10)
+    M.close()
+    m.close()
+
+
 def test_errors_and_mirror(ctx):
     conn, xyz, rule = mesh_for(O.H8, (2, 2, 2), 0.0, 0)
     m = core.DeviceMesh(ctx, O.H8, conn, xyz)
