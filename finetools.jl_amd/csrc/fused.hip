@@ -242,10 +242,11 @@ struct Staged {
 };
 // chunks prefetched per thread (measured: the FP64 and DOT kernels gain from
 // a second chunk on the larger T10 blobs and nothing from a third; the
-// mixed-precision gradient kernels are faster with one)
+// mixed-precision gradient kernels are faster with one, and so is FP64 scalar
+// diffusion, where the compiler spilled a second chunk around every task)
 template <typename R, int FORM>
 constexpr int staged_chunks() {
-    return (sizeof(R) == 4 && FORM != FEASM_FORM_DOT) ? 1 : 2;
+    return ((sizeof(R) == 4 && FORM != FEASM_FORM_DOT) || FORM == FEASM_FORM_DIFFUSION_ISO) ? 1 : 2;
 }
 
 // A tile header.  With `hdr` a __restrict__ kernel argument (nothing the
