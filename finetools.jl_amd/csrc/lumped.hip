@@ -134,8 +134,9 @@ __global__ __launch_bounds__(256) void k_hrz_elem(const int32_t *__restrict__ co
   }
 }
 
-// The node's list entries are read four at a time, so the loads of a group
-// are in flight together; the values are still added in list order.
+// The node's list entries are read eight at a time (all of an interior H8
+// node's), so the loads of a group are in flight together; the values are
+// still added in list order.
 template <int NDN>
 __global__ __launch_bounds__(256) void k_hrz_gather(const int64_t *__restrict__ n2e_ptr,
                                                     const uint32_t *__restrict__ n2e, int64_t nnodes, int npe,
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(256) void k_hrz_gather(const int64_t *__restrict__ 
     double acc[NDN];
 #pragma unroll
     for (int k = 0; k < NDN; k++) acc[k] = 0.0;
-    constexpr int U = 4;
+    constexpr int U = 8;
     for (int64_t s = s0; s < s1; s += U) {
         uint32_t v[U];
 #pragma unroll

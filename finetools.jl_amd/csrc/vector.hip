@@ -136,22 +136,40 @@ __global__ __launch_bounds__(256) void k_vec_factor_gather(const int64_t *__rest
          n += (int64_t)gridDim.x * blockDim.x) {
         if (n2e_ptr[n] == n2e_ptr[n + 1]) continue;  // a node of no element: its DOFs are never touched
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-        for (int64_t s = n2e_ptr[n]; s < n2e_ptr[n + 1]; s++) {
-            const uint32_t v = n2e[s];
-            const int64_t e = v >> 4;
-            const int a = (int)(v & 15u);
-            const double *fe = fac + e * NQP;
-            double ev0 = 0.0, ev1 = 0.0, ev2 = 0.0;
+        // the node's elements four at a time: their list entries, then their
+        // point factors, are loaded before any is used (the sums still run
+        // in list order)
+        constexpr int U = 4;
+        const int64_t s0 = n2e_ptr[n], s1 = n2e_ptr[n + 1];
+        for (int64_t s = s0; s < s1; s += U) {
+            uint32_t v[U];
 #pragma unroll
-            for (int q = 0; q < NQP; q++) {
-                const double NkxF = sN[q * NPE + a] * fe[q];
-                ev0 = ev0 + NkxF * f0;
-                ev1 = ev1 + NkxF * f1;
-                ev2 = ev2 + NkxF * f2;
+            for (int j = 0; j < U; j++) v[j] = s + j < s1 ? n2e[s + j] : 0u;
+            double fq[U][NQP];
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                const double *fe = fac + (int64_t)(v[j] >> 4) * NQP;
+                if (s + j < s1) {
+#pragma unroll
+                    for (int q = 0; q < NQP; q++) fq[j][q] = fe[q];
+                }
             }
-            acc0 = acc0 + ev0;
-            acc1 = acc1 + ev1;
-            acc2 = acc2 + ev2;
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                if (s + j >= s1) break;
+                const int a = (int)(v[j] & 15u);
+                double ev0 = 0.0, ev1 = 0.0, ev2 = 0.0;
+#pragma unroll
+                for (int q = 0; q < NQP; q++) {
+                    const double NkxF = sN[q * NPE + a] * fq[j][q];
+                    ev0 = ev0 + NkxF * f0;
+                    ev1 = ev1 + NkxF * f1;
+                    ev2 = ev2 + NkxF * f2;
+                }
+                acc0 = acc0 + ev0;
+                acc1 = acc1 + ev1;
+                acc2 = acc2 + ev2;
+            }
         }
         const int32_t *d = dofs + n * ndn;
         F[d[0]] = acc0;
