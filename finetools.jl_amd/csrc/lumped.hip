@@ -20,8 +20,8 @@ namespace {
 
 // det J of every quadrature point goes to the thread's column of the block's
 // LDS (jac[q * blockDim + thread]); the element matrix is then visited column
-// node by column node (b), each over the points (q) and every row node (a) at
-// once, so that each point's factor N_a N_b J w is formed once for all
+// node by column node (b), each over the points (q) and a group of row nodes
+// (a) at once, so that each point's factor N_a N_b J w is formed once for all
 // ndn x ndn components of the node pair.  Every entry still sums its points in
 // ascending order and the column sums add rows in (a, p) order, as the
 // reference's Ke += factor * c loop and sum(Ke) give them.
