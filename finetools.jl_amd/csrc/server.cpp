@@ -271,6 +271,8 @@ struct feasm_server {
                 unsigned char *st = static_cast<unsigned char *>(stage.get(((cb + 15) & ~size_t(15)) + xb + 16));
                 int32_t *sc = reinterpret_cast<int32_t *>(st);
                 double *sx = reinterpret_cast<double *>(st + ((cb + 15) & ~size_t(15)));
+                m->conn = dev_alloc(std::max<size_t>(cb, 4));
+                m->xyz = dev_alloc(std::max<size_t>(xb, 8));
                 if (r->bconn) {
                     // converted and range-checked in one branch-free pass
                     const int64_t nc = r->nelem * r->npe, nn = r->nnodes;
@@ -282,17 +284,15 @@ struct feasm_server {
                         sc[i] = (int32_t)(v - 1);
                     }
                     FEASM_REQUIRE(!bad, FEASM_EINVAL, "connectivity refers to a missing node");
-                    if (xb) std::memcpy(sx, r->bxyz, xb);
-                    r->bconn = nullptr;
-                    r->bxyz = nullptr;
                 } else {
                     if (cb) std::memcpy(sc, r->conn0.data(), cb);
-                    if (xb) std::memcpy(sx, r->xyz.data(), xb);
                 }
-                wt.convert += now_s() - t_conv;
-                m->conn = dev_alloc(std::max<size_t>(cb, 4));
-                m->xyz = dev_alloc(std::max<size_t>(xb, 8));
+                // the connectivity is copied while the coordinates are staged
                 if (cb) FEASM_HIP(hipMemcpyAsync(m->conn->p, sc, cb, hipMemcpyHostToDevice, ctx->stream));
+                if (xb) std::memcpy(sx, r->bxyz ? r->bxyz : r->xyz.data(), xb);
+                r->bconn = nullptr;
+                r->bxyz = nullptr;
+                wt.convert += now_s() - t_conv;
                 if (xb) FEASM_HIP(hipMemcpyAsync(m->xyz->p, sx, xb, hipMemcpyHostToDevice, ctx->stream));
                 m->ndn = r->ndn;
                 m->nalldofs = r->nnodes * r->ndn;
