@@ -273,27 +273,39 @@ struct feasm_server {
                 double *sx = reinterpret_cast<double *>(st + ((cb + 15) & ~size_t(15)));
                 m->conn = dev_alloc(std::max<size_t>(cb, 4));
                 m->xyz = dev_alloc(std::max<size_t>(xb, 8));
-                if (r->bconn) {
-                    // converted and range-checked in one branch-free pass
-                    const int64_t nc = r->nelem * r->npe, nn = r->nnodes;
-                    const int64_t *bc = r->bconn;
-                    int64_t bad = 0;
-                    for (int64_t i = 0; i < nc; i++) {
-                        const int64_t v = bc[i];
-                        bad |= (v < 1) | (v > nn);
-                        sc[i] = (int32_t)(v - 1);
+                // staged and copied in pieces of 2 MiB: each piece's copy runs
+                // while the next one is staged (a copy issued before the range
+                // check fails only fills the mesh's own buffer)
+                constexpr int64_t kPiece = 1 << 19;  // 4-byte entries, 2 MiB
+                const int64_t nc = r->nelem * r->npe, nn = r->nnodes;
+                int64_t bad = 0;
+                for (int64_t i0 = 0; i0 < nc; i0 += kPiece) {
+                    const int64_t i1 = std::min(nc, i0 + kPiece);
+                    if (r->bconn) {
+                        // converted and range-checked in one branch-free pass
+                        const int64_t *bc = r->bconn;
+                        for (int64_t i = i0; i < i1; i++) {
+                            const int64_t v = bc[i];
+                            bad |= (v < 1) | (v > nn);
+                            sc[i] = (int32_t)(v - 1);
+                        }
+                    } else {
+                        std::memcpy(sc + i0, r->conn0.data() + i0, (size_t)(i1 - i0) * 4);
                     }
-                    FEASM_REQUIRE(!bad, FEASM_EINVAL, "connectivity refers to a missing node");
-                } else {
-                    if (cb) std::memcpy(sc, r->conn0.data(), cb);
+                    FEASM_HIP(hipMemcpyAsync(dp<int32_t>(m->conn) + i0, sc + i0, (size_t)(i1 - i0) * 4,
+                                             hipMemcpyHostToDevice, ctx->stream));
                 }
-                // the connectivity is copied while the coordinates are staged
-                if (cb) FEASM_HIP(hipMemcpyAsync(m->conn->p, sc, cb, hipMemcpyHostToDevice, ctx->stream));
-                if (xb) std::memcpy(sx, r->bxyz ? r->bxyz : r->xyz.data(), xb);
+                FEASM_REQUIRE(!bad, FEASM_EINVAL, "connectivity refers to a missing node");
+                const double *x0 = r->bxyz ? r->bxyz : r->xyz.data();
+                for (int64_t i0 = 0; i0 < 3 * nn; i0 += kPiece / 2) {
+                    const int64_t i1 = std::min(3 * nn, i0 + kPiece / 2);
+                    std::memcpy(sx + i0, x0 + i0, (size_t)(i1 - i0) * 8);
+                    FEASM_HIP(hipMemcpyAsync(dp<double>(m->xyz) + i0, sx + i0, (size_t)(i1 - i0) * 8,
+                                             hipMemcpyHostToDevice, ctx->stream));
+                }
                 r->bconn = nullptr;
                 r->bxyz = nullptr;
                 wt.convert += now_s() - t_conv;
-                if (xb) FEASM_HIP(hipMemcpyAsync(m->xyz->p, sx, xb, hipMemcpyHostToDevice, ctx->stream));
                 m->ndn = r->ndn;
                 m->nalldofs = r->nnodes * r->ndn;
                 m->nodemajor = true;
