@@ -275,34 +275,78 @@ struct feasm_server {
                 m->xyz = dev_alloc(std::max<size_t>(xb, 8));
                 // staged and copied in pieces of 2 MiB: each piece's copy runs
                 // while the next one is staged (a copy issued before the range
-                // check fails only fills the mesh's own buffer)
+                // check fails only fills the mesh's own buffer); the pieces of
+                // a large mesh are shared out to helper threads
                 constexpr int64_t kPiece = 1 << 19;  // 4-byte entries, 2 MiB
                 const int64_t nc = r->nelem * r->npe, nn = r->nnodes;
-                int64_t bad = 0;
-                for (int64_t i0 = 0; i0 < nc; i0 += kPiece) {
-                    const int64_t i1 = std::min(nc, i0 + kPiece);
-                    if (r->bconn) {
-                        // converted and range-checked in one branch-free pass
-                        const int64_t *bc = r->bconn;
-                        for (int64_t i = i0; i < i1; i++) {
-                            const int64_t v = bc[i];
-                            bad |= (v < 1) | (v > nn);
-                            sc[i] = (int32_t)(v - 1);
+                const int64_t *bc = r->bconn;
+                const int32_t *oc = r->conn0.data();
+                const double *x0 = r->bxyz ? r->bxyz : r->xyz.data();
+                const int64_t ncp = (nc + kPiece - 1) / kPiece, nxp = (3 * nn + kPiece / 2 - 1) / (kPiece / 2);
+                int32_t *dconn = dp<int32_t>(m->conn);
+                double *dxyz = dp<double>(m->xyz);
+                hipStream_t cs = ctx->stream;
+                // piece p < ncp: connectivity (returns its range-check flag); then coordinates
+                auto piece = [&](int64_t p) -> int64_t {
+                    if (p < ncp) {
+                        const int64_t i0 = p * kPiece, i1 = std::min(nc, i0 + kPiece);
+                        int64_t bad = 0;
+                        if (bc) {
+                            // converted and range-checked in one branch-free pass
+                            for (int64_t i = i0; i < i1; i++) {
+                                const int64_t v = bc[i];
+                                bad |= (v < 1) | (v > nn);
+                                sc[i] = (int32_t)(v - 1);
+                            }
+                        } else {
+                            std::memcpy(sc + i0, oc + i0, (size_t)(i1 - i0) * 4);
                         }
-                    } else {
-                        std::memcpy(sc + i0, r->conn0.data() + i0, (size_t)(i1 - i0) * 4);
+                        FEASM_HIP(hipMemcpyAsync(dconn + i0, sc + i0, (size_t)(i1 - i0) * 4, hipMemcpyHostToDevice, cs));
+                        return bad;
                     }
-                    FEASM_HIP(hipMemcpyAsync(dp<int32_t>(m->conn) + i0, sc + i0, (size_t)(i1 - i0) * 4,
-                                             hipMemcpyHostToDevice, ctx->stream));
+                    const int64_t i0 = (p - ncp) * (kPiece / 2), i1 = std::min(3 * nn, i0 + kPiece / 2);
+                    std::memcpy(sx + i0, x0 + i0, (size_t)(i1 - i0) * 8);
+                    FEASM_HIP(hipMemcpyAsync(dxyz + i0, sx + i0, (size_t)(i1 - i0) * 8, hipMemcpyHostToDevice, cs));
+                    return 0;
+                };
+                const int64_t np = ncp + nxp;
+                const int nh = np >= 8 ? 3 : 0;  // helper threads: from 16 MiB on
+                int64_t bad = 0;
+                if (nh == 0) {
+                    for (int64_t p = 0; p < np; p++) bad |= piece(p);
+                } else {
+                    // contiguous ranges of pieces, one per thread (the worker takes the first)
+                    int64_t hbad[4] = {0, 0, 0, 0};
+                    int herr[4] = {0, 0, 0, 0};
+                    std::string hmsg[4];
+                    const int dev = device;
+                    auto run = [&](int t) {
+                        try {
+                            if (t > 0) FEASM_HIP(hipSetDevice(dev));
+                            const int64_t p0 = np * t / (nh + 1), p1 = np * (t + 1) / (nh + 1);
+                            for (int64_t p = p0; p < p1; p++) hbad[t] |= piece(p);
+                        } catch (const feasm::Error &e) {
+                            herr[t] = e.code;
+                            hmsg[t] = e.what();
+                        } catch (const std::exception &e) {
+                            herr[t] = FEASM_EINVAL;
+                            hmsg[t] = e.what();
+                        }
+                    };
+                    std::vector<std::thread> th;
+                    try {
+                        for (int t = 1; t <= nh; t++) th.emplace_back(run, t);
+                    } catch (...) {  // no thread: the worker stages the helpers' ranges as well
+                    }
+                    run(0);
+                    for (int t = (int)th.size() + 1; t <= nh; t++) run(t);
+                    for (auto &x : th) x.join();
+                    for (int t = 0; t <= nh; t++) {
+                        if (herr[t]) throw Error(herr[t], hmsg[t]);
+                        bad |= hbad[t];
+                    }
                 }
                 FEASM_REQUIRE(!bad, FEASM_EINVAL, "connectivity refers to a missing node");
-                const double *x0 = r->bxyz ? r->bxyz : r->xyz.data();
-                for (int64_t i0 = 0; i0 < 3 * nn; i0 += kPiece / 2) {
-                    const int64_t i1 = std::min(3 * nn, i0 + kPiece / 2);
-                    std::memcpy(sx + i0, x0 + i0, (size_t)(i1 - i0) * 8);
-                    FEASM_HIP(hipMemcpyAsync(dp<double>(m->xyz) + i0, sx + i0, (size_t)(i1 - i0) * 8,
-                                             hipMemcpyHostToDevice, ctx->stream));
-                }
                 r->bconn = nullptr;
                 r->bxyz = nullptr;
                 wt.convert += now_s() - t_conv;
