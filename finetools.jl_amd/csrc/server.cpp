@@ -177,6 +177,27 @@ struct feasm_server {
         ~Stage() { put(); }
     };
 
+    // Helper threads that stage large uploads, counted process-wide so that
+    // busy workers do not start more threads than the host has cores for.
+    static std::atomic<int> &helpers_busy() {
+        static std::atomic<int> n{0};
+        return n;
+    }
+    static int reserve_helpers(int want) {
+        static const int cap = std::max(2, (int)std::thread::hardware_concurrency() / 2);
+        std::atomic<int> &busy = helpers_busy();
+        int got = 0;
+        while (got < want) {
+            int cur = busy.load();
+            if (cur >= cap) break;
+            if (busy.compare_exchange_weak(cur, cur + 1)) got++;
+        }
+        return got;
+    }
+    static void release_helpers(int n) {
+        if (n) helpers_busy() -= n;
+    }
+
     void work(int w) {
         feasm_ctx *ctx = ctxs[w];
         (void)hipSetDevice(device);
@@ -310,7 +331,13 @@ struct feasm_server {
                     return 0;
                 };
                 const int64_t np = ncp + nxp;
-                const int nh = np >= 8 ? 3 : 0;  // helper threads: from 16 MiB on
+                // up to three helper threads from 16 MiB on, as many as the
+                // process-wide helper budget has free (half the host's cores)
+                const int nh = np >= 8 ? reserve_helpers(3) : 0;
+                struct HelperRelease {
+                    int n;
+                    ~HelperRelease() { release_helpers(n); }
+                } helper_release{nh};
                 int64_t bad = 0;
                 if (nh == 0) {
                     for (int64_t p = 0; p < np; p++) bad |= piece(p);
